@@ -1,0 +1,350 @@
+"""Pins for the CPU oracle (oracle/oracle.c).
+
+Each test fixes an oracle function with a check that a plausible mistake
+(dropped term, wrong sign or index, transposed operand, wrong factor) fails:
+closed forms, known answers computed by hand, brute force in numpy, and
+invariants.  The reference holds no tests or golden vectors for this path
+(its only data file on the path family is the van Hoof Gaunt table), so these
+are the pins.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+def w2d_ref_closed(r, h):
+    """The reference's projected quartic in closed form: 7/(pi h^2)(1-q^2)^2.5
+    (its float constants 2*1.04445*1.066667 = 2.228161 ~ 7/pi)."""
+    q2 = (r / h) ** 2
+    return 2.0 * 1.04445 * 1.066667 / (h * h) * max(1.0 - q2, 0.0) ** 2.5
+
+
+# ---------------------------------------------------------------- camera
+
+def test_camera_look_z_closed_form(orc):
+    cam = orc.Camera([10.0, 20.0, -5.0], [0.0, 0.0, 1.0], 0.0, 100.0, 8, 4)
+    assert np.allclose(cam.b0, [1, 0, 0]) and np.allclose(cam.b1, [0, 1, 0])
+    assert cam.pixel_size == pytest.approx(12.5)
+    # lower-left corner: pos - (nx/2) ps b0 - (ny/2) ps b1
+    assert np.allclose(cam.min_position, [10 - 4 * 12.5, 20 - 2 * 12.5, -5.0])
+
+
+def test_camera_random_orthonormal_and_roll(orc):
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        la = rng.normal(size=3)
+        la /= np.linalg.norm(la)
+        c0 = orc.Camera([0, 0, 0], la, 0.0, 1.0, 2, 2)
+        a = rng.uniform(0, 2 * math.pi)
+        c1 = orc.Camera([0, 0, 0], la, a, 1.0, 2, 2)
+        for c in (c0, c1):
+            assert abs(np.dot(c.b0, c.b1)) < 1e-12
+            assert abs(np.linalg.norm(c.b0) - 1) < 1e-12
+            assert np.allclose(np.cross(c.b0, c.b1), la, atol=1e-12)
+        # roll by a about look_at: b0(a) = cos a b0 + sin a b1
+        assert np.allclose(c1.b0, math.cos(a) * c0.b0 + math.sin(a) * c0.b1, atol=1e-12)
+
+
+def test_projection_matrix_is_transpose_for_orthonormal(orc):
+    rng = np.random.default_rng(6)
+    for _ in range(10):
+        la = rng.normal(size=3)
+        la /= np.linalg.norm(la)
+        cam = orc.Camera([0, 0, 0], la, rng.uniform(0, 6), 1.0, 2, 2)
+        m = cam.projection_matrix().reshape(3, 3).astype(np.float64)
+        M = np.stack([cam.b0, cam.b1, la], axis=1)
+        assert np.allclose(m, M.T, atol=2e-6)
+
+
+def test_project_particles_depth_mask(orc):
+    cam = orc.Camera([0.0, 0.0, 0.0], [0.0, 0.0, 1.0], 0.0, 10.0, 2, 2)
+    p = np.array([[1, 2, -1, 5], [1, 2, 0, 5], [1, 2, 50, 5], [1, 2, 100, 5],
+                  [1, 2, 100.5, 5], [3, -4, 20, 7]], np.float32)
+    out = cam.project_particles(p, 100.0)
+    assert np.array_equal(out[:, 3], [0, 5, 5, 5, 0, 7])
+    assert np.all(out[:, 2] == 0)
+    assert np.array_equal(out[:, :2], p[:, :2])
+    # rotated camera: camera-frame coords are (b0, b1) components of x - pos
+    la = np.array([1.0, 1.0, 0.0]) / math.sqrt(2)
+    cam = orc.Camera([0.0, 0.0, 0.0], la, 0.3, 10.0, 2, 2)
+    x = np.array([[3.0, 1.0, 2.0, 1.0]], np.float32)
+    out = cam.project_particles(x, 100.0)
+    exp = [np.dot(x[0, :3], cam.b0), np.dot(x[0, :3], cam.b1)]
+    assert np.allclose(out[0, :2], exp, atol=1e-5)
+
+
+def test_pixel_points_are_corners(orc):
+    cam = orc.Camera([100.0, 200.0, 0.0], [0.0, 0.0, 1.0], 0.0, 64.0, 8, 4)
+    pts = cam.pixel_points()
+    ps = 8.0
+    ix, iy = np.meshgrid(np.arange(8), np.arange(4))
+    # corner convention: (ix, iy) -> min + ix ps b0 + iy ps b1; camera frame
+    # origin at the screen centre -> x = -W/2 + ix ps, y = -(ny/2) ps + iy ps
+    assert np.array_equal(pts[:, 0], (-32.0 + ps * ix).reshape(-1).astype(np.float32))
+    assert np.array_equal(pts[:, 1], (-16.0 + ps * iy).reshape(-1).astype(np.float32))
+
+
+# ---------------------------------------------------------------- grid
+
+def test_grid_params_hand_computed(orc):
+    p = np.array([[0, 0, 0, 1], [10, 20, 0, 1]], np.float32)
+    g = orc.Grid(p, np.ones(2, np.float32), target=1)
+    ext = np.array([10.2, 20.2, 0.2])
+    w = (1.0 / 2.0 * (ext[0] * ext[1] * ext[2])) ** (1.0 / 3.0)
+    assert list(g.nc) == [math.ceil(10.2 / w), math.ceil(20.2 / w), 1] == [4, 8, 1]
+    assert g.cs[0] == np.float32(w)
+    assert np.array_equal(g.minc, np.float32([-0.1, -0.1, -0.1]))
+
+
+def _random_scene(n, seed, box=100.0, z0=True):
+    rng = np.random.default_rng(seed)
+    p = np.zeros((n, 4), np.float32)
+    p[:, :2] = rng.uniform(-box / 2, box / 2, (n, 2))
+    if not z0:
+        p[:, 2] = rng.uniform(-box / 2, box / 2, n)
+    p[:, 3] = rng.lognormal(0, 0.5, n)
+    # clustered half with duplicates to create multi-particle cells
+    p[: n // 4, :2] = rng.normal(0, 2.0, (n // 4, 2))
+    p[n // 4: n // 4 + 8, :2] = p[0, :2]
+    h = rng.uniform(0.5, 6.0, n).astype(np.float32)
+    return p, h
+
+
+@pytest.mark.parametrize("z0", [True, False])
+def test_grid_build_against_numpy(orc, z0):
+    p, h = _random_scene(3000, 1, z0=z0)
+    g = orc.Grid(p, h, target=20)
+    # keys: float32 floor((p - min)/w), key = x + y nx + z nx ny
+    idx = np.floor((p[:, :3] - g.minc) / g.cs).astype(np.int64)
+    nx, ny, nz = (int(v) for v in g.nc)
+    keys = idx[:, 0] + idx[:, 1] * nx + idx[:, 2] * nx * ny
+    assert np.array_equal(g.keys(p).astype(np.int64), keys)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(g.perm.astype(np.int64), order)
+    assert np.array_equal(g.sorted_p4, p[order])
+    assert np.array_equal(g.sorted_h, h[order])
+    cnt = np.bincount(keys, minlength=nx * ny * nz)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    exp_cells = np.where(cnt[:, None] > 0, np.stack([start, start + cnt], 1), 0)
+    assert np.array_equal(g.cells, exp_cells)
+    mx = np.zeros(nx * ny * nz, np.float32)
+    np.maximum.at(mx, keys, h)
+    assert np.array_equal(g.maxh_cell, mx)
+    assert g.max_h == h.max()
+
+
+def test_grid_key_out_of_range_is_an_error(orc):
+    p = np.array([[0, 0, 0, 1], [1, 1, 0, 1]], np.float32)
+    with pytest.raises(ValueError):
+        orc.Grid(p, np.ones(2, np.float32),
+                 params=(np.array([1, 1, 1], np.int32), np.zeros(3, np.float32),
+                         np.full(3, 0.5, np.float32)))
+
+
+# ---------------------------------------------------------------- kernel
+
+def test_quartic_closed_forms(orc):
+    for h in (0.3, 1.0, 7.5, 2500.0):
+        # 7/(pi h^2) up to the reference's rounded constants (1.04445, 1.066667)
+        assert orc.quartic_projection(0.0, h) == pytest.approx(7 / (math.pi * h * h), rel=1e-5)
+        assert orc.quartic_projection(0.0, h) == pytest.approx(
+            2 * 1.04445 * 1.066667 / (h * h), rel=1e-6)
+        assert orc.quartic_projection(h, h) == 0.0
+        assert orc.quartic_projection(1.5 * h, h) == 0.0
+        for q in (0.1, 0.5, 0.9):
+            assert orc.quartic_projection(q * h, h) == pytest.approx(w2d_ref_closed(q * h, h),
+                                                                     rel=2e-5)
+
+
+@pytest.mark.parametrize("kind,integral", [(0, 2.0), (1, 1.0), (2, 1.0)])
+def test_kernel_plane_integral(orc, kind, integral):
+    h = 3.0
+    n = 241
+    xs = np.linspace(-7.2, 7.2, n)
+    X, Y = np.meshgrid(xs, xs)
+    pts = np.zeros((n * n, 4), np.float32)
+    pts[:, 0] = X.reshape(-1)
+    pts[:, 1] = Y.reshape(-1)
+    p = np.array([[0.013, -0.021, 0, 1.0]], np.float32)
+    v = orc.gather_exact(p, np.array([h], np.float32), pts, kind)
+    dA = (xs[1] - xs[0]) ** 2
+    assert v.sum() * dA == pytest.approx(integral, rel=2e-4)
+
+
+def test_cubic_projected_closed_forms(orc):
+    assert orc.cubic_projected_unit(0.0) == pytest.approx(1.5 / math.pi, rel=1e-12)
+    assert orc.cubic_projected_unit(2.0) == 0.0
+    # b in (1,2): only the outer piece, closed form 1/(4pi) * 2 int (2-q)^3 dz
+    b = 1.5
+    zs = np.linspace(0, math.sqrt(4 - b * b), 200001)
+    q = np.sqrt(b * b + zs * zs)
+    f = 0.25 * (2 - q) ** 3 / math.pi
+    ref = 2 * np.trapezoid(f, zs)
+    assert orc.cubic_projected_unit(b) == pytest.approx(ref, rel=1e-8)
+
+
+# ---------------------------------------------------------------- gather
+
+def test_gather_ref_uniform_h_equals_exact(orc):
+    """With one smoothing length everywhere the F3 radius (own cell's max h
+    + cell radius) is conservative for points in non-empty cells, so the
+    reference gather includes every particle within h there.  In an empty
+    cell the radius is the cell radius alone: particles between it and h
+    are dropped."""
+    rng = np.random.default_rng(2)
+    n = 2000
+    p = np.zeros((n, 4), np.float32)
+    p[:, :2] = rng.uniform(-50, 50, (n, 2))
+    p[:, 3] = rng.lognormal(0, 0.5, n)
+    p[::7, 3] = 0.0  # depth-masked particles contribute nothing
+    h = np.full(n, 6.0, np.float32)
+    g = orc.Grid(p, h, target=100)
+    pts = np.zeros((800, 4), np.float32)
+    pts[:, :2] = rng.uniform(-48, 48, (800, 2))
+    ref = g.gather_ref(pts, acc_mode=1)
+    ex = orc.gather_exact(p, h, pts, 0)
+    own = g.maxh_cell[g.keys(pts).astype(np.int64)]
+    full = own > 0
+    assert full.sum() > 400 and (~full).sum() >= 5
+    m = full & (ex > 1e-6 * ex.max())
+    assert np.max(np.abs(ref[m] - ex[m]) / ex[m]) < 5e-6
+    assert np.all(ref[ex == 0] == 0)
+    # empty own cell: radius = cell radius < h -> never more than exact
+    radius = 0.5 * math.sqrt(3) * float(g.cs[0])
+    assert radius < 6.0
+    e = ~full
+    assert np.all(ref[e] <= ex[e] * (1 + 1e-6))
+    assert np.any(ref[e] < ex[e] * (1 - 1e-3))
+
+
+def test_gather_ref_f3_drop_known_answer(orc):
+    """F3: the search radius comes from the max h of the point's own cell.
+    A: small-h particle in the point's cell; B: big-h particle 6 units away
+    (inside h_B) but beyond max_h[cell(P)] + cell_radius -> dropped for P.
+    For Q, in B's cell, the radius is h_B-based, so both A and B count."""
+    p = np.array([[0.0, 0.0, 0, 1.0],    # A
+                  [6.0, 0.0, 0, 2.0],    # B
+                  [20.0, 20.0, 0, 1.0],  # C (bbox)
+                  [-20.0, -20.0, 0, 1.0]], np.float32)  # D (bbox)
+    h = np.array([0.5, 8.0, 0.5, 0.5], np.float32)
+    g = orc.Grid(p, h, params=(np.array([41, 41, 1], np.int32),
+                               np.array([-20.1, -20.1, -0.1], np.float32),
+                               np.ones(3, np.float32)))
+    # P in A's cell (20,20): R = 0.5 + sqrt(3)/2 -> x cells [18, 21]; B is in 26
+    # Q in B's cell (26,20): R = min(8 + 0.866, 8) = 8 -> A's cell included
+    P = np.array([[0.2, 0.1, 0, 0], [6.2, 0.2, 0, 0]], np.float32)
+    got = g.gather_ref(P, acc_mode=1)
+    rA = math.hypot(0.2, 0.1)
+    assert got[0] == pytest.approx(1.0 * w2d_ref_closed(rA, 0.5), rel=1e-5)
+    rBQ = math.hypot(0.2, 0.2)
+    assert got[1] == pytest.approx(2.0 * w2d_ref_closed(rBQ, 8.0), rel=1e-5)
+    # the exact oracle keeps B for P
+    ex = orc.gather_exact(p, h, P, 0)
+    rBP = math.hypot(5.8, 0.1)
+    assert ex[0] == pytest.approx(w2d_ref_closed(rA, 0.5) + 2.0 * w2d_ref_closed(rBP, 8.0),
+                                  rel=1e-6)
+
+
+def test_gather_ref_f4_alias_known_answer(orc):
+    """F4: a point right of the grid (x index = nx) has an unclamped key
+    that aliases cell (0, y+1); its radius then comes from that cell's max h
+    instead of the overall max."""
+    # grid with nx = ny = 2 cells of width ~10 (target chosen accordingly)
+    p = np.array([[1.0, 1.0, 0, 1.0],      # E in cell (0,0), small h
+                  [1.0, 11.0, 0, 1.0],     # F in cell (0,1), h 1 -> aliased radius
+                  [19.0, 19.0, 0, 1.0],    # G cell (1,1), big h (overall max)
+                  [19.0, 1.0, 0, 1.0]], np.float32)  # H cell (1,0)
+    h = np.array([0.5, 1.0, 30.0, 0.5], np.float32)
+    nc = np.array([2, 2, 1], np.int32)
+    minc = np.array([0.0, 0.0, -0.1], np.float32)
+    cs = np.array([10.0, 10.0, 10.0], np.float32)
+    g = orc.Grid(p, h, params=(nc, minc, cs))
+    # Q at x index 2 (= nx), y index 0 -> key 2 = cell (0,1) -> R = 1 + 8.66
+    Q = np.array([[21.0, 5.0, 0, 0]], np.float32)
+    got = g.gather_ref(Q, acc_mode=1)
+    # index box [floor((21-9.66)/10), floor((21+9.66)/10)] = [1, 3] in x,
+    # [floor((5-9.66)/10), floor(14.66/10)] = [-1, 1] in y -> cells (1,0),(1,1)
+    # both within the cull; G (r = hypot(2, 14) < 30) counts, H (r > 0.5) 0
+    rG = math.hypot(2.0, 14.0)
+    assert got[0] == pytest.approx(w2d_ref_closed(rG, 30.0), rel=1e-5)
+    # a point whose key is negative (below the grid) uses the overall max:
+    # index box covers every cell -> equals the exact sum
+    Qn = np.array([[5.0, -3.0, 0, 0]], np.float32)
+    got = g.gather_ref(Qn, acc_mode=1)
+    ex = orc.gather_exact(p, h, Qn, 0)
+    assert got[0] == pytest.approx(ex[0], rel=1e-5)
+
+
+def test_gather_ref_fp32_accumulator_is_reference_order(orc):
+    """acc_mode 0 accumulates in FP32: its difference from acc_mode 1 is
+    bounded by n * 2^-24 relative and is nonzero for long sums."""
+    p, h = _random_scene(4000, 3)
+    g = orc.Grid(p, h, target=40)
+    pts = np.zeros((300, 4), np.float32)
+    pts[:, :2] = np.random.default_rng(4).normal(0, 3, (300, 2))
+    r32, pairs = g.gather_ref(pts, acc_mode=0, with_pairs=True)
+    r64 = g.gather_ref(pts, acc_mode=1)
+    m = r64 > 0
+    rel = np.abs(r32 - r64)[m] / r64[m]
+    assert np.all(rel <= pairs[m] * 2.0 ** -24 + 1e-12)
+    assert np.any(rel > 0)
+    assert np.all(r32.astype(np.float32) == r32)
+
+
+def test_exact_counts_and_lattice_count(orc):
+    p, h = _random_scene(1500, 7)
+    p[::5, 3] = 0
+    cam = orc.Camera([0.0, 0.0, -10.0], [0.0, 0.0, 1.0], 0.0, 60.0, 48, 40)
+    pts = cam.pixel_points()
+    _, cnt = orc.gather_exact(p, h, pts, 0, with_counts=True)
+    # brute force in numpy
+    d2 = ((pts[:, None, 0].astype(np.float64) - p[None, :, 0]) ** 2
+          + (pts[:, None, 1].astype(np.float64) - p[None, :, 1]) ** 2)
+    bf = ((d2 < (h.astype(np.float64) ** 2)[None, :]) & (p[None, :, 3] != 0)).sum()
+    assert cnt.sum() == bf
+    assert orc.count_evals_lattice(p, h, pts, 48, 40) == bf
+    _, cnt2 = orc.gather_exact(p, h, pts, 2, with_counts=True)
+    assert orc.count_evals_lattice(p, h, pts, 48, 40, 2.0) == cnt2.sum()
+
+
+# ---------------------------------------------------------------- quantities
+
+def test_tabulated_1d_half_texel_and_border(orc):
+    y = [1.0, 2.0, 4.0]
+    cases = {0.0: 0.5, 0.5: 1.0, 1.0: 1.5, 1.5: 2.0, 2.5: 4.0, 3.0: 2.0, 3.5: 0.0,
+             -0.5: 0.0, 0.25: 0.75, 1.75: 2.5}
+    for x, v in cases.items():
+        assert orc.tabulated_1d(y, 0.0, 1.0, x) == pytest.approx(v, abs=1e-7), x
+    # x_min / dx scaling
+    assert orc.tabulated_1d(y, 10.0, 0.5, 10.25) == pytest.approx(1.0)
+
+
+def test_tabulated_2d_half_texel(orc):
+    data = np.array([[1.0, 2.0], [3.0, 5.0]], np.float32)   # (ny, nx)
+    assert orc.tabulated_2d(data, 0, 0, 1, 1, 0.5, 0.5) == pytest.approx(1.0)
+    assert orc.tabulated_2d(data, 0, 0, 1, 1, 1.5, 1.5) == pytest.approx(5.0)
+    assert orc.tabulated_2d(data, 0, 0, 1, 1, 1.0, 0.5) == pytest.approx(1.5)
+    assert orc.tabulated_2d(data, 0, 0, 1, 1, 0.5, 1.0) == pytest.approx(2.0)
+    assert orc.tabulated_2d(data, 0, 0, 1, 1, 1.0, 1.0) == pytest.approx(11 / 4)
+    assert orc.tabulated_2d(data, 0, 0, 1, 1, 0.0, 0.5) == pytest.approx(0.5)
+
+
+def test_temperature_closed_form(orc):
+    for u, xe in ((1e4, 1.1), (350.0, 1.0), (2e6, 1.2)):
+        mu = 4.0 / (1 + 3 * 0.76 + 4 * 0.76 * xe)
+        assert orc.temperature(u, xe) == pytest.approx(2.0 / 3.0 * u * mu * 121.1475502,
+                                                       rel=1e-6)
+
+
+def test_xray_bolometric_closed_form(orc):
+    gy = np.linspace(1.0, 2.0, 11).astype(np.float32)
+    x0, dx = np.float32(math.log(1e4)), np.float32(0.5)
+    # T at a texel centre (u = i + 0.5) reads gy[i] exactly
+    i = 4
+    T = np.float32(math.exp(float(x0) + (i + 0.5) * float(dx)))
+    rho = np.float32(2e-6)
+    xe = np.float32(1.1)
+    w = orc.xray_bolometric([rho], [T], [xe], gy, x0, dx)[0]
+    exp = (4.046263 * 2e-6 * 1.1) ** 2 * math.sqrt(float(T)) * float(gy[i])
+    assert w == pytest.approx(exp, rel=3e-5)
