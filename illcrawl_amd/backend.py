@@ -1,0 +1,239 @@
+"""Host-side mirror of illcrawl's projective smoothing backend interface.
+
+``ProjectiveSmoothingGrid`` has the method set of
+``reconstruction_backends::dm::projective_smoothing_grid``
+(reconstruction_backend.hpp:35-58, projective_smoothing_backend.hpp:35-42) and
+``ProjectiveSmoothing`` the one of the camera wrapper
+``reconstruction_backends::dm::projective_smoothing``
+(projective_smoothing_reconstruction.hpp:33-89).  Buffers are torch CUDA
+tensors (device memory and streams only); all work runs in the HIP library
+through the C ABI.  Errors raise ``IllcrawlError`` (a RuntimeError, like the
+reference's qcl_error / std::runtime_error).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+_MODES = {"reference": N.MODE_REFERENCE, "exact": N.MODE_EXACT}
+_KERNELS = {"quartic_ref": N.KERNEL_QUARTIC_REF, "quartic": N.KERNEL_QUARTIC,
+            "cubic": N.KERNEL_CUBIC}
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev_f32(x, device, shape_last=None):
+    if isinstance(x, np.ndarray):
+        x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32))
+    t = x.to(device=device, dtype=torch.float32).contiguous()
+    if shape_last is not None and (t.dim() != 2 or t.shape[1] != shape_last):
+        raise ValueError(f"expected an (n, {shape_last}) array, got {tuple(t.shape)}")
+    return t
+
+
+class ProjectiveSmoothingGrid:
+    """dm::projective_smoothing_grid on the HIP path (target 1000 particles
+    per cell, projective_smoothing_backend_grid.cpp:27-31)."""
+
+    def __init__(self, device=0, target_particles_per_cell=1000, mode="reference",
+                 kernel="quartic_ref", tile=None):
+        cfg = N.PsgConfig()
+        cfg.device = int(device)
+        cfg.target_particles_per_cell = int(target_particles_per_cell)
+        cfg.mode = _MODES[mode]
+        cfg.kernel = _KERNELS[kernel]
+        if tile is not None:
+            cfg.tile_w, cfg.tile_h = int(tile[0]), int(tile[1])
+        self._h = ctypes.c_void_p()
+        N.call("ilc_psg_create", ctypes.byref(cfg), ctypes.byref(self._h))
+        self.device = torch.device("cuda", int(device))
+        self.mode, self.kernel = mode, kernel
+        self._keep = []       # borrowed buffers kept alive while in use
+        self._npoints = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                N.lib().ilc_psg_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # reconstruction_backend interface ---------------------------------
+    def get_backend_name(self):
+        return N.lib().ilc_psg_backend_name(self._h).decode()
+
+    def init_backend(self, blocksize):
+        N.call("ilc_psg_init_backend", self._h, int(blocksize))
+
+    def setup_projected_particles(self, particles4, smoothing_lengths, stream=None):
+        p = _dev_f32(particles4, self.device, 4)
+        h = _dev_f32(smoothing_lengths, self.device).reshape(-1)
+        if h.shape[0] != p.shape[0]:
+            raise ValueError("one smoothing length per particle is required")
+        N.call("ilc_psg_setup_projected_particles", self._h, _ptr(p), p.shape[0], _ptr(h),
+               _stream(stream))
+
+    def setup_particles(self, particles4, additional_datasets, stream=None):
+        """reconstruction_backend::setup_particles: the one additional dataset
+        of this backend is the smoothing length."""
+        if len(additional_datasets) != 1:
+            raise ValueError("the grid backend takes exactly one additional dataset (h)")
+        self.setup_projected_particles(particles4, additional_datasets[0], stream)
+
+    def setup_evaluation_points(self, points4, layout_nx=0, stream=None):
+        pts = _dev_f32(points4, self.device, 4)
+        self._keep = [pts]
+        self._npoints = pts.shape[0]
+        N.call("ilc_psg_setup_evaluation_points", self._h, _ptr(pts), pts.shape[0],
+               int(layout_nx), _stream(stream))
+
+    def run(self, stream=None):
+        N.call("ilc_psg_run", self._h, _stream(stream))
+
+    def retrieve_results(self, scale=1.0, stream=None):
+        out = torch.empty(self._npoints, dtype=torch.float32, device=self.device)
+        N.call("ilc_psg_retrieve_results", self._h, ctypes.c_float(scale), _ptr(out),
+               _stream(stream))
+        return out
+
+    def retrieve_results_f64(self, stream=None):
+        out = torch.empty(self._npoints, dtype=torch.float64, device=self.device)
+        N.call("ilc_psg_retrieve_results_f64", self._h, _ptr(out), _stream(stream))
+        return out
+
+    # inspection ------------------------------------------------------------
+    def grid_info(self):
+        gi = N.GridInfo()
+        N.call("ilc_psg_grid_info", self._h, ctypes.byref(gi))
+        return dict(num_cells=list(gi.num_cells), min_corner=list(gi.min_corner),
+                    cell_size=list(gi.cell_size), max_h=gi.max_h,
+                    num_particles=gi.num_particles)
+
+    def export_grid(self, stream=None):
+        gi = self.grid_info()
+        ncells = int(np.prod(gi["num_cells"]))
+        cells = torch.empty((ncells, 2), dtype=torch.int32, device=self.device)
+        maxh = torch.empty(ncells, dtype=torch.float32, device=self.device)
+        perm = torch.empty(gi["num_particles"], dtype=torch.int32, device=self.device)
+        N.call("ilc_psg_export_grid", self._h, _ptr(cells), _ptr(maxh), _ptr(perm),
+               _stream(stream))
+        return cells, maxh, perm
+
+    def count_evals(self, per_point=False, stream=None):
+        total = ctypes.c_uint64()
+        pp = torch.empty(self._npoints, dtype=torch.int64, device=self.device) \
+            if per_point else None
+        N.call("ilc_psg_count_evals", self._h, _ptr(pp), ctypes.byref(total), _stream(stream))
+        return (total.value, pp) if per_point else total.value
+
+
+# ------------------------------------------------------------ camera wrapper
+
+def camera_basis(cam):
+    c = N.camera_struct(cam)
+    b0, b1, mp = (ctypes.c_double * 3)(), (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+    ps = ctypes.c_double()
+    N.call("ilc_camera_basis", ctypes.byref(c), ctypes.byref(b0), ctypes.byref(b1),
+           ctypes.byref(mp), ctypes.byref(ps))
+    return np.array(b0[:]), np.array(b1[:]), np.array(mp[:]), ps.value
+
+
+def camera_pixel_points(cam, device=0, stream=None):
+    """camera::generate_pixel_coordinates + project_evaluation_points."""
+    c = N.camera_struct(cam)
+    out = torch.empty((c.nx * c.ny, 4), dtype=torch.float32, device=torch.device("cuda", device))
+    N.call("ilc_camera_pixel_points", ctypes.byref(c), _ptr(out), _stream(stream))
+    return out
+
+
+def project_particles(cam, particles4, device=0, stream=None, out=None):
+    """projective_smoothing::project_particles (depth mask, z := 0)."""
+    c = N.camera_struct(cam)
+    p = _dev_f32(particles4, torch.device("cuda", device), 4)
+    if out is None:
+        out = torch.empty_like(p)
+    N.call("ilc_project_particles", ctypes.byref(c), _ptr(p), p.shape[0], _ptr(out),
+           _stream(stream))
+    return out
+
+
+def quantity_scaled(inp, scale, n=None, device=0, stream=None):
+    """quantity_transformation with unprocessed_quantity (inp given) or
+    constant_quantity (inp None): w = float(scale * inp)."""
+    dev = torch.device("cuda", device)
+    t = _dev_f32(inp, dev).reshape(-1) if inp is not None else None
+    n = t.shape[0] if t is not None else int(n)
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    N.call("ilc_quantity_scaled", _ptr(t), ctypes.c_double(scale), n, _ptr(out), _stream(stream))
+    return out
+
+
+def quantity_xray_bolometric(rho, temp, xe, table, x0, dx, device=0, stream=None):
+    dev = torch.device("cuda", device)
+    r = _dev_f32(rho, dev).reshape(-1)
+    t = _dev_f32(temp, dev).reshape(-1)
+    x = _dev_f32(xe, dev).reshape(-1)
+    tab = np.ascontiguousarray(table, dtype=np.float32)
+    out = torch.empty_like(r)
+    N.call("ilc_quantity_xray_bolometric", _ptr(r), _ptr(t), _ptr(x), r.shape[0],
+           tab.ctypes.data_as(ctypes.c_void_p), tab.shape[0], ctypes.c_float(x0),
+           ctypes.c_float(dx), _ptr(out), _stream(stream))
+    return out
+
+
+class ProjectiveSmoothing:
+    """dm::projective_smoothing: projects particles and pixels into the
+    camera frame, forwards to the grid backend, and applies the quantity's
+    effective dA at retrieval (projective_smoothing_reconstruction.cpp:49-142)."""
+
+    def __init__(self, camera, max_integration_depth, effective_dA, backend):
+        self.camera = dict(camera)
+        self.depth = float(max_integration_depth)
+        self.effective_dA = effective_dA   # callable(pixel_area, depth) -> scale, or a float
+        self.backend = backend
+
+    def get_backend_name(self):
+        return "projective_smoothing via " + self.backend.get_backend_name()
+
+    def init_backend(self, blocksize):
+        self.backend.init_backend(blocksize)
+
+    def set_camera(self, cam):
+        self.camera = dict(cam)
+
+    def set_integration_depth(self, depth):
+        self.depth = float(depth)
+
+    def setup_particles(self, particles4, additional_datasets, stream=None):
+        cam = dict(self.camera, depth=self.depth)
+        proj = project_particles(cam, particles4, self.backend.device.index, stream)
+        self.backend.setup_projected_particles(proj, additional_datasets[0], stream)
+
+    def setup_evaluation_points(self, stream=None):
+        pts = camera_pixel_points(self.camera, self.backend.device.index, stream)
+        self.backend.setup_evaluation_points(pts, layout_nx=int(self.camera["nx"]),
+                                             stream=stream)
+
+    def run(self, stream=None):
+        self.backend.run(stream)
+
+    def retrieve_results(self, stream=None):
+        ps = float(self.camera["width"]) / float(self.camera["nx"])
+        scale = self.effective_dA(ps * ps, self.depth) if callable(self.effective_dA) \
+            else float(self.effective_dA)
+        img = self.backend.retrieve_results(float(np.float32(scale)), stream)
+        return img.reshape(int(self.camera.get("ny", self.camera["nx"])),
+                           int(self.camera["nx"]))

@@ -1,0 +1,442 @@
+// C ABI of the projective smoothing path (declared in include/illcrawl_psg.h).
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace ilc {
+const char* last_error_cstr();
+std::vector<float> cubic_projected_table(int n);
+}  // namespace ilc
+
+using namespace ilc;
+
+struct ilc_psg {
+  int device = 0;
+  uint64_t target = 1000;
+  int mode = ILC_MODE_REFERENCE;
+  int kernel = ILC_KERNEL_QUARTIC_REF;
+  tile_shape ts{32, 4};
+  uint64_t blocksize = 0;
+
+  // grid
+  bool have_grid = false;
+  grid_params gp{};
+  float max_h = 0.f, cell_radius = 0.f;
+  int32_t kz0 = 0;
+  uint64_t n = 0;
+  dbuf<float4> part;
+  dbuf<float> hs;
+  dbuf<uint32_t> keys, vals, start, counts, maxh_bits, temp, flags;
+  dbuf<float> bbox_partial;
+  dbuf<bbox_result> bbox_dev;
+  bbox_result* bbox_host = nullptr;
+  uint32_t* flags_host = nullptr;
+
+  // points
+  const float4* pts = nullptr;
+  uint64_t npts = 0;
+  uint32_t lnx = 0, lny = 0;
+  dbuf<double> acc;
+
+  dbuf<float> cubic;
+  bool cubic_ready = false;
+  dbuf<unsigned long long> cnt, cnt_scratch;
+
+  ~ilc_psg() {
+    if (bbox_host) (void)hipHostFree(bbox_host);
+    if (flags_host) (void)hipHostFree(flags_host);
+  }
+};
+
+namespace {
+
+template <class F>
+ilc_status guarded(F&& f) {
+  try {
+    f();
+    set_last_error("");
+    return ILC_OK;
+  } catch (const ilc::error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return ILC_ENOMEM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return ILC_EINVAL;
+  }
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+struct device_guard {
+  int prev = -1;
+  explicit device_guard(int dev) {
+    ILC_HIP(hipGetDevice(&prev));
+    if (prev != dev) ILC_HIP(hipSetDevice(dev));
+  }
+  ~device_guard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+int bits_for(uint64_t maxval) {
+  int b = 0;
+  while (b < 64 && (maxval >> b)) ++b;
+  return b;
+}
+
+float kernel_norm(int kernel) {
+  switch (kernel) {
+    case ILC_KERNEL_QUARTIC_REF:
+      // 2 * norm * integral of smoothing_kernel.cl:57-60, with the h powers
+      // folded into 1/h^2 (h^5 of the s2^2.5 factor over h^3 * h^4)
+      return static_cast<float>(2.0 * double(1.04445f) * double(1.066667f));
+    case ILC_KERNEL_QUARTIC:
+      return static_cast<float>(3.5 / M_PI);
+    default:
+      return 1.0f;
+  }
+}
+
+void check_handle(const ilc_psg* h) { ILC_REQUIRE(h != nullptr, "null handle"); }
+
+}  // namespace
+
+extern "C" {
+
+const char* ilc_last_error(void) { return ilc::last_error_cstr(); }
+
+int ilc_abi_version(void) { return 1; }
+
+ilc_status ilc_psg_create(const ilc_psg_config* cfg, ilc_psg** out) {
+  return guarded([&] {
+    ILC_REQUIRE(out != nullptr, "out is null");
+    *out = nullptr;
+    ilc_psg_config c{};
+    if (cfg) c = *cfg;
+    ILC_REQUIRE(c.mode == ILC_MODE_REFERENCE || c.mode == ILC_MODE_EXACT, "bad mode %d", c.mode);
+    ILC_REQUIRE(c.kernel >= ILC_KERNEL_QUARTIC_REF && c.kernel <= ILC_KERNEL_CUBIC,
+                "bad kernel %d", c.kernel);
+    ILC_REQUIRE(!(c.mode == ILC_MODE_REFERENCE && c.kernel == ILC_KERNEL_CUBIC),
+                "reference mode uses a quartic kernel");
+    int ndev = 0;
+    ILC_HIP(hipGetDeviceCount(&ndev));
+    ILC_REQUIRE(c.device >= 0 && c.device < ndev, "device %d not present (%d devices)", c.device,
+                ndev);
+    tile_shape ts{32, 4};
+    if (c.tile_w || c.tile_h) {
+      ILC_REQUIRE(c.tile_w > 0 && c.tile_h > 0 && (c.tile_h * c.tile_w) % kBlock == 0,
+                  "tile %dx%d is not a multiple of %d points", c.tile_w, c.tile_h, kBlock);
+      ts.tw = c.tile_w;
+      ts.ppt = c.tile_w * c.tile_h / kBlock;
+    }
+    ilc_psg* h = new ilc_psg();
+    h->device = c.device;
+    h->target = c.target_particles_per_cell ? c.target_particles_per_cell : 1000;
+    h->mode = c.mode;
+    h->kernel = c.kernel;
+    h->ts = ts;
+    *out = h;
+  });
+}
+
+ilc_status ilc_psg_destroy(ilc_psg* h) {
+  return guarded([&] {
+    if (!h) return;
+    device_guard dg(h->device);
+    delete h;
+  });
+}
+
+const char* ilc_psg_backend_name(const ilc_psg* h) {
+  (void)h;
+  return "dm_smoothing/grid (hip gfx950)";
+}
+
+ilc_status ilc_psg_init_backend(ilc_psg* h, uint64_t blocksize) {
+  return guarded([&] {
+    check_handle(h);
+    h->blocksize = blocksize;
+  });
+}
+
+ilc_status ilc_psg_setup_projected_particles(ilc_psg* h, const float* particles4_dev, uint64_t n,
+                                             const float* hsml_dev, void* stream) {
+  return guarded([&] {
+    check_handle(h);
+    if (h->blocksize == 0) fail(ILC_ENOTREADY, "init_backend was not called");
+    ILC_REQUIRE(n > 0, "no particles");
+    ILC_REQUIRE(n <= h->blocksize, "%llu particles exceed the blocksize %llu",
+                (unsigned long long)n, (unsigned long long)h->blocksize);
+    ILC_REQUIRE(n < (uint64_t(1) << 31), "block of %llu particles exceeds 2^31",
+                (unsigned long long)n);
+    ILC_REQUIRE(particles4_dev && hsml_dev, "null particle or smoothing-length pointer");
+    device_guard dg(h->device);
+    hipStream_t s = as_stream(stream);
+    h->have_grid = false;
+    const float4* p4 = reinterpret_cast<const float4*>(particles4_dev);
+    if (!h->bbox_host) {
+      ILC_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->bbox_host), sizeof(bbox_result)));
+      ILC_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->flags_host), 2 * sizeof(uint32_t)));
+    }
+    // 1. bounding box (the host scan of particle_grid.cpp:43-67)
+    int nparts = static_cast<int>(std::min<uint64_t>(1024, (n + kBlock - 1) / kBlock));
+    h->bbox_partial.ensure(size_t(nparts) * 8);
+    h->bbox_dev.ensure(1);
+    h->flags.ensure(2);
+    launch_bbox(p4, hsml_dev, n, h->bbox_partial.p, nparts, h->flags.p, h->bbox_dev.p, s);
+    ILC_HIP(hipMemcpyAsync(h->bbox_host, h->bbox_dev.p, sizeof(bbox_result),
+                           hipMemcpyDeviceToHost, s));
+    ILC_HIP(hipStreamSynchronize(s));
+    const bbox_result bb = *h->bbox_host;
+    if (bb.bad)
+      fail(ILC_EINVAL, "%u particles have non-finite coordinates/w or h outside [1.1e-19, 1.8e19]",
+           bb.bad);
+    // 2. grid parameters (host, double, particle_grid.cpp:69-100)
+    const grid_params g = compute_grid_params(bb, n, h->target);
+    const uint64_t ncells = g.ncells();
+    int32_t kz0 = 0;
+    {
+      const float zlo = (bb.mn[2] - g.minc[2]) / g.cs[2];
+      const float zhi = (bb.mx[2] - g.minc[2]) / g.cs[2];
+      kz0 = static_cast<int32_t>(std::floor(zlo));
+      if (h->mode == ILC_MODE_REFERENCE && std::floor(zlo) != std::floor(zhi))
+        fail(ILC_EINVAL,
+             "reference mode needs projected particles (one z layer); z spans [%g, %g]",
+             double(bb.mn[2]), double(bb.mx[2]));
+    }
+    // 3. keys, counts, per-cell max h
+    h->keys.ensure(n);
+    h->vals.ensure(n);
+    h->counts.ensure(ncells + 1);
+    h->start.ensure(ncells + 1);
+    h->maxh_bits.ensure(ncells);
+    launch_fill_u32(h->counts.p, 0u, ncells + 1, s);
+    launch_fill_u32(h->maxh_bits.p, 0u, ncells, s);
+    launch_fill_u32(h->flags.p, 0u, 1, s);
+    launch_keys(p4, hsml_dev, uint32_t(n), g, h->keys.p, h->vals.p, h->counts.p, h->maxh_bits.p,
+                h->flags.p, s);
+    // 4. stable sort of (key, index) and cell offsets
+    h->temp.ensure(std::max(radix_temp_words(n), scan_temp_words(ncells + 1)));
+    launch_radix_sort(h->keys.p, h->vals.p, uint32_t(n), bits_for(ncells - 1), h->temp.p, s);
+    launch_exclusive_scan(h->counts.p, h->start.p, ncells + 1, h->temp.p, s);
+    // 5. sorted SoA
+    h->part.ensure(n);
+    h->hs.ensure(n);
+    launch_build_soa(p4, hsml_dev, h->vals.p, uint32_t(n), kernel_norm(h->kernel), h->part.p,
+                     h->hs.p, s);
+    ILC_HIP(hipMemcpyAsync(h->flags_host, h->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    ILC_HIP(hipStreamSynchronize(s));
+    if (h->flags_host[0])
+      fail(ILC_ERANGE, "%u particles fall outside the grid", h->flags_host[0]);
+    h->gp = g;
+    h->n = n;
+    h->max_h = bb.max_h;
+    float dd = g.cs[0] * g.cs[0];
+    dd = dd + g.cs[1] * g.cs[1];
+    dd = dd + g.cs[2] * g.cs[2];
+    h->cell_radius = 0.5f * std::sqrt(dd);
+    h->kz0 = kz0;
+    h->have_grid = true;
+  });
+}
+
+ilc_status ilc_psg_setup_evaluation_points(ilc_psg* h, const float* points4_dev, uint64_t n,
+                                           uint32_t layout_nx, void* stream) {
+  return guarded([&] {
+    check_handle(h);
+    ILC_REQUIRE(n > 0, "no evaluation points");
+    ILC_REQUIRE(points4_dev, "null point pointer");
+    ILC_REQUIRE(n < (uint64_t(1) << 32), "too many points");
+    device_guard dg(h->device);
+    h->pts = reinterpret_cast<const float4*>(points4_dev);
+    h->npts = n;
+    h->lnx = layout_nx ? layout_nx : uint32_t(h->ts.tw);
+    h->lny = uint32_t((n + h->lnx - 1) / h->lnx);
+    h->acc.ensure(n);
+    launch_fill_f64(h->acc.p, 0.0, n, as_stream(stream));
+  });
+}
+
+static void ensure_cubic(ilc_psg* h) {
+  if (h->kernel != ILC_KERNEL_CUBIC || h->cubic_ready) return;
+  std::vector<float> t = cubic_projected_table(4096);
+  h->cubic.ensure(t.size());
+  ILC_HIP(hipMemcpy(h->cubic.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+  h->cubic_ready = true;
+}
+
+static grid_view view_of(const ilc_psg* h) {
+  grid_view g;
+  g.part = h->part.p;
+  g.key = h->keys.p;
+  g.hs = h->hs.p;
+  g.start = h->start.p;
+  g.maxh_bits = h->maxh_bits.p;
+  for (int a = 0; a < 3; ++a) {
+    g.nc[a] = h->gp.nc[a];
+    g.minc[a] = h->gp.minc[a];
+    g.cs[a] = h->gp.cs[a];
+  }
+  g.max_h = h->max_h;
+  g.cell_radius = h->cell_radius;
+  g.kz0 = h->kz0;
+  g.ncells = static_cast<uint32_t>(h->gp.ncells());
+  return g;
+}
+
+ilc_status ilc_psg_run(ilc_psg* h, void* stream) {
+  return guarded([&] {
+    check_handle(h);
+    if (!h->have_grid) fail(ILC_ENOTREADY, "run before setup_projected_particles");
+    if (!h->pts) fail(ILC_ENOTREADY, "run before setup_evaluation_points");
+    device_guard dg(h->device);
+    ensure_cubic(h);
+    point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
+    launch_gather(view_of(h), p, h->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT,
+                  h->kernel, h->ts, h->acc.p, nullptr, h->cubic.p, as_stream(stream));
+  });
+}
+
+ilc_status ilc_psg_retrieve_results(ilc_psg* h, float scale, float* out_dev, void* stream) {
+  return guarded([&] {
+    check_handle(h);
+    if (!h->pts) fail(ILC_ENOTREADY, "no evaluation points");
+    ILC_REQUIRE(out_dev, "null output");
+    device_guard dg(h->device);
+    launch_retrieve(h->acc.p, h->npts, scale, out_dev, as_stream(stream));
+  });
+}
+
+ilc_status ilc_psg_retrieve_results_f64(ilc_psg* h, double* out_dev, void* stream) {
+  return guarded([&] {
+    check_handle(h);
+    if (!h->pts) fail(ILC_ENOTREADY, "no evaluation points");
+    ILC_REQUIRE(out_dev, "null output");
+    device_guard dg(h->device);
+    ILC_HIP(hipMemcpyAsync(out_dev, h->acc.p, sizeof(double) * h->npts, hipMemcpyDeviceToDevice,
+                           as_stream(stream)));
+  });
+}
+
+ilc_status ilc_psg_grid_info(const ilc_psg* h, ilc_grid_info* out) {
+  return guarded([&] {
+    check_handle(h);
+    ILC_REQUIRE(out, "null output");
+    if (!h->have_grid) fail(ILC_ENOTREADY, "no grid");
+    for (int a = 0; a < 3; ++a) {
+      out->num_cells[a] = h->gp.nc[a];
+      out->min_corner[a] = h->gp.minc[a];
+      out->cell_size[a] = h->gp.cs[a];
+    }
+    out->max_h = h->max_h;
+    out->num_particles = h->n;
+  });
+}
+
+ilc_status ilc_psg_export_grid(const ilc_psg* h, int32_t* cells2_dev, float* maxh_dev,
+                               uint32_t* perm_dev, void* stream) {
+  return guarded([&] {
+    check_handle(h);
+    if (!h->have_grid) fail(ILC_ENOTREADY, "no grid");
+    device_guard dg(h->device);
+    hipStream_t s = as_stream(stream);
+    launch_export_cells(h->start.p, h->maxh_bits.p, static_cast<uint32_t>(h->gp.ncells()),
+                        cells2_dev, maxh_dev, s);
+    if (perm_dev)
+      ILC_HIP(hipMemcpyAsync(perm_dev, h->vals.p, sizeof(uint32_t) * h->n,
+                             hipMemcpyDeviceToDevice, s));
+  });
+}
+
+ilc_status ilc_psg_count_evals(ilc_psg* h, uint64_t* counts_dev, uint64_t* total, void* stream) {
+  return guarded([&] {
+    check_handle(h);
+    if (!h->have_grid) fail(ILC_ENOTREADY, "no grid");
+    if (!h->pts) fail(ILC_ENOTREADY, "no evaluation points");
+    device_guard dg(h->device);
+    hipStream_t s = as_stream(stream);
+    h->cnt.ensure(h->npts);
+    h->cnt_scratch.ensure(1);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->cnt.p), 0u, 2 * h->npts, s);
+    point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
+    launch_gather(view_of(h), p, OP_COUNT, h->kernel, h->ts, nullptr, h->cnt.p, nullptr, s);
+    if (counts_dev)
+      ILC_HIP(hipMemcpyAsync(counts_dev, h->cnt.p, sizeof(unsigned long long) * h->npts,
+                             hipMemcpyDeviceToDevice, s));
+    const unsigned long long tot = launch_sum_u64(h->cnt.p, h->npts, h->cnt_scratch.p, s);
+    if (total) *total = tot;
+  });
+}
+
+ilc_status ilc_camera_basis(const ilc_camera* cam, double b0[3], double b1[3],
+                            double min_position[3], double* pixel_size) {
+  return guarded([&] {
+    ILC_REQUIRE(cam, "null camera");
+    const camera_frame f = make_camera_frame(*cam);
+    for (int i = 0; i < 3; ++i) {
+      if (b0) b0[i] = f.b0[i];
+      if (b1) b1[i] = f.b1[i];
+      if (min_position) min_position[i] = f.min_pos[i];
+    }
+    if (pixel_size) *pixel_size = f.pixel_size;
+  });
+}
+
+ilc_status ilc_project_particles(const ilc_camera* cam, const float* in4_dev, uint64_t n,
+                                 float* out4_dev, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(cam && in4_dev && out4_dev, "null argument");
+    const camera_frame f = make_camera_frame(*cam);
+    launch_project_particles(f, static_cast<float>(cam->depth),
+                             reinterpret_cast<const float4*>(in4_dev), n,
+                             reinterpret_cast<float4*>(out4_dev), as_stream(stream));
+  });
+}
+
+ilc_status ilc_camera_pixel_points(const ilc_camera* cam, float* out4_dev, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(cam && out4_dev, "null argument");
+    const camera_frame f = make_camera_frame(*cam);
+    launch_pixel_points(f, cam->nx, cam->ny, reinterpret_cast<float4*>(out4_dev),
+                        as_stream(stream));
+  });
+}
+
+ilc_status ilc_quantity_scaled(const float* in_dev, double scale, uint64_t n, float* w_dev,
+                               void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(w_dev, "null output");
+    launch_quantity_scaled(in_dev, scale, n, w_dev, as_stream(stream));
+  });
+}
+
+ilc_status ilc_quantity_xray_bolometric(const float* rho_dev, const float* temp_dev,
+                                        const float* xe_dev, uint64_t n,
+                                        const float* table_host, int32_t ntab, float x0,
+                                        float dx, float* w_dev, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(rho_dev && temp_dev && xe_dev && w_dev && table_host, "null argument");
+    ILC_REQUIRE(dx > 0.0f, "table spacing must be positive");
+    ILC_REQUIRE(ntab >= 1 && ntab <= 4096, "table size %d not in [1, 4096]", ntab);
+    hipStream_t s = as_stream(stream);
+    static thread_local dbuf<float> table;
+    table.ensure(size_t(ntab));
+    ILC_HIP(hipMemcpyAsync(table.p, table_host, sizeof(float) * ntab, hipMemcpyHostToDevice, s));
+    launch_xray_bolometric(rho_dev, temp_dev, xe_dev, n, table.p, ntab, x0, dx, w_dev, s);
+    ILC_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,428 @@
+// Particle-grid build on gfx950: bounding box, cell keys, stable LSD radix
+// sort, cell offsets, per-cell max h, sorted SoA.
+//
+// Replaces particle_grid / smoothing_particle_grid (particle_grid.cpp:25-340,
+// smoothing_particle_grid.cpp:29-54, particle_grid.cl:45-216, util.cl:24-49).
+// The reference sorts ulong keys with boost::compute::sort_by_key (stable for
+// integral keys) and then derives cell ranges; here a 8-bit-digit LSD radix
+// sort on 32-bit keys gives the same permutation (each pass is a stable
+// counting sort, ranks computed with wave64 ballots), and the cell ranges
+// come from a scan of per-cell counts.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace ilc {
+
+namespace {
+
+__device__ inline float wave_min(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline uint32_t wave_sum(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// lanes below the calling lane
+__device__ inline uint64_t lanes_below() {
+  const uint32_t lane = threadIdx.x & 63u;
+  return lane == 0 ? 0ull : (~0ull >> (64u - lane));
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// Per-block min/max of x, y, z, max h, and a count of invalid particles.
+// Non-finite coordinates or w, and h that is not a positive normal float
+// whose square stays finite, are invalid.
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(const float4* __restrict__ p4,
+                                                          const float* __restrict__ h,
+                                                          uint64_t n, float* __restrict__ part,
+                                                          uint32_t* __restrict__ bad) {
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float mh = 0.0f;
+  uint32_t nbad = 0;
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    const float4 q = p4[i];
+    const float hh = h[i];
+    const bool ok = isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w) &&
+                    hh >= 1.1e-19f && hh <= 1.8e19f;
+    if (!ok) {
+      ++nbad;
+      continue;
+    }
+    mn[0] = fminf(mn[0], q.x); mx[0] = fmaxf(mx[0], q.x);
+    mn[1] = fminf(mn[1], q.y); mx[1] = fmaxf(mx[1], q.y);
+    mn[2] = fminf(mn[2], q.z); mx[2] = fmaxf(mx[2], q.z);
+    mh = fmaxf(mh, hh);
+  }
+  __shared__ float red[4][7];
+  __shared__ uint32_t redb[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float v[7] = {wave_min(mn[0]), wave_min(mn[1]), wave_min(mn[2]), wave_max(mx[0]),
+                wave_max(mx[1]), wave_max(mx[2]), wave_max(mh)};
+  uint32_t b = wave_sum(nbad);
+  if (lane == 0) {
+    for (int k = 0; k < 7; ++k) red[wave][k] = v[k];
+    redb[wave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float r[7];
+    for (int k = 0; k < 7; ++k) r[k] = red[0][k];
+    uint32_t bb = redb[0];
+    for (int w = 1; w < 4; ++w) {
+      for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], red[w][k]);
+      for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], red[w][k]);
+      bb += redb[w];
+    }
+    for (int k = 0; k < 7; ++k) part[blockIdx.x * 8 + k] = r[k];
+    if (bb) atomicAdd(bad, bb);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const float* __restrict__ part, int nparts,
+                                                        const uint32_t* __restrict__ bad,
+                                                        bbox_result* __restrict__ out) {
+  __shared__ float red[kBlock][7];
+  float r[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0f};
+  for (int i = threadIdx.x; i < nparts; i += kBlock) {
+    for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], part[i * 8 + k]);
+    for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], part[i * 8 + k]);
+  }
+  for (int k = 0; k < 7; ++k) red[threadIdx.x][k] = r[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kBlock; ++i) {
+      for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], red[i][k]);
+      for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], red[i][k]);
+    }
+    for (int k = 0; k < 3; ++k) {
+      out->mn[k] = r[k];
+      out->mx[k] = r[3 + k];
+    }
+    out->max_h = r[6];
+    out->bad = *bad;
+  }
+}
+
+// Cell indices exactly as grid3d_get_cell_indices (particle_grid.cl:45-53):
+// floor((p - min) / size) in FP32 with a correctly rounded division.
+__device__ inline int32_t floor_to_int(float f) {
+  const float fl = floorf(f);
+  if (!(fl == fl)) return 0;
+  if (fl >= 2147483647.0f) return INT32_MAX;
+  if (fl <= -2147483648.0f) return INT32_MIN;
+  return static_cast<int32_t>(fl);
+}
+
+// keys[i] = cell of particle i, vals[i] = i; per-cell counts and max h via
+// atomics (count[c] += 1, maxh_bits[c] = max over the float bit patterns,
+// which order like the floats for h > 0).  Keys outside the grid (the
+// reference writes out of bounds there) are counted in *err and clamped.
+__global__ __launch_bounds__(kBlock) void k_keys(const float4* __restrict__ p4,
+                                                  const float* __restrict__ h, uint32_t n,
+                                                  grid_params g, uint32_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals,
+                                                  uint32_t* __restrict__ count,
+                                                  uint32_t* __restrict__ maxh_bits,
+                                                  uint32_t* __restrict__ err) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = p4[i];
+  const float c[3] = {q.x, q.y, q.z};
+  int32_t idx[3];
+  bool ok = true;
+  for (int a = 0; a < 3; ++a) {
+    const float d = c[a] - g.minc[a];
+    idx[a] = floor_to_int(d / g.cs[a]);
+    ok = ok && idx[a] >= 0 && idx[a] < g.nc[a];
+  }
+  uint32_t key = 0;
+  if (ok)
+    key = uint32_t(idx[0]) + uint32_t(idx[1]) * uint32_t(g.nc[0]) +
+          uint32_t(idx[2]) * uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
+  else
+    atomicAdd(err, 1u);
+  keys[i] = key;
+  vals[i] = i;
+  atomicAdd(&count[key], 1u);
+  atomicMax(&maxh_bits[key], __float_as_uint(h[i]));
+}
+
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;  // 4096 keys per workgroup
+
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys,
+                                                        uint32_t n, int shift, uint32_t ntiles,
+                                                        uint32_t* __restrict__ hist) {
+  __shared__ uint32_t hh[256];
+  hh[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kTile;
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&hh[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * ntiles + blockIdx.x] = hh[threadIdx.x];
+}
+
+// Stable scatter of one 8-bit digit.  Items of a tile are visited in input
+// order (round r, thread t -> item r*256 + t); within a wave, lanes sharing a
+// digit are found with 8 ballots and ranked by popcount of lower lanes;
+// across waves and rounds, per-digit running counts in LDS keep the order.
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout,
+                                                           uint32_t* __restrict__ vout, uint32_t n,
+                                                           int shift, uint32_t ntiles,
+                                                           const uint32_t* __restrict__ offs) {
+  __shared__ uint32_t base_d[256];
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wc[4][256];
+  const uint32_t t = threadIdx.x;
+  const int wave = t >> 6;
+  base_d[t] = offs[t * ntiles + blockIdx.x];
+  run[t] = 0;
+  for (int w = 0; w < 4; ++w) wc[w][t] = 0;
+  __syncthreads();
+  const uint64_t below = lanes_below();
+  const uint32_t tile0 = blockIdx.x * kTile;
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = tile0 + r * kBlock + t;
+    const bool valid = i < n;
+    const uint32_t k = valid ? kin[i] : 0u;
+    const uint32_t v = valid ? vin[i] : 0u;
+    const uint32_t d = (k >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t rank = __popcll(peers & below);
+    if (valid && rank == 0) wc[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = base_d[d] + run[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wc[w][d];
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+    for (int w = 0; w < 4; ++w) wc[w][t] = 0;
+    __syncthreads();
+  }
+}
+
+// ---- exclusive scan of uint32 (reduce, scan of block sums, apply)
+
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t ws[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) ws[wave] = incl;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) off += ws[w];
+    tot += ws[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                         uint32_t* __restrict__ sums) {
+  const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
+  uint32_t s = 0;
+  for (int k = 0; k < kItems; ++k)
+    if (base + k < n) s += in[base + k];
+  s = wave_sum(s);
+  __shared__ uint32_t ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) sums[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_sums(uint32_t* __restrict__ sums, uint32_t nb) {
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < nb; c0 += kBlock * kItems) {
+    const uint32_t base = c0 + threadIdx.x * kItems;
+    uint32_t v[kItems], s = 0;
+    for (int k = 0; k < kItems; ++k) {
+      v[k] = base + k < nb ? sums[base + k] : 0u;
+      s += v[k];
+    }
+    uint32_t tot;
+    uint32_t off = block_exclusive_scan(s, &tot) + carry;
+    for (int k = 0; k < kItems; ++k) {
+      if (base + k < nb) sums[base + k] = off;
+      off += v[k];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ in,
+                                                        uint32_t* __restrict__ out, uint64_t n,
+                                                        const uint32_t* __restrict__ sums) {
+  const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
+  uint32_t v[kItems], s = 0;
+  for (int k = 0; k < kItems; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t off = block_exclusive_scan(s, &tot) + sums[blockIdx.x];
+  for (int k = 0; k < kItems; ++k) {
+    if (base + k < n) out[base + k] = off;
+    off += v[k];
+  }
+}
+
+// Sorted SoA: part[s] = (x, y, coef, 1/h^2) of particle perm[s], hs[s] = h.
+// coef = w * C / h^2 with C the kernel's normalisation, so that the kernel
+// value is coef * shape(r^2 / h^2).
+__global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__ p4,
+                                                       const float* __restrict__ h,
+                                                       const uint32_t* __restrict__ perm,
+                                                       uint32_t n, float norm,
+                                                       float4* __restrict__ part,
+                                                       float* __restrict__ hs) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t i = perm[s];
+  const float4 q = p4[i];
+  const float hh = h[i];
+  const float ih2 = 1.0f / (hh * hh);
+  part[s] = make_float4(q.x, q.y, q.w * (norm * ih2), ih2);
+  hs[s] = hh;
+}
+
+// Reference layout of the grid for parity checks.
+__global__ __launch_bounds__(kBlock) void k_export_cells(const uint32_t* __restrict__ start,
+                                                          const uint32_t* __restrict__ maxh_bits,
+                                                          uint32_t ncells, int2* cells2,
+                                                          float* maxh) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= ncells) return;
+  const uint32_t b = start[c], e = start[c + 1];
+  if (cells2) cells2[c] = e > b ? make_int2(int(b), int(e)) : make_int2(0, 0);
+  if (maxh) maxh[c] = __uint_as_float(maxh_bits[c]);
+}
+
+inline uint32_t blocks_for(uint64_t n, uint64_t per) {
+  return static_cast<uint32_t>((n + per - 1) / per);
+}
+
+}  // namespace
+
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  uint32_t nb = blocks_for(n, kBlock);
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_fill_u32, dim3(nb), dim3(kBlock), 0, s, p, v, n);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_bbox(const float4* p4, const float* h, uint64_t n, float* partial, int nparts,
+                 uint32_t* bad, bbox_result* out, hipStream_t s) {
+  launch_fill_u32(bad, 0u, 1, s);
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, p4, h, n, partial, bad);
+  ILC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, partial, nparts, bad, out);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_keys(const float4* p4, const float* h, uint32_t n, const grid_params& g,
+                 uint32_t* keys, uint32_t* vals, uint32_t* count, uint32_t* maxh_bits,
+                 uint32_t* err, hipStream_t s) {
+  hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, n, g, keys,
+                     vals, count, maxh_bits, err);
+  ILC_HIP(hipGetLastError());
+}
+
+uint64_t scan_temp_words(uint64_t n) { return blocks_for(n, kTile) + 1; }
+
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* temp,
+                           hipStream_t s) {
+  if (!n) return;
+  const uint32_t nb = blocks_for(n, kTile);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, temp);
+  ILC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, temp, nb);
+  ILC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s, in, out, n, temp);
+  ILC_HIP(hipGetLastError());
+}
+
+uint64_t radix_temp_words(uint64_t n) {
+  const uint64_t ntiles = blocks_for(n, kTile);
+  return 2 * n + 256 * ntiles + scan_temp_words(256 * ntiles);
+}
+
+void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
+                       hipStream_t s) {
+  if (n <= 1 || key_bits <= 0) return;
+  const uint32_t ntiles = blocks_for(n, kTile);
+  uint32_t* k2 = temp;
+  uint32_t* v2 = temp + n;
+  uint32_t* hist = temp + 2 * uint64_t(n);
+  uint32_t* stemp = hist + 256 * uint64_t(ntiles);
+  uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
+  int passes = (key_bits + 7) / 8;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, ka, n, shift, ntiles, hist);
+    ILC_HIP(hipGetLastError());
+    launch_exclusive_scan(hist, hist, uint64_t(256) * ntiles, stemp, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, ka, va, kb, vb, n, shift,
+                       ntiles, hist);
+    ILC_HIP(hipGetLastError());
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    ILC_HIP(hipMemcpyAsync(keys, ka, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    ILC_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  }
+}
+
+void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm, uint32_t n,
+                      float norm, float4* part, float* hs, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_build_soa, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, perm, n,
+                     norm, part, hs);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_export_cells(const uint32_t* start, const uint32_t* maxh_bits, uint32_t ncells,
+                         int32_t* cells2, float* maxh, hipStream_t s) {
+  if (!ncells) return;
+  hipLaunchKernelGGL(k_export_cells, dim3(blocks_for(ncells, kBlock)), dim3(kBlock), 0, s, start,
+                     maxh_bits, ncells, reinterpret_cast<int2*>(cells2), maxh);
+  ILC_HIP(hipGetLastError());
+}
+
+}  // namespace ilc

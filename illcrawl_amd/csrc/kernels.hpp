@@ -1,0 +1,79 @@
+// Host launchers of the device kernels (defined in the .hip sources).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace ilc {
+
+// ---- grid.hip
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+void launch_bbox(const float4* p4, const float* h, uint64_t n, float* partial, int nparts,
+                 uint32_t* bad, bbox_result* out, hipStream_t s);
+void launch_keys(const float4* p4, const float* h, uint32_t n, const grid_params& g,
+                 uint32_t* keys, uint32_t* vals, uint32_t* count, uint32_t* maxh_bits,
+                 uint32_t* err, hipStream_t s);
+uint64_t scan_temp_words(uint64_t n);
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* temp,
+                           hipStream_t s);
+uint64_t radix_temp_words(uint64_t n);
+void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
+                       hipStream_t s);
+void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm, uint32_t n,
+                      float norm, float4* part, float* hs, hipStream_t s);
+void launch_export_cells(const uint32_t* start, const uint32_t* maxh_bits, uint32_t ncells,
+                         int32_t* cells2, float* maxh, hipStream_t s);
+
+// ---- gather.hip
+// Device view of one built grid (sorted SoA + cells).
+struct grid_view {
+  const float4* part;         // (x, y, coef, 1/h^2) sorted by cell
+  const uint32_t* key;        // sorted cell keys
+  const float* hs;            // sorted h
+  const uint32_t* start;      // ncells + 1 offsets
+  const uint32_t* maxh_bits;  // per-cell max h (float bits)
+  int32_t nc[3];
+  float minc[3];
+  float cs[3];
+  float max_h;
+  float cell_radius;
+  int32_t kz0;  // z layer of the particles (single layer in reference mode)
+  uint32_t ncells;
+};
+
+// Evaluation points laid out as an image (lnx x lny, row-major).
+struct point_view {
+  const float4* pts;
+  uint32_t n;
+  uint32_t lnx, lny;
+};
+
+enum gather_op { OP_REFERENCE = 0, OP_EXACT = 1, OP_COUNT = 2 };
+
+struct tile_shape {
+  int tw;   // tile width (points)
+  int ppt;  // points per thread (tile height = 256/tw * ppt)
+};
+
+void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,
+                   const tile_shape& ts, double* acc, unsigned long long* counts,
+                   const float* cubic_table, hipStream_t s);
+
+// ---- project.hip
+void launch_project_particles(const camera_frame& f, float depth, const float4* in, uint64_t n,
+                              float4* out, hipStream_t s);
+void launch_pixel_points(const camera_frame& f, uint32_t nx, uint32_t ny, float4* out,
+                         hipStream_t s);
+void launch_retrieve(const double* acc, uint64_t n, float scale, float* out, hipStream_t s);
+void launch_quantity_scaled(const float* in, double scale, uint64_t n, float* w, hipStream_t s);
+void launch_xray_bolometric(const float* rho, const float* temp, const float* xe, uint64_t n,
+                            const float* table_dev, int32_t ntab, float x0, float dx, float* w,
+                            hipStream_t s);
+void launch_fill_f64(double* p, double v, uint64_t n, hipStream_t s);
+unsigned long long launch_sum_u64(const unsigned long long* p, uint64_t n,
+                                  unsigned long long* scratch, hipStream_t s);
+
+}  // namespace ilc

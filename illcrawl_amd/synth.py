@@ -190,3 +190,30 @@ def gaunt_table_1d(n=512, t_min=1e4, t_max=1e9):
     lg = lnT / math.log(10.0)
     y = 1.1 + 0.34 * np.exp(-((5.5 - lg) ** 2) / 3.0)
     return y.astype(np.float32), np.float32(x0), np.float32(dx)
+
+
+def cached(name, cache_dir=None):
+    """Workload snapshot by name (``c1``, ``c2``, ``c4``), cached as an .npz
+    under ``cache_dir`` (default $ILC_CACHE or /tmp/ilc_cache) so that one
+    session generates it once.  Loaded with numpy's pickle-free loader."""
+    import os
+    cache_dir = cache_dir or os.environ.get("ILC_CACHE", "/tmp/ilc_cache")
+    path = os.path.join(cache_dir, f"{name}.npz")
+    makers = {"c1": config1, "c2": config2, "c4": config4}
+    if os.path.exists(path):
+        z = np.load(path)
+        halos = [(z["halo_c"][i], float(z["halo_rs"][i])) for i in range(z["halo_c"].shape[0])]
+        return Snapshot(box=float(z["box"]), pos=z["pos"], mass=z["mass"], rho=z["rho"],
+                        hsml=z["hsml"], temp=z["temp"] if "temp" in z else None,
+                        xe=z["xe"] if "xe" in z else None, halos=halos)
+    snap = makers[name]()
+    os.makedirs(cache_dir, exist_ok=True)
+    extra = {}
+    if snap.temp is not None:
+        extra = dict(temp=snap.temp, xe=snap.xe)
+    tmp = path + ".tmp.npz"
+    np.savez(tmp, box=snap.box, pos=snap.pos, mass=snap.mass, rho=snap.rho, hsml=snap.hsml,
+             halo_c=np.array([c for c, _ in snap.halos]),
+             halo_rs=np.array([rs for _, rs in snap.halos]), **extra)
+    os.replace(tmp, path)
+    return snap

@@ -1,0 +1,218 @@
+/*
+ * illcrawl_psg.h -- C ABI of the MI355X-native projective SPH smoothing path.
+ *
+ * The entry points are the device-side boundary that illcrawl's projective
+ * smoothing backend stack calls.  Each declaration names the reference
+ * interface it replaces (file:line in the illcrawl sources).
+ *
+ * Conventions
+ *  - Every function returns an ilc_status; on failure ilc_last_error()
+ *    returns a thread-local message.  Nothing throws across the ABI.
+ *  - Pointers named *_dev are device (HBM) pointers; they are borrowed for
+ *    the duration of the call (and of the work it enqueues on `stream`).
+ *  - `stream` is a hipStream_t (NULL = the default stream).  Work is
+ *    enqueued asynchronously unless the description says otherwise.
+ *  - A handle is used by one host thread at a time.
+ *  - Layouts follow the reference: particles are float4 (x, y, z, w)
+ *    (particle_grid.hpp:43-48); images are row-major with index ix + nx*iy
+ *    (camera.cl:37, multi_array.hpp:503-536).
+ */
+#ifndef ILLCRAWL_PSG_H
+#define ILLCRAWL_PSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  ILC_OK = 0,
+  ILC_EINVAL = 1,    /* invalid argument or precondition */
+  ILC_ENOMEM = 2,    /* device or host allocation failed */
+  ILC_EHIP = 3,      /* HIP runtime error */
+  ILC_ENOTREADY = 4, /* call order violated (e.g. run before setup) */
+  ILC_ERANGE = 5     /* data outside what the grid can index */
+} ilc_status;
+
+/* Culling semantics of the gather. */
+typedef enum {
+  /* The reference's: per point, cells within max h of the point's own
+   * cell + the cell half-diagonal (capped by the overall max h), from the
+   * unclamped cell key (smoothing_particle_grid.cl:54-100). */
+  ILC_MODE_REFERENCE = 0,
+  /* Every particle whose support covers the point. */
+  ILC_MODE_EXACT = 1
+} ilc_mode;
+
+/* Projected smoothing kernel. */
+typedef enum {
+  /* 2*1.04445/h^3 * 1.066667 (h^2-r^2)^2.5 / h^4: the reference's projected
+   * quartic including its factor 2 (smoothing_kernel.cl:46-61). */
+  ILC_KERNEL_QUARTIC_REF = 0,
+  /* the same shape normalised to unit integral: 7/(2 pi h^2) (1-q^2)^2.5 */
+  ILC_KERNEL_QUARTIC = 1,
+  /* projected M4 cubic spline, support 2h (smoothing_kernel.cl:64-84 in 3-D),
+   * tabulated on q^2 */
+  ILC_KERNEL_CUBIC = 2
+} ilc_kernel;
+
+typedef struct ilc_psg ilc_psg;
+
+typedef struct {
+  int device;                        /* HIP device ordinal */
+  uint64_t target_particles_per_cell; /* 0 -> 1000, the projective grid's
+                                        (projective_smoothing_backend_grid.cpp:29) */
+  int mode;                          /* ilc_mode */
+  int kernel;                        /* ilc_kernel */
+  int tile_w, tile_h;                /* gather tile in points; 0 -> autotuned default */
+} ilc_psg_config;
+
+typedef struct {
+  int32_t num_cells[3];   /* particle_grid::get_num_grid_cells (particle_grid.cpp:126-129) */
+  float min_corner[3];    /* particle_grid::get_grid_min_corner (:121-124) */
+  float cell_size[3];     /* particle_grid::get_grid_cell_sizes (:131-134) */
+  float max_h;            /* smoothing_particle_grid::get_overall_max_smoothing_length
+                             (smoothing_particle_grid.cpp:68-72) */
+  uint64_t num_particles; /* particle_grid::get_num_particles (:342-346) */
+} ilc_grid_info;
+
+/* Camera of illcrawl's camera class (camera.hpp:35-56) plus the
+ * projective wrapper's integration depth
+ * (projective_smoothing_reconstruction.hpp:61-63). */
+typedef struct {
+  double position[3];
+  double look_at[3];      /* unit vector */
+  double roll;            /* radians */
+  double width;           /* screen width; pixel size = width / nx */
+  uint32_t nx, ny;
+  double depth;           /* particles with camera z outside [0, depth] get w = 0 */
+} ilc_camera;
+
+/* ---------------------------------------------------------------- errors */
+
+/* Thread-local message of the last failure (qcl::qcl_error /
+ * std::runtime_error text in the reference, qcl.hpp:79-118). */
+const char* ilc_last_error(void);
+/* Version of this ABI. */
+int ilc_abi_version(void);
+
+/* ---------------------------------------- projective_smoothing_grid backend */
+
+/* dm::projective_smoothing_grid constructor
+ * (projective_smoothing_backend_grid.cpp:27-31; dm::grid ctor,
+ * dm_reconstruction_backend_grid.cpp:26-37). */
+ilc_status ilc_psg_create(const ilc_psg_config* cfg, ilc_psg** out);
+ilc_status ilc_psg_destroy(ilc_psg* h);
+
+/* reconstruction_backend::get_backend_name (reconstruction_backend.hpp:43;
+ * dm_reconstruction_backend_grid.cpp:51-55).  Static string. */
+const char* ilc_psg_backend_name(const ilc_psg* h);
+
+/* reconstruction_backend::init_backend (reconstruction_backend.hpp:45;
+ * dm_reconstruction_backend_grid.cpp:57-61): largest particle block a
+ * later setup may pass. */
+ilc_status ilc_psg_init_backend(ilc_psg* h, uint64_t blocksize);
+
+/* projective_smoothing_backend::setup_projected_particles
+ * (projective_smoothing_backend.hpp:35-42;
+ * projective_smoothing_backend_grid.cpp:33-45, which continues into
+ * dm::grid::setup_particles, dm_reconstruction_backend_grid.cpp:63-85, and
+ * the smoothing_particle_grid / particle_grid constructors).
+ * particles4_dev: n float4 (x, y, z, w) in the camera frame; hsml_dev: n
+ * smoothing lengths (the backend's one additional dataset).  Builds the
+ * reference's grid on the device.  Synchronises `stream` once (the grid
+ * size depends on the particle bounding box, as in the reference's host
+ * code, particle_grid.cpp:43-100). */
+ilc_status ilc_psg_setup_projected_particles(ilc_psg* h, const float* particles4_dev,
+                                             uint64_t n, const float* hsml_dev,
+                                             void* stream);
+
+/* reconstruction_backend::setup_evaluation_points
+ * (reconstruction_backend.hpp:50-51; dm_reconstruction_backend_grid.cpp:87-107):
+ * points4_dev: n float4 points (camera frame), kept by pointer until the next
+ * call.  layout_nx: row length of the point array when it is an image
+ * (tiles are formed on it); 0 = treat the points as one row.  Zeroes the
+ * FP64 result accumulator. */
+ilc_status ilc_psg_setup_evaluation_points(ilc_psg* h, const float* points4_dev, uint64_t n,
+                                           uint32_t layout_nx, void* stream);
+
+/* reconstruction_backend::run (reconstruction_backend.hpp:53;
+ * dm_reconstruction_backend_grid.cpp:109-134): adds the current particle
+ * block's contribution to every point (results accumulate until the next
+ * setup_evaluation_points). */
+ilc_status ilc_psg_run(ilc_psg* h, void* stream);
+
+/* reconstruction_backend::retrieve_results (reconstruction_backend.hpp:41)
+ * fused with the projective wrapper's dA scaling
+ * (projective_smoothing_reconstruction.cpp:49-84, math.cl:94-102):
+ * out_dev[i] = float(acc[i] * scale).  Unlike the reference, the stored
+ * accumulator is not modified, so retrieving twice gives the same image. */
+ilc_status ilc_psg_retrieve_results(ilc_psg* h, float scale, float* out_dev, void* stream);
+/* The FP64 accumulator itself (n doubles). */
+ilc_status ilc_psg_retrieve_results_f64(ilc_psg* h, double* out_dev, void* stream);
+
+/* Grid parameters of the last setup (particle_grid getters). */
+ilc_status ilc_psg_grid_info(const ilc_psg* h, ilc_grid_info* out);
+
+/* The grid in the reference's layout, for parity checks: cells2_dev
+ * (2*ncells int32: begin, end; (0,0) when empty; particle_grid.cpp:104-113,
+ * particle_grid.cl:114-190), maxh_dev (ncells floats,
+ * grid3d_determine_max_per_cell), perm_dev (n uint32: sorted position ->
+ * input index, the sorted_to_unsorted map of particle_grid.hpp:77-93).
+ * Any pointer may be NULL. */
+ilc_status ilc_psg_export_grid(const ilc_psg* h, int32_t* cells2_dev, float* maxh_dev,
+                               uint32_t* perm_dev, void* stream);
+
+/* Number of (point, particle) pairs with nonzero kernel coefficient
+ * (w != 0) and distance below the kernel support, per point written to
+ * counts_dev (n uint64, may be NULL) and in total to *total (synchronises
+ * the stream).  Independent of the culling
+ * mode: this is the unit of work "particle-kernel evaluation".  No
+ * reference counterpart (measurement only). */
+ilc_status ilc_psg_count_evals(ilc_psg* h, uint64_t* counts_dev, uint64_t* total,
+                               void* stream);
+
+/* -------------------------------------------- camera / projection wrapper */
+
+/* Camera basis, pixel size and lower-left screen corner in double, as
+ * camera::update_basis_vectors / update_min_position compute them
+ * (camera.cpp:124-155). */
+ilc_status ilc_camera_basis(const ilc_camera* cam, double b0[3], double b1[3],
+                            double min_position[3], double* pixel_size);
+
+/* projective_smoothing::project_particles (projective_smoothing_reconstruction.cpp:175-205;
+ * projection.cl:56-95): out4 = camera-frame float4 with z = 0 and w = 0 for
+ * particles outside [0, depth].  in4_dev may equal out4_dev. */
+ilc_status ilc_project_particles(const ilc_camera* cam, const float* in4_dev, uint64_t n,
+                                 float* out4_dev, void* stream);
+
+/* camera::generate_pixel_coordinates + projective_smoothing::project_evaluation_points
+ * (camera.cpp:157-189, camera.cl:23-41, projection.cl:97-122): nx*ny float4
+ * pixel-corner points in the camera frame. */
+ilc_status ilc_camera_pixel_points(const ilc_camera* cam, float* out4_dev, void* stream);
+
+/* ------------------------------------------------- quantity transformation */
+
+/* quantity_transformation with unprocessed_quantity / constant_quantity
+ * (quantity.cpp:84-131, quantities.cl:701-726): w[i] = float(scale * in[i])
+ * (in_dev == NULL: w[i] = float(scale)). */
+ilc_status ilc_quantity_scaled(const float* in_dev, double scale, uint64_t n, float* w_dev,
+                               void* stream);
+
+/* Bolometric free-free emissivity w = n_e n_i sqrt(T) g(T), n_e = n_i =
+ * 4.046263 rho x_e (the ion density convention of quantities.cl:112), g read
+ * from a 1-D table (ntab floats, node i at x0 + i*dx on ln T) with the
+ * reference's image sampler semantics (tabulated_function.cl:31-44: linear
+ * filter, half-texel centres, zero border).  The table is staged in LDS. */
+ilc_status ilc_quantity_xray_bolometric(const float* rho_dev, const float* temp_dev,
+                                        const float* xe_dev, uint64_t n,
+                                        const float* table_host, int32_t ntab, float x0,
+                                        float dx, float* w_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ILLCRAWL_PSG_H */
