@@ -216,26 +216,24 @@ ilc_status ilc_psg_setup_projected_particles(ilc_psg* h, const float* particles4
              "reference mode needs projected particles (one z layer); z spans [%g, %g]",
              double(bb.mn[2]), double(bb.mx[2]));
     }
-    // 3. keys, counts, per-cell max h
+    // 3. keys and the stable sort of (key, index)
     h->keys.ensure(n);
     h->vals.ensure(n);
     h->counts.ensure(ncells + 1);
     h->start.ensure(ncells + 1);
     h->maxh_bits.ensure(ncells);
-    launch_fill_u32(h->counts.p, 0u, ncells + 1, s);
-    launch_fill_u32(h->maxh_bits.p, 0u, ncells, s);
     launch_fill_u32(h->flags.p, 0u, 1, s);
-    launch_keys(p4, hsml_dev, uint32_t(n), g, h->keys.p, h->vals.p, h->counts.p, h->maxh_bits.p,
-                h->flags.p, s);
-    // 4. stable sort of (key, index) and cell offsets
+    launch_keys(p4, uint32_t(n), g, h->keys.p, h->vals.p, h->flags.p, s);
     h->temp.ensure(std::max(radix_temp_words(n), scan_temp_words(ncells + 1)));
     launch_radix_sort(h->keys.p, h->vals.p, uint32_t(n), bits_for(ncells - 1), h->temp.p, s);
-    launch_exclusive_scan(h->counts.p, h->start.p, ncells + 1, h->temp.p, s);
-    // 5. sorted SoA
+    // 4. sorted SoA, per-cell counts and max h, cell offsets
+    launch_fill_u32(h->counts.p, 0u, ncells + 1, s);
+    launch_fill_u32(h->maxh_bits.p, 0u, ncells, s);
     h->part.ensure(n);
     h->hs.ensure(n);
-    launch_build_soa(p4, hsml_dev, h->vals.p, uint32_t(n), kernel_norm(h->kernel), h->part.p,
-                     h->hs.p, s);
+    launch_build_soa(p4, hsml_dev, h->vals.p, h->keys.p, uint32_t(n), kernel_norm(h->kernel),
+                     h->part.p, h->hs.p, h->counts.p, h->maxh_bits.p, s);
+    launch_exclusive_scan(h->counts.p, h->start.p, ncells + 1, h->temp.p, s);
     ILC_HIP(hipMemcpyAsync(h->flags_host, h->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     ILC_HIP(hipStreamSynchronize(s));
     if (h->flags_host[0])

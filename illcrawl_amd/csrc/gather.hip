@@ -81,6 +81,27 @@ __device__ inline uint32_t prefix_count(uint64_t mask) {
 
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Exclusive scan over the 256 threads of the workgroup; *total = sum.
+// Contains two workgroup barriers.
+__device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* ws) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) ws[wave] = incl;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) off += ws[w];
+    tot += ws[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
 // Kernel shape times coefficient.  KERNEL 0: (1 - r^2/h^2)^2.5 (both
 // quartic normalisations); KERNEL 2: tabulated projected cubic spline on
 // u = r^2/h^2 in [0, 4), linear interpolation.
@@ -118,6 +139,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
   __shared__ float redf[4][4];
   __shared__ int redi[4][10];
   __shared__ float tab[KERNEL == 2 ? kCubicTable : 1];
+  __shared__ uint32_t cell_b[kBlock];
+  __shared__ uint32_t cell_off[kBlock];
+  __shared__ int2 cell_info[kBlock];
+  __shared__ uint32_t scan_ws[4];
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -141,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
   int lox[PPT], hix[PPT], loy[PPT], hiy[PPT];
   bool zin[PPT];
   double a64[PPT];
-  uint32_t cnt[PPT];
+  uint32_t cnt_acc[PPT];
 
   float bminx = INFINITY, bminy = INFINITY, bmaxx = -INFINITY, bmaxy = -INFINITY;
   int ulx = INT32_MAX, uly = INT32_MAX, uhx = INT32_MIN, uhy = INT32_MIN;
@@ -159,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
     ex[k] = q.x;
     ey[k] = q.y;
     a64[k] = 0.0;
-    cnt[k] = 0;
+    cnt_acc[k] = 0;
     lox[k] = hix[k] = loy[k] = hiy[k] = 0;
     zin[k] = false;
     if (pv[k]) {
@@ -288,14 +313,33 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
   int nf = 0, nm = 0;  // list sizes (identical in every thread)
   int slot = 0;
 
-  auto evaluate = [&]() {
-    // full list
-    for (int j0 = 0; j0 < nf; j0 += 32) {
-      const int jn = min(32, nf - j0);
+  // Evaluate every listed particle for this thread's points: FP32 partial
+  // sums over 32 particles, then FP64.  Loop bounds are wave-uniform.
+  auto eval_full = [&](int cnt) {
+    cnt = uniform(cnt);
+    for (int j0 = 0; j0 < cnt; j0 += 32) {
+      const int jn = min(32, cnt - j0);
       float a[PPT];
 #pragma unroll
       for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
-      for (int j = 0; j < jn; ++j) {
+      int j = 0;
+      if constexpr (!COUNT) {
+        for (; j + 4 <= jn; j += 4) {
+          float4 q[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) q[u] = Lf[j0 + j + u];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+              const float dx = q[u].x - ex[k], dy = q[u].y - ey[k];
+              const float r2 = fmaf(dy, dy, dx * dx);
+              a[k] = fmaf(q[u].z, shape<KERNEL>(r2, q[u].w, tab), a[k]);
+            }
+          }
+        }
+      }
+      for (; j < jn; ++j) {
         const float4 q = Lf[j0 + j];
         if constexpr (COUNT) {
 #pragma clang fp contract(off)
@@ -306,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
             const double dx = double(q.x) - double(ex[k]);
             const double dy = double(q.y) - double(ey[k]);
             const double r2 = dx * dx + dy * dy;
-            cnt[k] += (r2 < hs2) ? 1u : 0u;
+            cnt_acc[k] += (r2 < hs2) ? 1u : 0u;
           }
         } else {
 #pragma unroll
@@ -322,9 +366,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
         for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
       }
     }
+  };
+  auto eval_masked = [&](int cnt) {
     if constexpr (REF) {
-      for (int j0 = 0; j0 < nm; j0 += 32) {
-        const int jn = min(32, nm - j0);
+      cnt = uniform(cnt);
+      for (int j0 = 0; j0 < cnt; j0 += 32) {
+        const int jn = min(32, cnt - j0);
         float a[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
@@ -347,83 +394,132 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
     }
   };
 
+  // Candidate cells are the box [cx0,cx1] x [cy0,cy1] x [cz0,cz1], visited
+  // 256 at a time (one per thread, x fastest).  A cell is skipped unless
+  // its max h can reach the tile (the conservative form of the reference's
+  // cell cull); the particles of the reached cells are then streamed with a
+  // block-wide offset table, culled one by one against the tile, and
+  // appended to the LDS lists.
+  const int ncols = cx1 >= cx0 ? cx1 - cx0 + 1 : 0;
+  const int nrows = cy1 >= cy0 ? cy1 - cy0 + 1 : 0;
+  const int nlays = cz1 >= cz0 ? cz1 - cz0 + 1 : 0;
+  const int64_t ncand = int64_t(ncols) * nrows * nlays;
   const uint32_t nxny = uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
-  for (int cz = cz0; cz <= cz1; ++cz) {
-    for (int cy = cy0; cy <= cy1; ++cy) {
-      if (cx0 > cx1) break;
-      const uint32_t rowbase = uint32_t(cy) * uint32_t(g.nc[0]) + uint32_t(cz) * nxny;
-      const uint32_t sb = uint32_t(uniform(int(g.start[rowbase + uint32_t(cx0)])));
-      const uint32_t se = uint32_t(uniform(int(g.start[rowbase + uint32_t(cx1) + 1u])));
-      const bool row_in_all = REF && allz_u && cy >= r_ily && cy <= r_ihy;
-      for (uint32_t s0 = sb; s0 < se; s0 += kBlock) {
-        const uint32_t s = s0 + uint32_t(t);
-        const bool valid = s < se;
-        float4 q = make_float4(0.f, 0.f, 0.f, 1.f);
-        if (valid) q = g.part[s];
-        bool keep = false;
-        if (valid && q.z != 0.0f) {
+  // reached cells must overlap the tile box grown by the support; float
+  // rounding of particle coordinates vs cell edges is absorbed by a margin
+  const float sup_h2 = support * support * 1.00002f;
+  for (int64_t c0 = 0; c0 < ncand; c0 += kBlock) {
+    const int64_t ci = c0 + t;
+    uint32_t cb = 0, cn = 0;
+    int ccx = 0, ccy = 0, cfull = 0;
+    if (ci < ncand) {
+      const int col = int(ci % ncols);
+      const int64_t rest = ci / ncols;
+      ccx = cx0 + col;
+      ccy = cy0 + int(rest % nrows);
+      const int ccz = cz0 + int(rest / nrows);
+      const uint32_t key = uint32_t(ccx) + uint32_t(ccy) * uint32_t(g.nc[0]) + uint32_t(ccz) * nxny;
+      cb = g.start[key];
+      const uint32_t ce = g.start[key + 1];
+      if (ce > cb) {
+        const float mh = __uint_as_float(g.maxh_bits[key]);
+        const float lo_x = g.minc[0] + float(ccx) * g.cs[0];
+        const float lo_y = g.minc[1] + float(ccy) * g.cs[1];
+        // a particle's cell comes from fl((x - min) / w): its coordinate can
+        // sit a few ulps of |x| + |min| outside the cell's float edges
+        const float mgx = 1e-4f * g.cs[0] + 4e-7f * (fabsf(g.minc[0]) + fabsf(lo_x) + g.cs[0]);
+        const float mgy = 1e-4f * g.cs[1] + 4e-7f * (fabsf(g.minc[1]) + fabsf(lo_y) + g.cs[1]);
+        const float dx = fmaxf(fmaxf((lo_x - mgx) - tbx1, tbx0 - (lo_x + g.cs[0] + mgx)), 0.0f);
+        const float dy = fmaxf(fmaxf((lo_y - mgy) - tby1, tby0 - (lo_y + g.cs[1] + mgy)), 0.0f);
+        if (dx * dx + dy * dy < mh * mh * sup_h2) cn = ce - cb;
+        if constexpr (REF)
+          cfull = (allz_u && ccy >= r_ily && ccy <= r_ihy && ccx >= r_ilx && ccx <= r_ihx) ? 1 : 0;
+      }
+    }
+    // block exclusive scan of the reached cells' particle counts
+    uint32_t tot;
+    const uint32_t coff = block_scan_excl(cn, &tot, scan_ws);
+    cell_b[t] = cb;
+    cell_off[t] = coff;
+    cell_info[t] = make_int2(ccx, (ccy << 1) | cfull);
+    __syncthreads();
+    tot = uint32_t(uniform(int(tot)));
+    for (uint32_t v0 = 0; v0 < tot; v0 += kBlock) {
+      const uint32_t v = v0 + uint32_t(t);
+      const bool valid = v < tot;
+      float4 q = make_float4(0.f, 0.f, 0.f, 1.f);
+      bool keep = false, full = false;
+      int2 cinf = make_int2(0, 0);
+      if (valid) {
+        // last cell whose offset is <= v (it has a nonzero count)
+        int lo = 0;
+#pragma unroll
+        for (int step = kBlock / 2; step > 0; step >>= 1)
+          if (cell_off[lo + step] <= v) lo += step;
+        const uint32_t sidx = cell_b[lo] + (v - cell_off[lo]);
+        cinf = cell_info[lo];
+        q = g.part[sidx];
+        if (q.z != 0.0f) {
           const float dx = fmaxf(fmaxf(tbx0 - q.x, q.x - tbx1), 0.0f);
           const float dy = fmaxf(fmaxf(tby0 - q.y, q.y - tby1), 0.0f);
           const float d2 = fmaf(dy, dy, dx * dx);
           keep = d2 * q.w < cull2;
         }
-        bool full = keep;
-        int cxx = 0;
-        if constexpr (REF) {
-          if (valid) cxx = int(g.key[s] - rowbase);
-          full = keep && row_in_all && cxx >= r_ilx && cxx <= r_ihx;
-        }
+        full = keep && (!REF || (cinf.y & 1));
         if constexpr (COUNT) {
-          if (keep) q.z = g.hs[s];
-        }
-        const uint64_t bf = __ballot(full);
-        const uint64_t bm = __ballot(keep && !full);
-        if (lane == 0) {
-          wcnt[slot][wave] = __popcll(bf);
-          wcnt[slot][4 + wave] = __popcll(bm);
-        }
-        __syncthreads();
-        int basef = nf, basem = nm, totf = 0, totm = 0;
-        for (int w = 0; w < 4; ++w) {
-          const int cf = wcnt[slot][w], cm = wcnt[slot][4 + w];
-          if (w < wave) {
-            basef += cf;
-            basem += cm;
-          }
-          totf += cf;
-          totm += cm;
-        }
-        if (full) Lf[basef + prefix_count(bf)] = q;
-        if constexpr (REF) {
-          if (keep && !full) {
-            const int pos = basem + prefix_count(bm);
-            Lm[pos] = q;
-            Lmc[pos] = make_int2(cxx, cy);
-          }
-        }
-        nf += totf;
-        nm += totm;
-        slot ^= 1;
-        if (nf >= kFlush || nm >= kFlush) {
-          __syncthreads();
-          evaluate();
-          __syncthreads();
-          nf = 0;
-          nm = 0;
+          if (keep) q.z = g.hs[sidx];
         }
       }
+      const uint64_t bf = __ballot(full);
+      const uint64_t bm = __ballot(keep && !full);
+      if (lane == 0) {
+        wcnt[slot][wave] = __popcll(bf);
+        wcnt[slot][4 + wave] = __popcll(bm);
+      }
+      __syncthreads();
+      int basef = nf, basem = nm, totf = 0, totm = 0;
+      for (int w = 0; w < 4; ++w) {
+        const int cf = wcnt[slot][w], cm = wcnt[slot][4 + w];
+        if (w < wave) {
+          basef += cf;
+          basem += cm;
+        }
+        totf += cf;
+        totm += cm;
+      }
+      if (full) Lf[basef + prefix_count(bf)] = q;
+      if constexpr (REF) {
+        if (keep && !full) {
+          const int pos = basem + prefix_count(bm);
+          Lm[pos] = q;
+          Lmc[pos] = make_int2(cinf.x, cinf.y >> 1);
+        }
+      }
+      nf += totf;
+      nm += totm;
+      slot ^= 1;
+      if (nf >= kFlush || nm >= kFlush) {
+        __syncthreads();
+        eval_full(nf);
+        eval_masked(nm);
+        __syncthreads();
+        nf = 0;
+        nm = 0;
+      }
     }
+    __syncthreads();  // the cell table is rewritten by the next chunk
   }
   if (nf > 0 || nm > 0) {
     __syncthreads();
-    evaluate();
+    eval_full(nf);
+    eval_masked(nm);
   }
 
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
     if (!pv[k]) continue;
     if constexpr (COUNT) {
-      if (counts) counts[pid[k]] += cnt[k];
+      if (counts) counts[pid[k]] += cnt_acc[k];
     } else {
       acc_out[pid[k]] += a64[k];
     }

@@ -127,16 +127,12 @@ __device__ inline int32_t floor_to_int(float f) {
   return static_cast<int32_t>(fl);
 }
 
-// keys[i] = cell of particle i, vals[i] = i; per-cell counts and max h via
-// atomics (count[c] += 1, maxh_bits[c] = max over the float bit patterns,
-// which order like the floats for h > 0).  Keys outside the grid (the
+// keys[i] = cell of particle i (grid3d_generate_sort_keys,
+// particle_grid.cl:96-112), vals[i] = i.  Keys outside the grid (the
 // reference writes out of bounds there) are counted in *err and clamped.
-__global__ __launch_bounds__(kBlock) void k_keys(const float4* __restrict__ p4,
-                                                  const float* __restrict__ h, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k_keys(const float4* __restrict__ p4, uint32_t n,
                                                   grid_params g, uint32_t* __restrict__ keys,
                                                   uint32_t* __restrict__ vals,
-                                                  uint32_t* __restrict__ count,
-                                                  uint32_t* __restrict__ maxh_bits,
                                                   uint32_t* __restrict__ err) {
 #pragma clang fp contract(off)
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -158,8 +154,6 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float4* __restrict__ p4,
     atomicAdd(err, 1u);
   keys[i] = key;
   vals[i] = i;
-  atomicAdd(&count[key], 1u);
-  atomicMax(&maxh_bits[key], __float_as_uint(h[i]));
 }
 
 constexpr int kItems = 16;
@@ -304,20 +298,52 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
 // Sorted SoA: part[s] = (x, y, coef, 1/h^2) of particle perm[s], hs[s] = h.
 // coef = w * C / h^2 with C the kernel's normalisation, so that the kernel
 // value is coef * shape(r^2 / h^2).
+// The same pass derives the per-cell particle counts and max h
+// (grid3d_determine_cells_begin/_end and grid3d_determine_max_per_cell,
+// particle_grid.cl:114-216): keys are sorted, so the particles of a cell form
+// one run; each wave reduces its part of every run (count, max h) and one
+// lane per run segment adds it with an atomic (a cell spans at most a few
+// waves, so there is no contention).  count and maxh_bits start at 0.
 __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__ p4,
                                                        const float* __restrict__ h,
                                                        const uint32_t* __restrict__ perm,
+                                                       const uint32_t* __restrict__ skeys,
                                                        uint32_t n, float norm,
                                                        float4* __restrict__ part,
-                                                       float* __restrict__ hs) {
+                                                       float* __restrict__ hs,
+                                                       uint32_t* __restrict__ count,
+                                                       uint32_t* __restrict__ maxh_bits) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= n) return;
-  const uint32_t i = perm[s];
-  const float4 q = p4[i];
-  const float hh = h[i];
-  const float ih2 = 1.0f / (hh * hh);
-  part[s] = make_float4(q.x, q.y, q.w * (norm * ih2), ih2);
-  hs[s] = hh;
+  const bool valid = s < n;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t key = 0xffffffffu;
+  float hh = 0.0f;
+  if (valid) {
+    const uint32_t i = perm[s];
+    const float4 q = p4[i];
+    hh = h[i];
+    key = skeys[s];
+    const float ih2 = 1.0f / (hh * hh);
+    part[s] = make_float4(q.x, q.y, q.w * (norm * ih2), ih2);
+    hs[s] = hh;
+  }
+  // run heads within the wave
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const uint64_t heads = __ballot(head);
+  // end (exclusive) of this lane's run inside the wave
+  const uint64_t above = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+  const uint32_t run_end = above ? uint32_t(__builtin_ctzll(above)) : 64u;
+  // segmented max towards the run head
+  float m = hh;
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const float other = __shfl_down(m, o, 64);
+    if (lane + o < run_end) m = fmaxf(m, other);
+  }
+  if (valid && head) {
+    atomicAdd(&count[key], run_end - lane);
+    atomicMax(&maxh_bits[key], __float_as_uint(m));
+  }
 }
 
 // Reference layout of the grid for parity checks.
@@ -355,11 +381,10 @@ void launch_bbox(const float4* p4, const float* h, uint64_t n, float* partial, i
   ILC_HIP(hipGetLastError());
 }
 
-void launch_keys(const float4* p4, const float* h, uint32_t n, const grid_params& g,
-                 uint32_t* keys, uint32_t* vals, uint32_t* count, uint32_t* maxh_bits,
-                 uint32_t* err, hipStream_t s) {
-  hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, n, g, keys,
-                     vals, count, maxh_bits, err);
+void launch_keys(const float4* p4, uint32_t n, const grid_params& g, uint32_t* keys,
+                 uint32_t* vals, uint32_t* err, hipStream_t s) {
+  hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, n, g, keys,
+                     vals, err);
   ILC_HIP(hipGetLastError());
 }
 
@@ -409,11 +434,12 @@ void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits,
   }
 }
 
-void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm, uint32_t n,
-                      float norm, float4* part, float* hs, hipStream_t s) {
+void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
+                      const uint32_t* skeys, uint32_t n, float norm, float4* part, float* hs,
+                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_build_soa, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, perm, n,
-                     norm, part, hs);
+  hipLaunchKernelGGL(k_build_soa, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, perm,
+                     skeys, n, norm, part, hs, count, maxh_bits);
   ILC_HIP(hipGetLastError());
 }
 

@@ -13,17 +13,17 @@ namespace ilc {
 void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_bbox(const float4* p4, const float* h, uint64_t n, float* partial, int nparts,
                  uint32_t* bad, bbox_result* out, hipStream_t s);
-void launch_keys(const float4* p4, const float* h, uint32_t n, const grid_params& g,
-                 uint32_t* keys, uint32_t* vals, uint32_t* count, uint32_t* maxh_bits,
-                 uint32_t* err, hipStream_t s);
+void launch_keys(const float4* p4, uint32_t n, const grid_params& g, uint32_t* keys,
+                 uint32_t* vals, uint32_t* err, hipStream_t s);
 uint64_t scan_temp_words(uint64_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* temp,
                            hipStream_t s);
 uint64_t radix_temp_words(uint64_t n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
                        hipStream_t s);
-void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm, uint32_t n,
-                      float norm, float4* part, float* hs, hipStream_t s);
+void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
+                      const uint32_t* skeys, uint32_t n, float norm, float4* part, float* hs,
+                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s);
 void launch_export_cells(const uint32_t* start, const uint32_t* maxh_bits, uint32_t ncells,
                          int32_t* cells2, float* maxh, hipStream_t s);
 
