@@ -1,23 +1,29 @@
 """Benchmark of the projective SPH smoothing path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1]
 
-A step is one pass of the hot path over the workload's synthetic snapshot
-(already resident in HBM): per view, the per-particle quantity (X-ray
-emissivity or mass), the camera projection with depth mask, the reference
-particle-grid build and the projected-kernel gather, plus the retrieval of
-the image.  The metric is particle-kernel evaluations per second, where an
-evaluation is a (view, pixel, particle) triple with w != 0 and the pixel
-inside the particle's support, counted exactly (independent of culling).
+Workloads (synthetic, seeded; see illcrawl_amd/synth.py):
+  c3 (default)  the batched multi-view workload: the 16M-cell X-ray snapshot
+                (75000 kpc/h box, 8 NFW halos) projected into 64 views of
+                random orientation (16 each at 256^2, 512^2, 1024^2, 2048^2,
+                widths 1000-5000 kpc/h, quantity alternating column density /
+                X-ray), one persistent gather launch per batch;
+  c2            one 2048^2 X-ray view of the same snapshot (plain path);
+  c1            the 2^20-cell column-density view, 512^2 (plain path).
 
-Prints ONE JSON line.  See the "measurement" section of DESIGN.md for the
-roofline and CPU-baseline definitions.
+A step is one pass of the hot path over the workload, with the snapshot
+already resident in HBM: per view the quantity functor, the camera projection
+with depth mask, the reference particle-grid build, and the projected-kernel
+gather, then the image retrieval.  The metric counts particle-kernel
+evaluations: (view, pixel, particle) triples with w != 0 and the pixel inside
+the particle's support, counted exactly on the GPU (independent of culling).
+
+Prints ONE JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -26,8 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "particle-kernel evals/s, SPH projection: batched vs plain on MI355X @1e-5 rel err"
-PEAK_FP32_TFLOPS = 157.3
-PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (the spec figure counts packed FP32)
 FLOP_PER_EVAL = 12
 
 
@@ -35,98 +40,44 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def workload(name):
-    from illcrawl_amd import synth
-    if name == "c1":
-        snap = synth.cached("c1")
-        views = [dict(synth.centred_view(snap.box, width=2000.0, nx=512), quantity=0)]
-        desc = dict(workload="plain projected gas density (column density)", n_cells=snap.n,
-                    box_kpc_h=snap.box, resolution=[512, 512], width_kpc_h=2000.0,
-                    kernel="quartic_ref", mode="reference", views=1)
-    elif name == "c2":
-        snap = synth.cached("c2")
-        views = [dict(synth.centred_view(snap.box, width=5000.0, nx=2048), quantity=1)]
-        desc = dict(workload="X-ray bremsstrahlung projection", n_cells=snap.n,
-                    box_kpc_h=snap.box, halos=8, resolution=[2048, 2048], width_kpc_h=5000.0,
-                    kernel="quartic_ref", mode="reference", views=1, accumulator="fp64")
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    return snap, views, desc
-
-
-class Renderer:
-    """Plain path: one view per launch sequence through the C ABI."""
-
-    def __init__(self, snap, views, mode="reference", tile=None):
-        import torch
-        from illcrawl_amd import backend as B, synth
-        self.B = B
-        self.views = views
-        dev = torch.device("cuda", 0)
-        self.pos4 = torch.empty((snap.n, 4), dtype=torch.float32, device=dev)
-        self.pos4[:, :3] = torch.from_numpy(snap.pos).to(dev)
-        self.mass = torch.from_numpy(snap.mass).to(dev)
-        self.h = torch.from_numpy(snap.hsml).to(dev)
-        self.has_gas = snap.temp is not None
-        if self.has_gas:
-            self.rho = torch.from_numpy(snap.rho).to(dev)
-            self.temp = torch.from_numpy(snap.temp).to(dev)
-            self.xe = torch.from_numpy(snap.xe).to(dev)
-            self.gtab, self.gx0, self.gdx = synth.gaunt_table_1d()
-        self.backend = B.ProjectiveSmoothingGrid(mode=mode, tile=tile)
-        self.backend.init_backend(snap.n)
-        self.pts = [B.camera_pixel_points(v) for v in views]
-        self.gather_ms = []
-        self.images = [None] * len(views)
-
-    def step(self, record=False):
-        import torch
-        B = self.B
-        for k, v in enumerate(self.views):
-            if v.get("quantity", 0) == 1:
-                w = B.quantity_xray_bolometric(self.rho, self.temp, self.xe, self.gtab,
-                                               self.gx0, self.gdx)
-            else:
-                w = B.quantity_scaled(self.mass, 1.0)
-            self.pos4[:, 3] = w
-            proj = B.project_particles(v, self.pos4)
-            self.backend.setup_projected_particles(proj, self.h)
-            self.backend.setup_evaluation_points(self.pts[k], layout_nx=v["nx"])
-            if record:
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record()
-            self.backend.run()
-            if record:
-                e1.record()
-                self.gather_ms.append((e0, e1))
-            self.images[k] = self.backend.retrieve_results(1.0)
-
-    def count_evals(self):
-        B = self.B
-        total, per = 0, []
-        for k, v in enumerate(self.views):
-            w = (B.quantity_xray_bolometric(self.rho, self.temp, self.xe, self.gtab, self.gx0,
-                                            self.gdx) if v.get("quantity", 0) == 1
-                 else B.quantity_scaled(self.mass, 1.0))
-            self.pos4[:, 3] = w
-            proj = B.project_particles(v, self.pos4)
-            self.backend.setup_projected_particles(proj, self.h)
-            self.backend.setup_evaluation_points(self.pts[k], layout_nx=v["nx"])
-            t, pp = self.backend.count_evals(per_point=True)
-            total += t
-            per.append(pp)
-        return total, per
-
-
-def cpu_baseline(snap, view, per_point_counts, seconds=15.0, threads=None):
-    """The oracle (the reference algorithm restated in C, OpenMP over
-    pixels) on a bounded random sample of the view's pixels; evals from the
-    exact per-pixel pair counts of the same pixels."""
+def sampled_parity(snap, view, img_f64, n=1500, seed=7):
+    """Relative L-inf / sum error of the GPU image against the oracle's
+    reference-semantics gather on a random pixel sample (plus the maximum)."""
     import numpy as np
     import oracle
     from illcrawl_amd import synth
-    threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count())
+    cam = oracle.Camera(view["position"], view["look_at"], view["roll"], view["width"],
+                        view["nx"], view["ny"])
+    if view.get("quantity", 0) == 1:
+        tab, x0, dx = synth.gaunt_table_1d()
+        w = oracle.xray_bolometric(snap.rho, snap.temp, snap.xe, tab, x0, dx)
+    else:
+        w = snap.mass
+    pp = cam.project_particles(snap.particles4(w), view["depth"])
+    pts = cam.pixel_points()
+    g = oracle.Grid(pp, snap.hsml, target=1000)
+    img = img_f64.reshape(-1)
+    rng = np.random.default_rng(seed)
+    idx = np.unique(np.concatenate([rng.choice(img.size, min(n, img.size), replace=False),
+                                    [int(np.argmax(img))]]))
+    ref = g.gather_ref(pts[idx], acc_mode=1)
+    ref32 = g.gather_ref(pts[idx], acc_mode=0)
+    m = ref > 1e-6 * ref.max()
+    linf = float(np.max(np.abs(img[idx][m] - ref[m]) / ref[m])) if m.any() else 0.0
+    l32 = float(np.max(np.abs(ref32[m] - ref[m]) / ref[m])) if m.any() else 0.0
+    return dict(pixels=int(idx.size), linf_vs_oracle=linf,
+                oracle_fp32_accumulator_vs_fp64=l32)
+
+
+def cpu_baseline(snap, view, per_point_counts, seconds=15.0):
+    """The oracle (the reference algorithm restated in C, OpenMP over
+    pixels) on a bounded random sample of one view's pixels; evals from the
+    exact per-pixel pair counts of the same pixels (GPU count kernel, which
+    the tests pin to the oracle's count)."""
+    import numpy as np
+    import oracle
+    from illcrawl_amd import synth
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count())
     oracle.set_num_threads(threads)
     cam = oracle.Camera(view["position"], view["look_at"], view["roll"], view["width"],
                         view["nx"], view["ny"])
@@ -153,58 +104,148 @@ def cpu_baseline(snap, view, per_point_counts, seconds=15.0, threads=None):
         done += idx.size
         batch = min(batch * 2, 4096)
     return dict(value=evals / elapsed, unit="evals/s", cores=threads, kind="port",
-                sample=f"{done} random pixels of view 0 ({elapsed:.1f} s of the reference "
-                       f"gather restated in C, FP32 accumulator); grid build "
-                       f"{t_grid:.2f} s single-threaded not included")
+                sample=f"{done} random pixels of one {view['nx']}x{view['ny']} view "
+                       f"({elapsed:.1f} s of the reference gather restated in C, FP32 "
+                       f"accumulator, {threads} OpenMP threads); its grid build "
+                       f"({t_grid:.2f} s, one thread) not included")
+
+
+def single_view_counts(snap, view):
+    """Per-pixel exact eval counts of one view (single-view backend)."""
+    import torch
+    from illcrawl_amd import backend as B, synth
+    if view.get("quantity", 0) == 1:
+        tab, x0, dx = synth.gaunt_table_1d()
+        w = B.quantity_xray_bolometric(snap.rho, snap.temp, snap.xe, tab, x0, dx)
+    else:
+        w = B.quantity_scaled(snap.mass, 1.0)
+    p4 = torch.empty((snap.n, 4), dtype=torch.float32, device="cuda")
+    p4[:, :3] = torch.from_numpy(snap.pos).cuda()
+    p4[:, 3] = w
+    proj = B.project_particles(view, p4)
+    del p4
+    b = B.ProjectiveSmoothingGrid(mode="exact")
+    b.init_backend(snap.n)
+    b.setup_projected_particles(proj, snap.hsml)
+    b.setup_evaluation_points(B.camera_pixel_points(view), layout_nx=view["nx"])
+    total, per = b.count_evals(per_point=True)
+    return total, per
+
+
+def workload(name):
+    from illcrawl_amd import synth
+    if name == "c1":
+        snap = synth.cached("c1")
+        views = [dict(synth.centred_view(snap.box, width=2000.0, nx=512), quantity=0)]
+        desc = dict(workload="plain projected gas density (column density), 1 view",
+                    n_cells=snap.n, box_kpc_h=snap.box, resolution=[512, 512],
+                    width_kpc_h=2000.0)
+    elif name == "c2":
+        snap = synth.cached("c2")
+        views = [dict(synth.centred_view(snap.box, width=5000.0, nx=2048), quantity=1)]
+        desc = dict(workload="X-ray bremsstrahlung projection, 1 view", n_cells=snap.n,
+                    box_kpc_h=snap.box, halos=8, resolution=[2048, 2048], width_kpc_h=5000.0)
+    elif name == "c3":
+        snap = synth.cached("c2")
+        views = synth.random_views(snap.box, n_views=64, seed=2)
+        desc = dict(workload="batched multi-view: 64 random views of the 16M-cell X-ray "
+                             "snapshot, one persistent launch", n_cells=snap.n,
+                    box_kpc_h=snap.box, halos=8, views=64,
+                    resolutions={"256^2": 16, "512^2": 16, "1024^2": 16, "2048^2": 16},
+                    width_kpc_h=[1000.0, 5000.0], quantity="alternating column density / X-ray")
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return snap, views, desc
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default=os.environ.get("ILC_BENCH_WORKLOAD", "c2"))
-    ap.add_argument("--mode", default="reference")
-    ap.add_argument("--tile", default=None)
+    ap.add_argument("--workload", default=os.environ.get("ILC_BENCH_WORKLOAD", "c3"))
+    ap.add_argument("--mode", default="reference", choices=["reference", "exact"])
+    ap.add_argument("--tile", default=None, help="WxH gather tile; default: autotuned")
+    ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
-              f"--gpus {args.gpus}); views would shard across GPUs with no exchange",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(views would shard across GPUs with no exchange; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
     import torch
+    from illcrawl_amd import backend as B, synth
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
 
     t0 = time.time()
     snap, views, desc = workload(args.workload)
-    log(f"[bench] generated {snap.n} cells in {time.time() - t0:.1f}s")
-    tile = tuple(int(x) for x in args.tile.split("x")) if args.tile else None
-    r = Renderer(snap, views, mode=args.mode, tile=tile)
-    total_evals, per_point = r.count_evals()
+    log(f"[bench] {args.workload}: {snap.n} cells, {len(views)} views ({time.time() - t0:.1f}s)")
+    r = B.BatchRenderer(mode=args.mode)
+    r.set_particles(snap.pos, snap.mass, snap.hsml, snap.rho, snap.temp, snap.xe)
+    tab, x0, dx = synth.gaunt_table_1d()
+    r.set_gaunt_table(tab, x0, dx)
+    r.set_views(views)
+    if args.tile:
+        w, h = (int(x) for x in args.tile.split("x"))
+        r.set_tile(w, h)
+        tile, tuned = (w, h), None
+    else:
+        tile, best_ms = r.autotune()
+        tuned = dict(tile=list(tile), best_ms=best_ms)
+        log(f"[bench] autotuned tile {tile} ({best_ms:.1f} ms)")
+    r.render()
+    counts = r.count_evals()
+    total_evals = int(sum(counts))
     log(f"[bench] exact evals per step: {total_evals:.6g}")
     for _ in range(args.warmup):
-        r.step()
+        r.render()
     torch.cuda.synchronize()
-    log("[bench] warmup done")
+
+    # timed: batched (one persistent gather launch per step)
+    gather = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.render()
+        gather.append(r.last_gather_ms())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    gather_ms = float(np.mean(gather))
+    log(f"[bench] batched: {ms_per_step:.2f} ms/step, gather {gather_ms:.2f} ms")
+
+    # hipGraph replays of the same device work
+    r.render(graph=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        r.step(record=True)
+        r.render(graph=True)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    gather = [a.elapsed_time(b) for a, b in r.gather_ms]
-    gather_avg_ms = float(np.mean(gather))
-    log(f"[bench] {args.steps} steps in {dt:.3f}s, gather avg {gather_avg_ms:.2f} ms")
+    graph_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
-    ms_per_step = dt * 1e3 / args.steps
+    # plain: one view per launch sequence
+    plain_ms, plain_gather = None, None
+    if args.plain_steps > 0:
+        r.render(plain=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pg = []
+        for _ in range(args.plain_steps):
+            r.render(plain=True)
+            pg.append(r.last_gather_ms())
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - t0) * 1e3 / args.plain_steps
+        plain_gather = float(np.mean(pg))
+        log(f"[bench] plain: {plain_ms:.2f} ms/step, gathers {plain_gather:.2f} ms")
+        r.render()
+
     value = total_evals / (ms_per_step * 1e-3)
-    evals_per_launch = total_evals / len(views)
-    achieved_tflops = evals_per_launch * FLOP_PER_EVAL / (gather_avg_ms * 1e-3) / 1e12
+    achieved = total_evals * FLOP_PER_EVAL / (gather_ms * 1e-3) / 1e12
     npix = sum(v["nx"] * v["ny"] for v in views)
     line = {
         "metric": METRIC,
@@ -217,20 +258,33 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 accumulation)",
-        "data": "synthetic (seeded generator: uniform + NFW halos, 32-NN density)",
-        "config": dict(desc, path="plain", parallelism="single GPU",
-                       evals_per_step=total_evals, pixels_per_step=npix),
+        "dtype": "f32 (f64 per-pixel accumulation)",
+        "data": "synthetic (seeded: uniform + NFW halos, 32-NN density, lognormal masses)",
+        "config": dict(desc, mode=args.mode, kernel="quartic_ref (the reference's projected "
+                       "quartic)", tile=list(tile), autotune=tuned,
+                       evals_per_step=total_evals, pixels_per_step=npix,
+                       parallelism="single GPU"),
         "pixels_per_s": npix / (ms_per_step * 1e-3),
-        "gather_ms": gather_avg_ms,
-        "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-                     "traffic": None, "kernel": "k_gather (reference mode)",
-                     "flop_per_eval": FLOP_PER_EVAL},
+        "batched_vs_plain": (plain_ms / ms_per_step) if plain_ms else None,
+        "plain": dict(ms_per_step=plain_ms, evals_per_s=(total_evals / (plain_ms * 1e-3))
+                      if plain_ms else None, gather_ms_sum=plain_gather),
+        "graph_replay": dict(ms_per_step=graph_ms,
+                             evals_per_s=total_evals / (graph_ms * 1e-3)),
+        "gather_ms": gather_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_gather_batch (persistent gather, all views)",
+                     "flop_per_eval": FLOP_PER_EVAL, "evals_per_launch": total_evals},
     }
+    big = max(range(len(views)), key=lambda k: views[k]["nx"] * views[k]["ny"])
+    if not args.no_parity:
+        log("[bench] sampled parity vs oracle ...")
+        img = r.image_f64(big).cpu().numpy()
+        line["parity"] = dict(view=big, **sampled_parity(snap, views[big], img))
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(snap, views[0], per_point[0], args.cpu_seconds)
+        _, per = single_view_counts(snap, views[big])
+        line["cpu_baseline"] = cpu_baseline(snap, views[big], per, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

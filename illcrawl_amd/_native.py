@@ -50,6 +50,13 @@ class CameraC(ctypes.Structure):
                 ("ny", ctypes.c_uint32), ("depth", ctypes.c_double)]
 
 
+class ViewC(ctypes.Structure):
+    _fields_ = [("cam", CameraC), ("quantity", ctypes.c_int32), ("scale", ctypes.c_float)]
+
+
+QUANTITY_MASS, QUANTITY_XRAY = 0, 1
+RENDER_PLAIN, RENDER_GRAPH = 1, 2
+
 # name -> (restype, argtypes); every symbol declared in include/illcrawl_psg.h
 _P = ctypes.c_void_p
 _U64 = ctypes.c_uint64
@@ -78,6 +85,20 @@ SIGNATURES = {
     "ilc_quantity_scaled": (_ST, [_P, ctypes.c_double, _U64, _P, _P]),
     "ilc_quantity_xray_bolometric": (_ST, [_P, _P, _P, _U64, _P, ctypes.c_int32, ctypes.c_float,
                                            ctypes.c_float, _P, _P]),
+    "ilc_batch_create": (_ST, [ctypes.POINTER(PsgConfig), ctypes.POINTER(_P)]),
+    "ilc_batch_destroy": (_ST, [_P]),
+    "ilc_batch_set_particles": (_ST, [_P, _P, _P, _P, _P, _P, _P, _U64]),
+    "ilc_batch_set_gaunt_table": (_ST, [_P, _P, ctypes.c_int32, ctypes.c_float, ctypes.c_float]),
+    "ilc_batch_set_views": (_ST, [_P, ctypes.POINTER(ViewC), ctypes.c_int32, _P]),
+    "ilc_batch_render": (_ST, [_P, ctypes.c_int32, _P]),
+    "ilc_batch_image": (_ST, [_P, ctypes.c_int32, _P, _P]),
+    "ilc_batch_image_f64": (_ST, [_P, ctypes.c_int32, _P, _P]),
+    "ilc_batch_count_evals": (_ST, [_P, ctypes.POINTER(_U64), _P]),
+    "ilc_batch_set_tile": (_ST, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "ilc_batch_last_gather_ms": (_ST, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "ilc_batch_autotune": (_ST, [_P, ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.POINTER(ctypes.c_double), _P]),
 }
 
 _lib = None

@@ -237,3 +237,95 @@ class ProjectiveSmoothing:
         img = self.backend.retrieve_results(float(np.float32(scale)), stream)
         return img.reshape(int(self.camera.get("ny", self.camera["nx"])),
                            int(self.camera["nx"]))
+
+
+# ------------------------------------------------------------ batched views
+
+class BatchRenderer:
+    """Many projections of one resident snapshot (ilc_batch_* ABI): one
+    persistent gather over every (view, tile), optionally replayed from a
+    hipGraph; ``render(plain=True)`` runs the one-view-per-launch path."""
+
+    def __init__(self, device=0, mode="reference", kernel="quartic_ref",
+                 target_particles_per_cell=1000, tile=None):
+        cfg = N.PsgConfig()
+        cfg.device = int(device)
+        cfg.target_particles_per_cell = int(target_particles_per_cell)
+        cfg.mode = _MODES[mode]
+        cfg.kernel = _KERNELS[kernel]
+        if tile is not None:
+            cfg.tile_w, cfg.tile_h = int(tile[0]), int(tile[1])
+        self._h = ctypes.c_void_p()
+        N.call("ilc_batch_create", ctypes.byref(cfg), ctypes.byref(self._h))
+        self.device = torch.device("cuda", int(device))
+        self._keep = []
+        self.views = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                N.lib().ilc_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def set_particles(self, pos, mass, hsml, rho=None, temp=None, xe=None):
+        d = self.device
+        pos = _dev_f32(pos, d, 3)
+        arrs = [pos] + [(_dev_f32(a, d).reshape(-1) if a is not None else None)
+                        for a in (mass, hsml, rho, temp, xe)]
+        self._keep = arrs
+        N.call("ilc_batch_set_particles", self._h, *[_ptr(a) for a in arrs], pos.shape[0])
+
+    def set_gaunt_table(self, table, x0, dx):
+        tab = np.ascontiguousarray(table, dtype=np.float32)
+        self._gaunt = tab
+        N.call("ilc_batch_set_gaunt_table", self._h, tab.ctypes.data_as(ctypes.c_void_p),
+               tab.shape[0], ctypes.c_float(x0), ctypes.c_float(dx))
+
+    def set_views(self, views, stream=None):
+        arr = (N.ViewC * len(views))()
+        for k, v in enumerate(views):
+            arr[k].cam = N.camera_struct(v)
+            arr[k].quantity = int(v.get("quantity", 0))
+            arr[k].scale = float(v.get("scale", 1.0))
+        self.views = [dict(v) for v in views]
+        N.call("ilc_batch_set_views", self._h, arr, len(views), _stream(stream))
+
+    def render(self, plain=False, graph=False, stream=None):
+        flags = (N.RENDER_PLAIN if plain else 0) | (N.RENDER_GRAPH if graph else 0)
+        N.call("ilc_batch_render", self._h, flags, _stream(stream))
+
+    def image(self, v, stream=None):
+        cam = self.views[v]
+        out = torch.empty((int(cam.get("ny", cam["nx"])), int(cam["nx"])), dtype=torch.float32,
+                          device=self.device)
+        N.call("ilc_batch_image", self._h, int(v), _ptr(out), _stream(stream))
+        return out
+
+    def image_f64(self, v, stream=None):
+        cam = self.views[v]
+        out = torch.empty((int(cam.get("ny", cam["nx"])), int(cam["nx"])), dtype=torch.float64,
+                          device=self.device)
+        N.call("ilc_batch_image_f64", self._h, int(v), _ptr(out), _stream(stream))
+        return out
+
+    def count_evals(self, stream=None):
+        arr = (ctypes.c_uint64 * len(self.views))()
+        N.call("ilc_batch_count_evals", self._h, arr, _stream(stream))
+        return [int(x) for x in arr]
+
+    def set_tile(self, w, h):
+        N.call("ilc_batch_set_tile", self._h, int(w), int(h))
+
+    def last_gather_ms(self):
+        ms = ctypes.c_float()
+        N.call("ilc_batch_last_gather_ms", self._h, ctypes.byref(ms))
+        return ms.value
+
+    def autotune(self, stream=None):
+        w, h, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        N.call("ilc_batch_autotune", self._h, ctypes.byref(w), ctypes.byref(h), ctypes.byref(ms),
+               _stream(stream))
+        return (w.value, h.value), ms.value

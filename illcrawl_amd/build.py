@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libilcpsg.so")
 
-SOURCES = ["host.cpp", "capi.cpp", "grid.hip", "gather.hip", "project.hip"]
+SOURCES = ["host.cpp", "gridbuild.cpp", "capi.cpp", "batch.cpp", "grid.hip", "gather.hip",
+           "project.hip", "batch.hip"]
 
 ARCH = "gfx950"
 

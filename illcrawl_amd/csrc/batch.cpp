@@ -1,0 +1,580 @@
+// Batched multi-view rendering (C ABI ilc_batch_*).
+//
+// Many projections of one resident snapshot (animation frames, rotated
+// cameras, slabs) in one persistent gather launch.  This replaces the
+// reference's frame loop distributed over MPI ranks (animation.cpp:44-100,
+// uniform_work_partitioner.cpp:36-52) on a single GPU.
+//
+// A render is
+//   1. plan: for every view, one fused pass (quantity, projection, depth
+//      mask) reduces the projected bounding box; one synchronisation; the
+//      host derives each view's reference grid (particle_grid.cpp:43-100);
+//   2. device work: per view, the projected particles and the grid build
+//      (keys, stable radix sort, sorted SoA, cell offsets), all resident;
+//      then ONE persistent gather over every (view, tile) work item, ordered
+//      largest views first, pulled from a device counter; then per-view
+//      retrieval (FP64 -> FP32 times the view's dA scale).
+// Step 2 can be captured in a hipGraph and replayed while the plan is
+// unchanged.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "gridbuild.hpp"
+#include "kernels.hpp"
+
+namespace ilc {
+std::vector<float> cubic_projected_table(int n);
+}
+
+using namespace ilc;
+
+namespace {
+
+struct view_state {
+  ilc_view v{};
+  camera_frame f{};
+  grid_store grid;
+  dbuf<float4> pts;
+  dbuf<double> acc;
+  dbuf<float> img;
+  uint32_t tiles_x = 0, ntiles = 0;
+  grid_params gp{};
+  int32_t kz0 = 0;
+  float max_h = 0.f;
+};
+
+template <class T>
+struct pinned {
+  T* p = nullptr;
+  size_t cap = 0;
+  ~pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+  T* ensure(size_t n) {
+    if (n <= cap && p) return p;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    ILC_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (n ? n : 1)));
+    cap = n;
+    return p;
+  }
+};
+
+}  // namespace
+
+struct ilc_batch {
+  int device = 0;
+  uint64_t target = 1000;
+  int mode = ILC_MODE_REFERENCE;
+  int kernel = ILC_KERNEL_QUARTIC_REF;
+  tile_shape ts{32, 4};
+
+  particle_soa ps{};
+  bool have_particles = false;
+  dbuf<float> gaunt;
+  int gaunt_n = 0;
+  float gx0 = 0.f, gdx = 1.f;
+  dbuf<float> cubic;
+  bool cubic_ready = false;
+
+  std::vector<std::unique_ptr<view_state>> views;
+  grid_scratch scratch;
+  dbuf<float4> p4tmp;
+  dbuf<float> bbox_parts;
+  dbuf<bbox_result> bbox_dev;
+  dbuf<uint32_t> bad;
+  pinned<bbox_result> bbox_host;
+  dbuf<view_desc> desc_dev;
+  pinned<view_desc> desc_host;
+  dbuf<uint2> items_dev;
+  pinned<uint2> items_host;
+  uint32_t nitems = 0;
+  dbuf<uint32_t> counter;  // [0] work counter, [1] out-of-grid particles
+  pinned<uint32_t> flags_host;
+
+  bool planned = false;
+  std::vector<uint64_t> plan_sig;
+  int grid_blocks = 0;
+
+  hipStream_t own = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // timing of the gather launches of the last eager render
+  std::vector<hipEvent_t> tev;
+  int tev_used = 0;
+  float last_gather_ms = 0.f;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+  bool ran_eagerly = false;
+
+  ~ilc_batch() {
+    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (own) (void)hipStreamDestroy(own);
+  }
+};
+
+namespace {
+
+template <class F>
+ilc_status guarded(F&& f) {
+  try {
+    f();
+    set_last_error("");
+    return ILC_OK;
+  } catch (const ilc::error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return ILC_ENOMEM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return ILC_EINVAL;
+  }
+}
+
+struct device_guard {
+  int prev = -1;
+  explicit device_guard(int dev) {
+    ILC_HIP(hipGetDevice(&prev));
+    if (prev != dev) ILC_HIP(hipSetDevice(dev));
+  }
+  ~device_guard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+view_proj proj_of(const ilc_batch* b, const view_state& vs) {
+  view_proj vp;
+  for (int k = 0; k < 9; ++k) vp.m[k] = vs.f.minv[k];
+  for (int k = 0; k < 3; ++k) vp.c[k] = vs.f.center[k];
+  vp.depth = static_cast<float>(vs.v.cam.depth);
+  vp.quantity = vs.v.quantity;
+  vp.scale = 1.0;
+  (void)b;
+  return vp;
+}
+
+gaunt_table gaunt_of(const ilc_batch* b) {
+  return gaunt_table{b->gaunt.p, b->gaunt_n, b->gx0, b->gdx};
+}
+
+void require_ready(const ilc_batch* b) {
+  ILC_REQUIRE(b != nullptr, "null batch");
+  if (!b->have_particles) fail(ILC_ENOTREADY, "no particles set");
+  if (b->views.empty()) fail(ILC_ENOTREADY, "no views set");
+}
+
+void ensure_cubic(ilc_batch* b) {
+  if (b->kernel != ILC_KERNEL_CUBIC || b->cubic_ready) return;
+  std::vector<float> t = cubic_projected_table(4096);
+  b->cubic.ensure(t.size());
+  ILC_HIP(hipMemcpy(b->cubic.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+  b->cubic_ready = true;
+}
+
+// 1. plan: bounding boxes of every view, one sync, grid parameters.
+void plan(ilc_batch* b, hipStream_t s) {
+  const size_t nv = b->views.size();
+  const int nparts = view_bbox_parts(b->ps.n);
+  b->bbox_parts.ensure(nv * size_t(nparts) * 8);
+  b->bbox_dev.ensure(nv);
+  b->bad.ensure(nv);
+  b->bbox_host.ensure(nv);
+  for (size_t v = 0; v < nv; ++v)
+    launch_view_bbox(b->ps, proj_of(b, *b->views[v]), gaunt_of(b),
+                     b->bbox_parts.p + v * size_t(nparts) * 8, b->bad.p + v, b->bbox_dev.p + v, s);
+  ILC_HIP(hipMemcpyAsync(b->bbox_host.p, b->bbox_dev.p, sizeof(bbox_result) * nv,
+                         hipMemcpyDeviceToHost, s));
+  ILC_HIP(hipStreamSynchronize(s));
+  std::vector<uint64_t> sig;
+  for (size_t v = 0; v < nv; ++v) {
+    view_state& vs = *b->views[v];
+    vs.gp = plan_grid(b->bbox_host.p[v], b->ps.n, b->target, b->mode, &vs.kz0);
+    vs.max_h = b->bbox_host.p[v].max_h;
+    uint32_t bits[10];
+    std::memcpy(bits, vs.gp.nc, 12);
+    std::memcpy(bits + 3, vs.gp.minc, 12);
+    std::memcpy(bits + 6, vs.gp.cs, 12);
+    std::memcpy(bits + 9, &vs.max_h, 4);
+    for (uint32_t x : bits) sig.push_back(x);
+    sig.push_back(uint64_t(vs.kz0));
+  }
+  sig.push_back(uint64_t(b->ts.tw) << 32 | uint64_t(b->ts.ppt));
+  // work items: largest views first, tiles in image order
+  std::vector<uint32_t> order(nv);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+    const uint64_t pa = uint64_t(b->views[a]->v.cam.nx) * b->views[a]->v.cam.ny;
+    const uint64_t pc = uint64_t(b->views[c]->v.cam.nx) * b->views[c]->v.cam.ny;
+    return pa > pc;
+  });
+  uint64_t nitems = 0;
+  for (size_t v = 0; v < nv; ++v) {
+    view_state& vs = *b->views[v];
+    tile_grid(b->ts, vs.v.cam.nx, vs.v.cam.ny, &vs.tiles_x, &vs.ntiles);
+    nitems += vs.ntiles;
+  }
+  ILC_REQUIRE(nitems < (uint64_t(1) << 32), "too many work items");
+  uint2* it = b->items_host.ensure(nitems);
+  uint64_t k = 0;
+  for (uint32_t v : order)
+    for (uint32_t t = 0; t < b->views[v]->ntiles; ++t) it[k++] = make_uint2(v, t);
+  b->nitems = static_cast<uint32_t>(nitems);
+  b->items_dev.ensure(nitems);
+  b->desc_dev.ensure(nv);
+  b->desc_host.ensure(nv);
+  b->counter.ensure(2);
+  b->flags_host.ensure(2);
+  const bool same = b->planned && sig == b->plan_sig;
+  if (!same) b->graph_valid = false;
+  b->plan_sig = sig;
+  b->planned = true;
+  b->grid_blocks = gather_batch_grid(b->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT,
+                                     b->kernel, b->ts);
+}
+
+// 2. device work of a planned render, persistent (one gather launch over
+// all views) or plain (one gather launch per view).
+hipEvent_t timing_event(ilc_batch* b) {
+  if (b->tev_used == int(b->tev.size())) {
+    hipEvent_t e;
+    ILC_HIP(hipEventCreate(&e));
+    b->tev.push_back(e);
+  }
+  return b->tev[b->tev_used++];
+}
+
+void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = false) {
+  const size_t nv = b->views.size();
+  b->tev_used = 0;
+  const int op = b->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT;
+  b->p4tmp.ensure(b->ps.n);
+  launch_fill_u32(b->counter.p, 0u, 2, s);
+  for (size_t v = 0; v < nv; ++v) {
+    view_state& vs = *b->views[v];
+    launch_view_project(b->ps, proj_of(b, vs), gaunt_of(b), b->p4tmp.p, s);
+    enqueue_grid_build(vs.grid, b->scratch, b->p4tmp.p, b->ps.hsml, b->ps.n, vs.gp, vs.kz0,
+                       vs.max_h, b->kernel, b->counter.p + 1, s);
+    const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
+    launch_fill_f64(vs.acc.p, 0.0, P, s);
+    view_desc& d = b->desc_host.p[v];
+    d.g = vs.grid.view();
+    d.p = point_view{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
+    d.acc = vs.acc.p;
+    d.tiles_x = vs.tiles_x;
+    d.ntiles = vs.ntiles;
+    if (!persistent) {
+      if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
+      launch_gather(d.g, d.p, op, b->kernel, b->ts, vs.acc.p, nullptr, b->cubic.p, s);
+      if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
+      launch_retrieve(vs.acc.p, P, vs.v.scale, vs.img.p, s);
+    }
+  }
+  if (persistent) {
+    ILC_HIP(hipMemcpyAsync(b->desc_dev.p, b->desc_host.p, sizeof(view_desc) * nv,
+                           hipMemcpyHostToDevice, s));
+    ILC_HIP(hipMemcpyAsync(b->items_dev.p, b->items_host.p, sizeof(uint2) * b->nitems,
+                           hipMemcpyHostToDevice, s));
+    if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
+    launch_gather_batch(b->desc_dev.p, b->items_dev.p, b->nitems, b->counter.p, op, b->kernel,
+                        b->ts, b->cubic.p, b->grid_blocks, s);
+    if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
+    for (size_t v = 0; v < nv; ++v) {
+      view_state& vs = *b->views[v];
+      launch_retrieve(vs.acc.p, uint64_t(vs.v.cam.nx) * vs.v.cam.ny, vs.v.scale, vs.img.p, s);
+    }
+  }
+  ILC_HIP(hipMemcpyAsync(b->flags_host.p, b->counter.p, sizeof(uint32_t) * 2,
+                         hipMemcpyDeviceToHost, s));
+}
+
+// sum of the timed gather launches (pairs of events) of the last eager render
+void collect_timing(ilc_batch* b) {
+  float tot = 0.f;
+  for (int k = 0; k + 1 < b->tev_used; k += 2) {
+    float ms = 0.f;
+    ILC_HIP(hipEventElapsedTime(&ms, b->tev[k], b->tev[k + 1]));
+    tot += ms;
+  }
+  b->last_gather_ms = tot;
+}
+
+void check_flags(ilc_batch* b) {
+  if (b->flags_host.p[1])
+    fail(ILC_ERANGE, "%u particles fall outside their view's grid", b->flags_host.p[1]);
+}
+
+}  // namespace
+
+extern "C" {
+
+ilc_status ilc_batch_create(const ilc_psg_config* cfg, ilc_batch** out) {
+  return guarded([&] {
+    ILC_REQUIRE(out != nullptr, "out is null");
+    *out = nullptr;
+    ilc_psg_config c{};
+    if (cfg) c = *cfg;
+    ILC_REQUIRE(c.mode == ILC_MODE_REFERENCE || c.mode == ILC_MODE_EXACT, "bad mode %d", c.mode);
+    ILC_REQUIRE(c.kernel >= ILC_KERNEL_QUARTIC_REF && c.kernel <= ILC_KERNEL_CUBIC,
+                "bad kernel %d", c.kernel);
+    ILC_REQUIRE(!(c.mode == ILC_MODE_REFERENCE && c.kernel == ILC_KERNEL_CUBIC),
+                "reference mode uses a quartic kernel");
+    int ndev = 0;
+    ILC_HIP(hipGetDeviceCount(&ndev));
+    ILC_REQUIRE(c.device >= 0 && c.device < ndev, "device %d not present", c.device);
+    std::unique_ptr<ilc_batch> b(new ilc_batch());
+    b->device = c.device;
+    b->target = c.target_particles_per_cell ? c.target_particles_per_cell : 1000;
+    b->mode = c.mode;
+    b->kernel = c.kernel;
+    if (c.tile_w || c.tile_h) {
+      ILC_REQUIRE(c.tile_w > 0 && c.tile_h > 0 && (c.tile_h * c.tile_w) % kBlock == 0,
+                  "tile %dx%d is not a multiple of %d points", c.tile_w, c.tile_h, kBlock);
+      b->ts = tile_shape{c.tile_w, c.tile_w * c.tile_h / kBlock};
+    }
+    device_guard dg(b->device);
+    ILC_HIP(hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking));
+    ILC_HIP(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming));
+    ILC_HIP(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming));
+    *out = b.release();
+  });
+}
+
+ilc_status ilc_batch_destroy(ilc_batch* b) {
+  return guarded([&] {
+    if (!b) return;
+    device_guard dg(b->device);
+    (void)hipStreamSynchronize(b->own);
+    delete b;
+  });
+}
+
+ilc_status ilc_batch_set_particles(ilc_batch* b, const float* pos3_dev, const float* mass_dev,
+                                   const float* hsml_dev, const float* rho_dev,
+                                   const float* temp_dev, const float* xe_dev, uint64_t n) {
+  return guarded([&] {
+    ILC_REQUIRE(b != nullptr, "null batch");
+    ILC_REQUIRE(n > 0 && n < (uint64_t(1) << 31), "particle count %llu not in [1, 2^31)",
+                (unsigned long long)n);
+    ILC_REQUIRE(pos3_dev && hsml_dev, "positions and smoothing lengths are required");
+    b->ps = particle_soa{pos3_dev, mass_dev, hsml_dev, rho_dev, temp_dev, xe_dev, n};
+    b->have_particles = true;
+    b->graph_valid = false;
+    b->planned = false;
+  });
+}
+
+ilc_status ilc_batch_set_gaunt_table(ilc_batch* b, const float* table_host, int32_t ntab,
+                                     float x0, float dx) {
+  return guarded([&] {
+    ILC_REQUIRE(b && table_host, "null argument");
+    ILC_REQUIRE(ntab >= 1 && ntab <= 4096, "table size %d not in [1, 4096]", ntab);
+    ILC_REQUIRE(dx > 0.0f, "table spacing must be positive");
+    device_guard dg(b->device);
+    b->gaunt.ensure(size_t(ntab));
+    ILC_HIP(hipMemcpy(b->gaunt.p, table_host, sizeof(float) * ntab, hipMemcpyHostToDevice));
+    b->gaunt_n = ntab;
+    b->gx0 = x0;
+    b->gdx = dx;
+    b->graph_valid = false;
+  });
+}
+
+ilc_status ilc_batch_set_views(ilc_batch* b, const ilc_view* views, int32_t nviews,
+                               void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(b && views && nviews > 0, "null or empty view list");
+    device_guard dg(b->device);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    std::vector<std::unique_ptr<view_state>> vs;
+    for (int32_t k = 0; k < nviews; ++k) {
+      const ilc_view& v = views[k];
+      ILC_REQUIRE(v.quantity == ILC_QUANTITY_MASS || v.quantity == ILC_QUANTITY_XRAY,
+                  "view %d: bad quantity %d", k, v.quantity);
+      std::unique_ptr<view_state> st(new view_state());
+      st->v = v;
+      st->f = make_camera_frame(v.cam);
+      const uint64_t P = uint64_t(v.cam.nx) * v.cam.ny;
+      ILC_REQUIRE(P < (uint64_t(1) << 32), "view %d: too many pixels", k);
+      st->pts.ensure(P);
+      st->acc.ensure(P);
+      st->img.ensure(P);
+      launch_pixel_points(st->f, v.cam.nx, v.cam.ny, st->pts.p, s);
+      vs.push_back(std::move(st));
+    }
+    b->views = std::move(vs);
+    b->planned = false;
+    b->graph_valid = false;
+    b->ran_eagerly = false;
+    ILC_HIP(hipStreamSynchronize(s));
+  });
+}
+
+ilc_status ilc_batch_render(ilc_batch* b, int32_t flags, void* stream) {
+  return guarded([&] {
+    require_ready(b);
+    device_guard dg(b->device);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const bool persistent = (flags & ILC_RENDER_PLAIN) == 0;
+    const bool use_graph = persistent && (flags & ILC_RENDER_GRAPH);
+    for (auto& v : b->views)
+      if (v->v.quantity == ILC_QUANTITY_XRAY && !b->gaunt_n)
+        fail(ILC_ENOTREADY, "X-ray view without a Gaunt table");
+    ensure_cubic(b);
+    plan(b, s);
+    if (!use_graph || !b->ran_eagerly) {
+      // eager (also sizes every buffer before a later capture)
+      device_work(b, persistent, s, true);
+      ILC_HIP(hipStreamSynchronize(s));
+      collect_timing(b);
+      if (persistent) b->ran_eagerly = true;
+      check_flags(b);
+      return;
+    }
+    if (!b->graph_valid) {
+      if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->gexec = nullptr;
+      b->graph = nullptr;
+      ILC_HIP(hipStreamBeginCapture(b->own, hipStreamCaptureModeThreadLocal));
+      try {
+        device_work(b, true, b->own);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b->own, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      ILC_HIP(hipStreamEndCapture(b->own, &b->graph));
+      ILC_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      b->graph_valid = true;
+    }
+    ILC_HIP(hipEventRecord(b->ev_in, s));
+    ILC_HIP(hipStreamWaitEvent(b->own, b->ev_in, 0));
+    ILC_HIP(hipGraphLaunch(b->gexec, b->own));
+    ILC_HIP(hipEventRecord(b->ev_out, b->own));
+    ILC_HIP(hipStreamWaitEvent(s, b->ev_out, 0));
+    ILC_HIP(hipStreamSynchronize(s));
+    check_flags(b);
+  });
+}
+
+ilc_status ilc_batch_image(const ilc_batch* b, int32_t view, float* out_dev, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(b && out_dev, "null argument");
+    ILC_REQUIRE(view >= 0 && size_t(view) < b->views.size(), "view %d out of range", view);
+    device_guard dg(b->device);
+    const view_state& vs = *b->views[view];
+    ILC_HIP(hipMemcpyAsync(out_dev, vs.img.p, sizeof(float) * vs.v.cam.nx * vs.v.cam.ny,
+                           hipMemcpyDeviceToDevice, reinterpret_cast<hipStream_t>(stream)));
+  });
+}
+
+ilc_status ilc_batch_image_f64(const ilc_batch* b, int32_t view, double* out_dev, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(b && out_dev, "null argument");
+    ILC_REQUIRE(view >= 0 && size_t(view) < b->views.size(), "view %d out of range", view);
+    device_guard dg(b->device);
+    const view_state& vs = *b->views[view];
+    ILC_HIP(hipMemcpyAsync(out_dev, vs.acc.p, sizeof(double) * vs.v.cam.nx * vs.v.cam.ny,
+                           hipMemcpyDeviceToDevice, reinterpret_cast<hipStream_t>(stream)));
+  });
+}
+
+ilc_status ilc_batch_count_evals(ilc_batch* b, uint64_t* per_view_host, void* stream) {
+  return guarded([&] {
+    require_ready(b);
+    ILC_REQUIRE(per_view_host, "null output");
+    device_guard dg(b->device);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (!b->ran_eagerly) fail(ILC_ENOTREADY, "render once before counting");
+    dbuf<unsigned long long> cnt, tmp;
+    tmp.ensure(1);
+    for (size_t v = 0; v < b->views.size(); ++v) {
+      view_state& vs = *b->views[v];
+      const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
+      cnt.ensure(P);
+      launch_fill_u32(reinterpret_cast<uint32_t*>(cnt.p), 0u, 2 * P, s);
+      point_view p{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
+      launch_gather(vs.grid.view(), p, OP_COUNT, b->kernel, b->ts, nullptr, cnt.p, nullptr, s);
+      per_view_host[v] = launch_sum_u64(cnt.p, P, tmp.p, s);
+    }
+  });
+}
+
+ilc_status ilc_batch_last_gather_ms(const ilc_batch* b, float* ms) {
+  return guarded([&] {
+    ILC_REQUIRE(b && ms, "null argument");
+    *ms = b->last_gather_ms;
+  });
+}
+
+ilc_status ilc_batch_set_tile(ilc_batch* b, int32_t tile_w, int32_t tile_h) {
+  return guarded([&] {
+    ILC_REQUIRE(b, "null batch");
+    ILC_REQUIRE(tile_w > 0 && tile_h > 0 && (tile_w * tile_h) % kBlock == 0,
+                "tile %dx%d is not a multiple of %d points", tile_w, tile_h, kBlock);
+    tile_shape ts{tile_w, tile_w * tile_h / kBlock};
+    (void)tile_grid;  // validated on use
+    b->ts = ts;
+    b->graph_valid = false;
+    b->ran_eagerly = false;
+  });
+}
+
+ilc_status ilc_batch_autotune(ilc_batch* b, int32_t* tile_w, int32_t* tile_h, double* best_ms,
+                              void* stream) {
+  return guarded([&] {
+    require_ready(b);
+    device_guard dg(b->device);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    static const tile_shape cand[] = {{16, 1}, {32, 2}, {32, 4}, {64, 4}};
+    hipEvent_t e0, e1;
+    ILC_HIP(hipEventCreate(&e0));
+    ILC_HIP(hipEventCreate(&e1));
+    tile_shape best = b->ts;
+    float best_t = INFINITY;
+    for (const tile_shape& ts : cand) {
+      b->ts = ts;
+      b->graph_valid = false;
+      plan(b, s);
+      device_work(b, true, s);  // warm (allocations, caches)
+      ILC_HIP(hipEventRecord(e0, s));
+      device_work(b, true, s);
+      ILC_HIP(hipEventRecord(e1, s));
+      ILC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      ILC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_t) {
+        best_t = ms;
+        best = ts;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->ts = best;
+    b->graph_valid = false;
+    b->ran_eagerly = false;
+    if (tile_w) *tile_w = best.tw;
+    if (tile_h) *tile_h = (kBlock / best.tw) * best.ppt;
+    if (best_ms) *best_ms = best_t;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+// Per-view particle pass of the batched renderer: the quantity functor
+// (mass, or bolometric X-ray emissivity from a Gaunt table in LDS), the
+// camera projection with depth mask, and either bounding-box partials or
+// the projected float4 particles.
+//
+// Fuses, per view and in one read of the resident snapshot, what the
+// reference does in the quantity transformation (quantity.cpp:84-131,
+// quantities.cl:97-136, 701-726), projective_smoothing::setup_particles
+// (projective_smoothing_reconstruction.cpp:98-114, projection.cl:56-95) and
+// the bounding-box scan of the grid constructor (particle_grid.cpp:43-67).
+// The FP32 operation order is the one of project.hip, so the projected
+// particles are bit-identical to the single-view path's.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace ilc {
+
+namespace {
+
+constexpr int kMaxGaunt = 4096;
+
+__device__ inline float dot3(const float* r, float x, float y, float z) {
+#pragma clang fp contract(off)
+  const float a = r[0] * x;
+  const float b = r[1] * y;
+  const float s = a + b;
+  const float d = r[2] * z;
+  return s + d;
+}
+
+__device__ inline float sample_1d(const float* tab, int n, float x0, float dx, float x) {
+#pragma clang fp contract(off)
+  const float d = x - x0;
+  const float u = d / dx;
+  const float us = u - 0.5f;
+  const float fl = floorf(us);
+  const float a = us - fl;
+  int i0 = 0;
+  if (fl >= 2147483647.0f) i0 = INT32_MAX;
+  else if (fl <= -2147483648.0f) i0 = INT32_MIN;
+  else if (fl == fl) i0 = int(fl);
+  const float t0 = (i0 >= 0 && i0 < n) ? tab[i0] : 0.0f;
+  const float t1 = (i0 >= -1 && i0 < n - 1) ? tab[i0 + 1] : 0.0f;
+  const float wa = 1.0f - a;
+  const float p0 = wa * t0;
+  const float p1 = a * t1;
+  return p0 + p1;
+}
+
+__device__ inline float wmin(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline float wmax(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// w and camera-frame (x, y, z) of particle i
+__device__ inline void particle_of(const particle_soa& ps, const view_proj& vp, const float* tab,
+                                   int ntab, float gx0, float gdx, uint64_t i, float* out,
+                                   float* w) {
+#pragma clang fp contract(off)
+  const float x = ps.pos3[3 * i], y = ps.pos3[3 * i + 1], z = ps.pos3[3 * i + 2];
+  if (vp.quantity == 1) {
+    const float T = ps.temp[i];
+    float ion = 4.046263f * ps.rho[i];
+    ion = ion * ps.xe[i];
+    const float g = sample_1d(tab, ntab, gx0, gdx, logf(T));
+    float e = ion * ion;
+    e = e * sqrtf(T);
+    *w = e * g;
+  } else {
+    *w = static_cast<float>(vp.scale * static_cast<double>(ps.mass[i]));
+  }
+  const float vx = x - vp.c[0], vy = y - vp.c[1], vz = z - vp.c[2];
+  out[0] = dot3(vp.m + 0, vx, vy, vz);
+  out[1] = dot3(vp.m + 3, vx, vy, vz);
+  out[2] = dot3(vp.m + 6, vx, vy, vz);
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void k_view_pass(particle_soa ps, view_proj vp,
+                                                       gaunt_table gt, float4* __restrict__ out,
+                                                       float* __restrict__ part,
+                                                       uint32_t* __restrict__ bad) {
+  __shared__ float tab[kMaxGaunt];
+  if (vp.quantity == 1) {
+    for (int i = threadIdx.x; i < gt.n; i += kBlock) tab[i] = gt.y[i];
+    __syncthreads();
+  }
+  float mn0 = INFINITY, mn1 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY, mh = 0.0f;
+  uint32_t nbad = 0;
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < ps.n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    float pc[3], w;
+    particle_of(ps, vp, tab, gt.n, gt.x0, gt.dx, i, pc, &w);
+    if (pc[2] < 0.0f || pc[2] > vp.depth) w = 0.0f;
+    if constexpr (WRITE) {
+      out[i] = make_float4(pc[0], pc[1], 0.0f, w);
+    } else {
+      const float hh = ps.hsml[i];
+      const bool ok = isfinite(pc[0]) && isfinite(pc[1]) && isfinite(w) && hh >= 1.1e-19f &&
+                      hh <= 1.8e19f;
+      if (!ok) {
+        ++nbad;
+        continue;
+      }
+      mn0 = fminf(mn0, pc[0]);
+      mx0 = fmaxf(mx0, pc[0]);
+      mn1 = fminf(mn1, pc[1]);
+      mx1 = fmaxf(mx1, pc[1]);
+      mh = fmaxf(mh, hh);
+    }
+  }
+  if constexpr (!WRITE) {
+    __shared__ float red[4][5];
+    __shared__ uint32_t redb[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float v[5] = {wmin(mn0), wmin(mn1), wmax(mx0), wmax(mx1), wmax(mh)};
+    uint32_t b = nbad;
+    for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o, 64);
+    if (lane == 0) {
+      for (int k = 0; k < 5; ++k) red[wave][k] = v[k];
+      redb[wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float r[5];
+      for (int k = 0; k < 5; ++k) r[k] = red[0][k];
+      uint32_t bb = redb[0];
+      for (int w = 1; w < 4; ++w) {
+        r[0] = fminf(r[0], red[w][0]);
+        r[1] = fminf(r[1], red[w][1]);
+        r[2] = fmaxf(r[2], red[w][2]);
+        r[3] = fmaxf(r[3], red[w][3]);
+        r[4] = fmaxf(r[4], red[w][4]);
+        bb += redb[w];
+      }
+      // layout of the bbox partials: min x, y, z, max x, y, z, max h
+      float* o = part + blockIdx.x * 8;
+      o[0] = r[0];
+      o[1] = r[1];
+      o[2] = 0.0f;
+      o[3] = r[2];
+      o[4] = r[3];
+      o[5] = 0.0f;
+      o[6] = r[4];
+      if (bb) atomicAdd(bad, bb);
+    }
+  }
+}
+
+}  // namespace
+
+int view_bbox_parts(uint64_t n) {
+  return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(1024, (n + kBlock - 1) / kBlock)));
+}
+
+void launch_view_bbox(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
+                      float* partial, uint32_t* bad, bbox_result* out, hipStream_t s) {
+  if (vp.quantity == 1 && (gt.n < 1 || gt.n > kMaxGaunt || !gt.y))
+    fail(ILC_EINVAL, "X-ray view without a Gaunt table (1..%d entries)", kMaxGaunt);
+  const int nparts = view_bbox_parts(ps.n);
+  launch_fill_u32(bad, 0u, 1, s);
+  hipLaunchKernelGGL(k_view_pass<false>, dim3(nparts), dim3(kBlock), 0, s, ps, vp, gt, nullptr,
+                     partial, bad);
+  ILC_HIP(hipGetLastError());
+  launch_bbox_final(partial, nparts, bad, out, s);
+}
+
+void launch_view_project(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
+                         float4* out, hipStream_t s) {
+  uint64_t nb = (ps.n + kBlock - 1) / kBlock;
+  if (nb > 16384) nb = 16384;
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL(k_view_pass<true>, dim3(uint32_t(nb)), dim3(kBlock), 0, s, ps, vp, gt, out,
+                     nullptr, nullptr);
+  ILC_HIP(hipGetLastError());
+}
+
+}  // namespace ilc

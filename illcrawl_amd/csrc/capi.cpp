@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gridbuild.hpp"
 #include "kernels.hpp"
 
 namespace ilc {
@@ -26,17 +27,8 @@ struct ilc_psg {
   tile_shape ts{32, 4};
   uint64_t blocksize = 0;
 
-  // grid
-  bool have_grid = false;
-  grid_params gp{};
-  float max_h = 0.f, cell_radius = 0.f;
-  int32_t kz0 = 0;
-  uint64_t n = 0;
-  dbuf<float4> part;
-  dbuf<float> hs;
-  dbuf<uint32_t> keys, vals, start, counts, maxh_bits, temp, flags;
-  dbuf<float> bbox_partial;
-  dbuf<bbox_result> bbox_dev;
+  grid_store grid;
+  grid_scratch scratch;
   bbox_result* bbox_host = nullptr;
   uint32_t* flags_host = nullptr;
 
@@ -89,25 +81,6 @@ struct device_guard {
     if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
   }
 };
-
-int bits_for(uint64_t maxval) {
-  int b = 0;
-  while (b < 64 && (maxval >> b)) ++b;
-  return b;
-}
-
-float kernel_norm(int kernel) {
-  switch (kernel) {
-    case ILC_KERNEL_QUARTIC_REF:
-      // 2 * norm * integral of smoothing_kernel.cl:57-60, with the h powers
-      // folded into 1/h^2 (h^5 of the s2^2.5 factor over h^3 * h^4)
-      return static_cast<float>(2.0 * double(1.04445f) * double(1.066667f));
-    case ILC_KERNEL_QUARTIC:
-      return static_cast<float>(3.5 / M_PI);
-    default:
-      return 1.0f;
-  }
-}
 
 void check_handle(const ilc_psg* h) { ILC_REQUIRE(h != nullptr, "null handle"); }
 
@@ -184,69 +157,34 @@ ilc_status ilc_psg_setup_projected_particles(ilc_psg* h, const float* particles4
     ILC_REQUIRE(particles4_dev && hsml_dev, "null particle or smoothing-length pointer");
     device_guard dg(h->device);
     hipStream_t s = as_stream(stream);
-    h->have_grid = false;
     const float4* p4 = reinterpret_cast<const float4*>(particles4_dev);
     if (!h->bbox_host) {
       ILC_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->bbox_host), sizeof(bbox_result)));
       ILC_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->flags_host), 2 * sizeof(uint32_t)));
     }
+    grid_scratch& sc = h->scratch;
     // 1. bounding box (the host scan of particle_grid.cpp:43-67)
-    int nparts = static_cast<int>(std::min<uint64_t>(1024, (n + kBlock - 1) / kBlock));
-    h->bbox_partial.ensure(size_t(nparts) * 8);
-    h->bbox_dev.ensure(1);
-    h->flags.ensure(2);
-    launch_bbox(p4, hsml_dev, n, h->bbox_partial.p, nparts, h->flags.p, h->bbox_dev.p, s);
-    ILC_HIP(hipMemcpyAsync(h->bbox_host, h->bbox_dev.p, sizeof(bbox_result),
+    const int nparts = static_cast<int>(std::min<uint64_t>(1024, (n + kBlock - 1) / kBlock));
+    sc.bbox_partial.ensure(size_t(nparts) * 8);
+    sc.bbox_dev.ensure(1);
+    sc.flags.ensure(2);
+    launch_bbox(p4, hsml_dev, n, sc.bbox_partial.p, nparts, sc.flags.p, sc.bbox_dev.p, s);
+    ILC_HIP(hipMemcpyAsync(h->bbox_host, sc.bbox_dev.p, sizeof(bbox_result),
                            hipMemcpyDeviceToHost, s));
     ILC_HIP(hipStreamSynchronize(s));
     const bbox_result bb = *h->bbox_host;
-    if (bb.bad)
-      fail(ILC_EINVAL, "%u particles have non-finite coordinates/w or h outside [1.1e-19, 1.8e19]",
-           bb.bad);
     // 2. grid parameters (host, double, particle_grid.cpp:69-100)
-    const grid_params g = compute_grid_params(bb, n, h->target);
-    const uint64_t ncells = g.ncells();
     int32_t kz0 = 0;
-    {
-      const float zlo = (bb.mn[2] - g.minc[2]) / g.cs[2];
-      const float zhi = (bb.mx[2] - g.minc[2]) / g.cs[2];
-      kz0 = static_cast<int32_t>(std::floor(zlo));
-      if (h->mode == ILC_MODE_REFERENCE && std::floor(zlo) != std::floor(zhi))
-        fail(ILC_EINVAL,
-             "reference mode needs projected particles (one z layer); z spans [%g, %g]",
-             double(bb.mn[2]), double(bb.mx[2]));
-    }
-    // 3. keys and the stable sort of (key, index)
-    h->keys.ensure(n);
-    h->vals.ensure(n);
-    h->counts.ensure(ncells + 1);
-    h->start.ensure(ncells + 1);
-    h->maxh_bits.ensure(ncells);
-    launch_fill_u32(h->flags.p, 0u, 1, s);
-    launch_keys(p4, uint32_t(n), g, h->keys.p, h->vals.p, h->flags.p, s);
-    h->temp.ensure(std::max(radix_temp_words(n), scan_temp_words(ncells + 1)));
-    launch_radix_sort(h->keys.p, h->vals.p, uint32_t(n), bits_for(ncells - 1), h->temp.p, s);
-    // 4. sorted SoA, per-cell counts and max h, cell offsets
-    launch_fill_u32(h->counts.p, 0u, ncells + 1, s);
-    launch_fill_u32(h->maxh_bits.p, 0u, ncells, s);
-    h->part.ensure(n);
-    h->hs.ensure(n);
-    launch_build_soa(p4, hsml_dev, h->vals.p, h->keys.p, uint32_t(n), kernel_norm(h->kernel),
-                     h->part.p, h->hs.p, h->counts.p, h->maxh_bits.p, s);
-    launch_exclusive_scan(h->counts.p, h->start.p, ncells + 1, h->temp.p, s);
-    ILC_HIP(hipMemcpyAsync(h->flags_host, h->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    const grid_params g = plan_grid(bb, n, h->target, h->mode, &kz0);
+    // 3. keys, stable sort, SoA, counts, max h, offsets
+    launch_fill_u32(sc.flags.p, 0u, 1, s);
+    enqueue_grid_build(h->grid, sc, p4, hsml_dev, n, g, kz0, bb.max_h, h->kernel, sc.flags.p, s);
+    ILC_HIP(hipMemcpyAsync(h->flags_host, sc.flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     ILC_HIP(hipStreamSynchronize(s));
-    if (h->flags_host[0])
+    if (h->flags_host[0]) {
+      h->grid.ready = false;
       fail(ILC_ERANGE, "%u particles fall outside the grid", h->flags_host[0]);
-    h->gp = g;
-    h->n = n;
-    h->max_h = bb.max_h;
-    float dd = g.cs[0] * g.cs[0];
-    dd = dd + g.cs[1] * g.cs[1];
-    dd = dd + g.cs[2] * g.cs[2];
-    h->cell_radius = 0.5f * std::sqrt(dd);
-    h->kz0 = kz0;
-    h->have_grid = true;
+    }
   });
 }
 
@@ -275,34 +213,15 @@ static void ensure_cubic(ilc_psg* h) {
   h->cubic_ready = true;
 }
 
-static grid_view view_of(const ilc_psg* h) {
-  grid_view g;
-  g.part = h->part.p;
-  g.key = h->keys.p;
-  g.hs = h->hs.p;
-  g.start = h->start.p;
-  g.maxh_bits = h->maxh_bits.p;
-  for (int a = 0; a < 3; ++a) {
-    g.nc[a] = h->gp.nc[a];
-    g.minc[a] = h->gp.minc[a];
-    g.cs[a] = h->gp.cs[a];
-  }
-  g.max_h = h->max_h;
-  g.cell_radius = h->cell_radius;
-  g.kz0 = h->kz0;
-  g.ncells = static_cast<uint32_t>(h->gp.ncells());
-  return g;
-}
-
 ilc_status ilc_psg_run(ilc_psg* h, void* stream) {
   return guarded([&] {
     check_handle(h);
-    if (!h->have_grid) fail(ILC_ENOTREADY, "run before setup_projected_particles");
+    if (!h->grid.ready) fail(ILC_ENOTREADY, "run before setup_projected_particles");
     if (!h->pts) fail(ILC_ENOTREADY, "run before setup_evaluation_points");
     device_guard dg(h->device);
     ensure_cubic(h);
     point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
-    launch_gather(view_of(h), p, h->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT,
+    launch_gather(h->grid.view(), p, h->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT,
                   h->kernel, h->ts, h->acc.p, nullptr, h->cubic.p, as_stream(stream));
   });
 }
@@ -332,14 +251,14 @@ ilc_status ilc_psg_grid_info(const ilc_psg* h, ilc_grid_info* out) {
   return guarded([&] {
     check_handle(h);
     ILC_REQUIRE(out, "null output");
-    if (!h->have_grid) fail(ILC_ENOTREADY, "no grid");
+    if (!h->grid.ready) fail(ILC_ENOTREADY, "no grid");
     for (int a = 0; a < 3; ++a) {
-      out->num_cells[a] = h->gp.nc[a];
-      out->min_corner[a] = h->gp.minc[a];
-      out->cell_size[a] = h->gp.cs[a];
+      out->num_cells[a] = h->grid.gp.nc[a];
+      out->min_corner[a] = h->grid.gp.minc[a];
+      out->cell_size[a] = h->grid.gp.cs[a];
     }
-    out->max_h = h->max_h;
-    out->num_particles = h->n;
+    out->max_h = h->grid.max_h;
+    out->num_particles = h->grid.n;
   });
 }
 
@@ -347,13 +266,13 @@ ilc_status ilc_psg_export_grid(const ilc_psg* h, int32_t* cells2_dev, float* max
                                uint32_t* perm_dev, void* stream) {
   return guarded([&] {
     check_handle(h);
-    if (!h->have_grid) fail(ILC_ENOTREADY, "no grid");
+    if (!h->grid.ready) fail(ILC_ENOTREADY, "no grid");
     device_guard dg(h->device);
     hipStream_t s = as_stream(stream);
-    launch_export_cells(h->start.p, h->maxh_bits.p, static_cast<uint32_t>(h->gp.ncells()),
-                        cells2_dev, maxh_dev, s);
+    launch_export_cells(h->grid.start.p, h->grid.maxh_bits.p,
+                        static_cast<uint32_t>(h->grid.gp.ncells()), cells2_dev, maxh_dev, s);
     if (perm_dev)
-      ILC_HIP(hipMemcpyAsync(perm_dev, h->vals.p, sizeof(uint32_t) * h->n,
+      ILC_HIP(hipMemcpyAsync(perm_dev, h->grid.perm.p, sizeof(uint32_t) * h->grid.n,
                              hipMemcpyDeviceToDevice, s));
   });
 }
@@ -361,7 +280,7 @@ ilc_status ilc_psg_export_grid(const ilc_psg* h, int32_t* cells2_dev, float* max
 ilc_status ilc_psg_count_evals(ilc_psg* h, uint64_t* counts_dev, uint64_t* total, void* stream) {
   return guarded([&] {
     check_handle(h);
-    if (!h->have_grid) fail(ILC_ENOTREADY, "no grid");
+    if (!h->grid.ready) fail(ILC_ENOTREADY, "no grid");
     if (!h->pts) fail(ILC_ENOTREADY, "no evaluation points");
     device_guard dg(h->device);
     hipStream_t s = as_stream(stream);
@@ -369,7 +288,7 @@ ilc_status ilc_psg_count_evals(ilc_psg* h, uint64_t* counts_dev, uint64_t* total
     h->cnt_scratch.ensure(1);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->cnt.p), 0u, 2 * h->npts, s);
     point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
-    launch_gather(view_of(h), p, OP_COUNT, h->kernel, h->ts, nullptr, h->cnt.p, nullptr, s);
+    launch_gather(h->grid.view(), p, OP_COUNT, h->kernel, h->ts, nullptr, h->cnt.p, nullptr, s);
     if (counts_dev)
       ILC_HIP(hipMemcpyAsync(counts_dev, h->cnt.p, sizeof(unsigned long long) * h->npts,
                              hipMemcpyDeviceToDevice, s));

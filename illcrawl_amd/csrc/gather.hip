@@ -33,6 +33,8 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace ilc {
 
 namespace {
@@ -121,43 +123,38 @@ __device__ inline float shape(float r2, float ih2, const float* tab) {
   }
 }
 
+template <bool REF, int KERNEL>
+struct gather_smem {
+  float4 Lf[kCap];
+  float4 Lm[REF ? kCap : 1];
+  int2 Lmc[REF ? kCap : 1];
+  uint32_t wcnt[2][8];
+  float redf[4][4];
+  int redi[4][10];
+  float tab[KERNEL == 2 ? kCubicTable : 1];
+  uint32_t cell_b[kBlock];
+  uint32_t cell_off[kBlock];
+  int2 cell_info[kBlock];
+  uint32_t scan_ws[4];
+};
+
+// One tile (tx, ty) of the point image: everything between the point loads
+// and the accumulator update.  All 256 threads of the workgroup call it with
+// the same arguments.
 template <int TW, int PPT, int OP, int KERNEL>
-__global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, uint32_t tiles_x,
-                                                   uint32_t ntiles, double* __restrict__ acc_out,
-                                                   unsigned long long* __restrict__ counts,
-                                                   const float* __restrict__ cubic_table,
-                                                   float support) {
+__device__ inline void gather_tile(const grid_view& g, const point_view& p, uint32_t tx,
+                                   uint32_t ty, double* __restrict__ acc_out,
+                                   unsigned long long* __restrict__ counts, float support,
+                                   gather_smem<OP == OP_REFERENCE, KERNEL>& sm) {
   constexpr int ROWS = kBlock / TW;
   constexpr int TH = ROWS * PPT;
   constexpr bool REF = OP == OP_REFERENCE;
   constexpr bool COUNT = OP == OP_COUNT;
 
-  __shared__ float4 Lf[kCap];
-  __shared__ float4 Lm[REF ? kCap : 1];
-  __shared__ int2 Lmc[REF ? kCap : 1];
-  __shared__ uint32_t wcnt[2][8];
-  __shared__ float redf[4][4];
-  __shared__ int redi[4][10];
-  __shared__ float tab[KERNEL == 2 ? kCubicTable : 1];
-  __shared__ uint32_t cell_b[kBlock];
-  __shared__ uint32_t cell_off[kBlock];
-  __shared__ int2 cell_info[kBlock];
-  __shared__ uint32_t scan_ws[4];
-
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
 
-  if constexpr (KERNEL == 2) {
-    for (int i = t; i < kCubicTable; i += kBlock) tab[i] = cubic_table[i];
-  }
 
-  // XCD-contiguous tile order: workgroups are dealt round-robin to the 8
-  // XCDs, so give XCD x the x-th eighth of the tiles (shared L2 reuse of the
-  // neighbouring tiles' particle spans).
-  const uint32_t bid = blockIdx.x;
-  const uint32_t per = ntiles / 8u;
-  const uint32_t tile = bid < per * 8u ? (bid % 8u) * per + bid / 8u : bid;
-  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
   const int col = t % TW, rg = t / TW;
 
   float ex[PPT], ey[PPT];
@@ -236,44 +233,44 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
                   wmaxi(ily), wmini(ihx), wmini(ihy), wmaxi(anyz), wmini(allz)};
     const int av = wmaxi(anyv);
     if (lane == 0) {
-      redf[wave][0] = v0;
-      redf[wave][1] = v1;
-      redf[wave][2] = v2;
-      redf[wave][3] = v3;
-      for (int j = 0; j < 10; ++j) redi[wave][j] = iv[j];
+      sm.redf[wave][0] = v0;
+      sm.redf[wave][1] = v1;
+      sm.redf[wave][2] = v2;
+      sm.redf[wave][3] = v3;
+      for (int j = 0; j < 10; ++j) sm.redi[wave][j] = iv[j];
       // waves without a valid point must not widen the intersection
       if (!av) {
-        redi[wave][4] = INT32_MIN;
-        redi[wave][5] = INT32_MIN;
-        redi[wave][6] = INT32_MAX;
-        redi[wave][7] = INT32_MAX;
-        redi[wave][9] = 1;
+        sm.redi[wave][4] = INT32_MIN;
+        sm.redi[wave][5] = INT32_MIN;
+        sm.redi[wave][6] = INT32_MAX;
+        sm.redi[wave][7] = INT32_MAX;
+        sm.redi[wave][9] = 1;
       }
     }
-    if (lane == 0) wcnt[0][4 + wave] = av;
+    if (lane == 0) sm.wcnt[0][4 + wave] = av;
   }
   __syncthreads();
-  float tbx0 = redf[0][0], tby0 = redf[0][1], tbx1 = redf[0][2], tby1 = redf[0][3];
-  int r_ulx = redi[0][0], r_uly = redi[0][1], r_uhx = redi[0][2], r_uhy = redi[0][3];
-  int r_ilx = redi[0][4], r_ily = redi[0][5], r_ihx = redi[0][6], r_ihy = redi[0][7];
-  int r_anyz = redi[0][8], r_allz = redi[0][9];
-  int any_valid = wcnt[0][4];
+  float tbx0 = sm.redf[0][0], tby0 = sm.redf[0][1], tbx1 = sm.redf[0][2], tby1 = sm.redf[0][3];
+  int r_ulx = sm.redi[0][0], r_uly = sm.redi[0][1], r_uhx = sm.redi[0][2], r_uhy = sm.redi[0][3];
+  int r_ilx = sm.redi[0][4], r_ily = sm.redi[0][5], r_ihx = sm.redi[0][6], r_ihy = sm.redi[0][7];
+  int r_anyz = sm.redi[0][8], r_allz = sm.redi[0][9];
+  int any_valid = sm.wcnt[0][4];
   for (int w = 1; w < 4; ++w) {
-    tbx0 = fminf(tbx0, redf[w][0]);
-    tby0 = fminf(tby0, redf[w][1]);
-    tbx1 = fmaxf(tbx1, redf[w][2]);
-    tby1 = fmaxf(tby1, redf[w][3]);
-    r_ulx = min(r_ulx, redi[w][0]);
-    r_uly = min(r_uly, redi[w][1]);
-    r_uhx = max(r_uhx, redi[w][2]);
-    r_uhy = max(r_uhy, redi[w][3]);
-    r_ilx = max(r_ilx, redi[w][4]);
-    r_ily = max(r_ily, redi[w][5]);
-    r_ihx = min(r_ihx, redi[w][6]);
-    r_ihy = min(r_ihy, redi[w][7]);
-    r_anyz = max(r_anyz, redi[w][8]);
-    r_allz = min(r_allz, redi[w][9]);
-    any_valid |= wcnt[0][4 + w];
+    tbx0 = fminf(tbx0, sm.redf[w][0]);
+    tby0 = fminf(tby0, sm.redf[w][1]);
+    tbx1 = fmaxf(tbx1, sm.redf[w][2]);
+    tby1 = fmaxf(tby1, sm.redf[w][3]);
+    r_ulx = min(r_ulx, sm.redi[w][0]);
+    r_uly = min(r_uly, sm.redi[w][1]);
+    r_uhx = max(r_uhx, sm.redi[w][2]);
+    r_uhy = max(r_uhy, sm.redi[w][3]);
+    r_ilx = max(r_ilx, sm.redi[w][4]);
+    r_ily = max(r_ily, sm.redi[w][5]);
+    r_ihx = min(r_ihx, sm.redi[w][6]);
+    r_ihy = min(r_ihy, sm.redi[w][7]);
+    r_anyz = max(r_anyz, sm.redi[w][8]);
+    r_allz = min(r_allz, sm.redi[w][9]);
+    any_valid |= sm.wcnt[0][4 + w];
   }
   __syncthreads();
   if (!uniform(any_valid)) return;
@@ -327,20 +324,20 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
         for (; j + 4 <= jn; j += 4) {
           float4 q[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) q[u] = Lf[j0 + j + u];
+          for (int u = 0; u < 4; ++u) q[u] = sm.Lf[j0 + j + u];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
               const float dx = q[u].x - ex[k], dy = q[u].y - ey[k];
               const float r2 = fmaf(dy, dy, dx * dx);
-              a[k] = fmaf(q[u].z, shape<KERNEL>(r2, q[u].w, tab), a[k]);
+              a[k] = fmaf(q[u].z, shape<KERNEL>(r2, q[u].w, sm.tab), a[k]);
             }
           }
         }
       }
       for (; j < jn; ++j) {
-        const float4 q = Lf[j0 + j];
+        const float4 q = sm.Lf[j0 + j];
         if constexpr (COUNT) {
 #pragma clang fp contract(off)
           const double hs = double(support) * double(q.z);
@@ -357,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
           for (int k = 0; k < PPT; ++k) {
             const float dx = q.x - ex[k], dy = q.y - ey[k];
             const float r2 = fmaf(dy, dy, dx * dx);
-            a[k] = fmaf(q.z, shape<KERNEL>(r2, q.w, tab), a[k]);
+            a[k] = fmaf(q.z, shape<KERNEL>(r2, q.w, sm.tab), a[k]);
           }
         }
       }
@@ -376,15 +373,15 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
 #pragma unroll
         for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
         for (int j = 0; j < jn; ++j) {
-          const float4 q = Lm[j0 + j];
-          const int2 c = Lmc[j0 + j];
+          const float4 q = sm.Lm[j0 + j];
+          const int2 c = sm.Lmc[j0 + j];
 #pragma unroll
           for (int k = 0; k < PPT; ++k) {
             const bool in = zin[k] & (c.x >= lox[k]) & (c.x <= hix[k]) & (c.y >= loy[k]) &
                             (c.y <= hiy[k]);
             const float dx = q.x - ex[k], dy = q.y - ey[k];
             const float r2 = fmaf(dy, dy, dx * dx);
-            const float v = shape<KERNEL>(r2, q.w, tab);
+            const float v = shape<KERNEL>(r2, q.w, sm.tab);
             a[k] = fmaf(q.z, in ? v : 0.0f, a[k]);
           }
         }
@@ -438,10 +435,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
     }
     // block exclusive scan of the reached cells' particle counts
     uint32_t tot;
-    const uint32_t coff = block_scan_excl(cn, &tot, scan_ws);
-    cell_b[t] = cb;
-    cell_off[t] = coff;
-    cell_info[t] = make_int2(ccx, (ccy << 1) | cfull);
+    const uint32_t coff = block_scan_excl(cn, &tot, sm.scan_ws);
+    sm.cell_b[t] = cb;
+    sm.cell_off[t] = coff;
+    sm.cell_info[t] = make_int2(ccx, (ccy << 1) | cfull);
     __syncthreads();
     tot = uint32_t(uniform(int(tot)));
     for (uint32_t v0 = 0; v0 < tot; v0 += kBlock) {
@@ -455,9 +452,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
         int lo = 0;
 #pragma unroll
         for (int step = kBlock / 2; step > 0; step >>= 1)
-          if (cell_off[lo + step] <= v) lo += step;
-        const uint32_t sidx = cell_b[lo] + (v - cell_off[lo]);
-        cinf = cell_info[lo];
+          if (sm.cell_off[lo + step] <= v) lo += step;
+        const uint32_t sidx = sm.cell_b[lo] + (v - sm.cell_off[lo]);
+        cinf = sm.cell_info[lo];
         q = g.part[sidx];
         if (q.z != 0.0f) {
           const float dx = fmaxf(fmaxf(tbx0 - q.x, q.x - tbx1), 0.0f);
@@ -473,13 +470,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
       const uint64_t bf = __ballot(full);
       const uint64_t bm = __ballot(keep && !full);
       if (lane == 0) {
-        wcnt[slot][wave] = __popcll(bf);
-        wcnt[slot][4 + wave] = __popcll(bm);
+        sm.wcnt[slot][wave] = __popcll(bf);
+        sm.wcnt[slot][4 + wave] = __popcll(bm);
       }
       __syncthreads();
       int basef = nf, basem = nm, totf = 0, totm = 0;
       for (int w = 0; w < 4; ++w) {
-        const int cf = wcnt[slot][w], cm = wcnt[slot][4 + w];
+        const int cf = sm.wcnt[slot][w], cm = sm.wcnt[slot][4 + w];
         if (w < wave) {
           basef += cf;
           basem += cm;
@@ -487,12 +484,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
         totf += cf;
         totm += cm;
       }
-      if (full) Lf[basef + prefix_count(bf)] = q;
+      if (full) sm.Lf[basef + prefix_count(bf)] = q;
       if constexpr (REF) {
         if (keep && !full) {
           const int pos = basem + prefix_count(bm);
-          Lm[pos] = q;
-          Lmc[pos] = make_int2(cinf.x, cinf.y >> 1);
+          sm.Lm[pos] = q;
+          sm.Lmc[pos] = make_int2(cinf.x, cinf.y >> 1);
         }
       }
       nf += totf;
@@ -527,6 +524,56 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
 }
 
 template <int TW, int PPT, int OP, int KERNEL>
+__global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, uint32_t tiles_x,
+                                                   uint32_t ntiles, double* __restrict__ acc_out,
+                                                   unsigned long long* __restrict__ counts,
+                                                   const float* __restrict__ cubic_table,
+                                                   float support) {
+  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL> sm;
+  if constexpr (KERNEL == 2) {
+    for (int i = threadIdx.x; i < kCubicTable; i += kBlock) sm.tab[i] = cubic_table[i];
+  }
+  // XCD-contiguous tile order: workgroups are dealt round-robin to the 8
+  // XCDs, so give XCD x the x-th eighth of the tiles (shared L2 reuse of the
+  // neighbouring tiles' particle spans).
+  const uint32_t bid = blockIdx.x;
+  const uint32_t per = ntiles / 8u;
+  const uint32_t tile = bid < per * 8u ? (bid % 8u) * per + bid / 8u : bid;
+  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+  gather_tile<TW, PPT, OP, KERNEL>(g, p, tx, ty, acc_out, counts, support, sm);
+}
+
+// Persistent form over many views: every workgroup takes the next
+// (view, tile) item from a device counter until the list is exhausted.
+// Items are ordered by estimated cost, largest first.
+template <int TW, int PPT, int OP, int KERNEL>
+__global__ __launch_bounds__(kBlock) void k_gather_batch(const view_desc* __restrict__ views,
+                                                         const uint2* __restrict__ items,
+                                                         uint32_t nitems,
+                                                         uint32_t* __restrict__ counter,
+                                                         const float* __restrict__ cubic_table,
+                                                         float support) {
+  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL> sm;
+  __shared__ uint32_t next;
+  if constexpr (KERNEL == 2) {
+    for (int i = threadIdx.x; i < kCubicTable; i += kBlock) sm.tab[i] = cubic_table[i];
+  }
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t w = uint32_t(uniform(int(next)));
+    if (w >= nitems) break;
+    const uint2 it = items[w];
+    const uint32_t vi = uint32_t(uniform(int(it.x))), ti = uint32_t(uniform(int(it.y)));
+    const view_desc& vd = views[vi];
+    const uint32_t tiles_x = vd.tiles_x;
+    gather_tile<TW, PPT, OP, KERNEL>(vd.g, vd.p, ti % tiles_x, ti / tiles_x, vd.acc, nullptr,
+                                     support, sm);
+  }
+}
+
+template <int TW, int PPT, int OP, int KERNEL>
 void launch_one(const grid_view& g, const point_view& p, double* acc, unsigned long long* counts,
                 const float* cubic_table, float support, hipStream_t s) {
   constexpr int TH = (kBlock / TW) * PPT;
@@ -554,7 +601,86 @@ void launch_shape(const grid_view& g, const point_view& p, const tile_shape& ts,
     fail(ILC_EINVAL, "gather: unsupported tile shape tw=%d ppt=%d", ts.tw, ts.ppt);
 }
 
+template <int TW, int PPT, int OP, int KERNEL>
+void launch_batch_one(const view_desc* views, const uint2* items, uint32_t nitems,
+                      uint32_t* counter, const float* tab, float support, int grid,
+                      hipStream_t s) {
+  hipLaunchKernelGGL((k_gather_batch<TW, PPT, OP, KERNEL>), dim3(grid), dim3(kBlock), 0, s, views,
+                     items, nitems, counter, tab, support);
+  ILC_HIP(hipGetLastError());
+}
+
+template <int TW, int PPT, int OP, int KERNEL>
+int batch_occupancy() {
+  int per_cu = 0;
+  ILC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_batch<TW, PPT, OP, KERNEL>,
+                                                      kBlock, 0));
+  return per_cu;
+}
+
+// dispatch over (tile shape, op, kernel) to F<TW, PPT, OP, KERNEL>
+template <template <int, int, int, int> class F, class... A>
+auto dispatch(const tile_shape& ts, int op, int kernel, A&&... a) {
+  auto by_kernel = [&](auto tw, auto ppt) {
+    constexpr int TW = decltype(tw)::value, PPT = decltype(ppt)::value;
+    if (op == OP_REFERENCE) {
+      if (kernel == ILC_KERNEL_CUBIC) fail(ILC_EINVAL, "reference mode uses the quartic kernel");
+      return F<TW, PPT, OP_REFERENCE, 0>::run(a...);
+    }
+    if (kernel == ILC_KERNEL_CUBIC) return F<TW, PPT, OP_EXACT, 2>::run(a...);
+    return F<TW, PPT, OP_EXACT, 0>::run(a...);
+  };
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  using I64 = std::integral_constant<int, 64>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I4 = std::integral_constant<int, 4>;
+  if (ts.tw == 16 && ts.ppt == 1) return by_kernel(I16{}, I1{});
+  if (ts.tw == 32 && ts.ppt == 2) return by_kernel(I32{}, I2{});
+  if (ts.tw == 32 && ts.ppt == 4) return by_kernel(I32{}, I4{});
+  if (ts.tw == 64 && ts.ppt == 4) return by_kernel(I64{}, I4{});
+  fail(ILC_EINVAL, "gather: unsupported tile shape tw=%d ppt=%d", ts.tw, ts.ppt);
+}
+
+template <int TW, int PPT, int OP, int KERNEL>
+struct batch_launcher {
+  static int run(const view_desc* v, const uint2* it, uint32_t ni, uint32_t* c, const float* tab,
+                 float sup, int grid, hipStream_t s) {
+    launch_batch_one<TW, PPT, OP, KERNEL>(v, it, ni, c, tab, sup, grid, s);
+    return 0;
+  }
+};
+
+template <int TW, int PPT, int OP, int KERNEL>
+struct batch_occ {
+  static int run() { return batch_occupancy<TW, PPT, OP, KERNEL>(); }
+};
+
 }  // namespace
+
+void tile_grid(const tile_shape& ts, uint32_t lnx, uint32_t lny, uint32_t* tiles_x,
+               uint32_t* ntiles) {
+  const uint32_t th = uint32_t(kBlock / ts.tw) * uint32_t(ts.ppt);
+  *tiles_x = (lnx + ts.tw - 1) / ts.tw;
+  *ntiles = *tiles_x * ((lny + th - 1) / th);
+}
+
+int gather_batch_grid(int op, int kernel, const tile_shape& ts) {
+  int dev = 0, cus = 0;
+  ILC_HIP(hipGetDevice(&dev));
+  ILC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int per_cu = dispatch<batch_occ>(ts, op, kernel);
+  return std::max(1, per_cu) * cus;
+}
+
+void launch_gather_batch(const view_desc* views, const uint2* items, uint32_t nitems,
+                         uint32_t* counter, int op, int kernel, const tile_shape& ts,
+                         const float* cubic_table, int grid_blocks, hipStream_t s) {
+  const float support = kernel == ILC_KERNEL_CUBIC ? 2.0f : 1.0f;
+  dispatch<batch_launcher>(ts, op, kernel, views, items, nitems, counter, cubic_table, support,
+                           grid_blocks, s);
+}
 
 void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,
                    const tile_shape& ts, double* acc, unsigned long long* counts,

@@ -381,6 +381,12 @@ void launch_bbox(const float4* p4, const float* h, uint64_t n, float* partial, i
   ILC_HIP(hipGetLastError());
 }
 
+void launch_bbox_final(const float* partial, int nparts, const uint32_t* bad, bbox_result* out,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, partial, nparts, bad, out);
+  ILC_HIP(hipGetLastError());
+}
+
 void launch_keys(const float4* p4, uint32_t n, const grid_params& g, uint32_t* keys,
                  uint32_t* vals, uint32_t* err, hipStream_t s) {
   hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, n, g, keys,

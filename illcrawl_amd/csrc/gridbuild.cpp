@@ -1,0 +1,97 @@
+// Grid construction on the device (see gridbuild.hpp).
+#pragma clang fp contract(off)
+
+#include "gridbuild.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ilc {
+
+namespace {
+int bits_for(uint64_t maxval) {
+  int b = 0;
+  while (b < 64 && (maxval >> b)) ++b;
+  return b;
+}
+}  // namespace
+
+float kernel_norm(int kernel) {
+  switch (kernel) {
+    case ILC_KERNEL_QUARTIC_REF:
+      // 2 * norm * integral of smoothing_kernel.cl:57-60, with the h powers
+      // folded into 1/h^2 (h^5 of the s2^2.5 factor over h^3 * h^4)
+      return static_cast<float>(2.0 * double(1.04445f) * double(1.066667f));
+    case ILC_KERNEL_QUARTIC:
+      return static_cast<float>(3.5 / M_PI);
+    default:
+      return 1.0f;
+  }
+}
+
+grid_view grid_store::view() const {
+  grid_view g;
+  g.part = part.p;
+  g.key = key.p;
+  g.hs = hs.p;
+  g.start = start.p;
+  g.maxh_bits = maxh_bits.p;
+  for (int a = 0; a < 3; ++a) {
+    g.nc[a] = gp.nc[a];
+    g.minc[a] = gp.minc[a];
+    g.cs[a] = gp.cs[a];
+  }
+  g.max_h = max_h;
+  g.cell_radius = cell_radius;
+  g.kz0 = kz0;
+  g.ncells = static_cast<uint32_t>(gp.ncells());
+  return g;
+}
+
+grid_params plan_grid(const bbox_result& bb, uint64_t n, uint64_t target, int mode,
+                      int32_t* kz0) {
+  if (bb.bad)
+    fail(ILC_EINVAL, "%u particles have non-finite coordinates/w or h outside [1.1e-19, 1.8e19]",
+         bb.bad);
+  const grid_params g = compute_grid_params(bb, n, target);
+  const float zlo = (bb.mn[2] - g.minc[2]) / g.cs[2];
+  const float zhi = (bb.mx[2] - g.minc[2]) / g.cs[2];
+  *kz0 = static_cast<int32_t>(std::floor(zlo));
+  if (mode == ILC_MODE_REFERENCE && std::floor(zlo) != std::floor(zhi))
+    fail(ILC_EINVAL, "reference mode needs projected particles (one z layer); z spans [%g, %g]",
+         double(bb.mn[2]), double(bb.mx[2]));
+  return g;
+}
+
+void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
+                        uint64_t n, const grid_params& g, int32_t kz0, float max_h, int kernel,
+                        uint32_t* err, hipStream_t s) {
+  const uint64_t ncells = g.ncells();
+  gs.ready = false;
+  gs.key.ensure(n);
+  gs.perm.ensure(n);
+  gs.start.ensure(ncells + 1);
+  gs.maxh_bits.ensure(ncells);
+  gs.part.ensure(n);
+  gs.hs.ensure(n);
+  sc.counts.ensure(ncells + 1);
+  sc.temp.ensure(std::max(radix_temp_words(n), scan_temp_words(ncells + 1)));
+  launch_keys(p4, uint32_t(n), g, gs.key.p, gs.perm.p, err, s);
+  launch_radix_sort(gs.key.p, gs.perm.p, uint32_t(n), bits_for(ncells - 1), sc.temp.p, s);
+  launch_fill_u32(sc.counts.p, 0u, ncells + 1, s);
+  launch_fill_u32(gs.maxh_bits.p, 0u, ncells, s);
+  launch_build_soa(p4, h, gs.perm.p, gs.key.p, uint32_t(n), kernel_norm(kernel), gs.part.p,
+                   gs.hs.p, sc.counts.p, gs.maxh_bits.p, s);
+  launch_exclusive_scan(sc.counts.p, gs.start.p, ncells + 1, sc.temp.p, s);
+  gs.gp = g;
+  gs.n = n;
+  gs.max_h = max_h;
+  float dd = g.cs[0] * g.cs[0];
+  dd = dd + g.cs[1] * g.cs[1];
+  dd = dd + g.cs[2] * g.cs[2];
+  gs.cell_radius = 0.5f * std::sqrt(dd);
+  gs.kz0 = kz0;
+  gs.ready = true;
+}
+
+}  // namespace ilc

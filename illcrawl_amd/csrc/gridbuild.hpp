@@ -1,0 +1,48 @@
+// Device-resident particle grid and its construction, shared by the
+// single-view backend and the batched renderer.
+#pragma once
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace ilc {
+
+// One built grid: the reference's smoothing_particle_grid
+// (smoothing_particle_grid.hpp:29-46) as sorted SoA + cell offsets.
+struct grid_store {
+  dbuf<float4> part;          // (x, y, coef, 1/h^2), cell-sorted
+  dbuf<uint32_t> key;         // sorted cell keys
+  dbuf<float> hs;             // sorted h
+  dbuf<uint32_t> perm;        // sorted position -> input index
+  dbuf<uint32_t> start;       // ncells + 1
+  dbuf<uint32_t> maxh_bits;   // ncells
+  grid_params gp{};
+  float max_h = 0.f, cell_radius = 0.f;
+  int32_t kz0 = 0;
+  uint64_t n = 0;
+  bool ready = false;
+  grid_view view() const;
+};
+
+// Scratch shared by successive builds on one stream.
+struct grid_scratch {
+  dbuf<uint32_t> temp, counts, flags;
+  dbuf<float> bbox_partial;
+  dbuf<bbox_result> bbox_dev;
+};
+
+// Grid parameters from a bounding box (host, double; particle_grid.cpp:43-100)
+// plus the z layer of the particles; reference mode requires one layer.
+grid_params plan_grid(const bbox_result& bb, uint64_t n, uint64_t target, int mode,
+                      int32_t* kz0);
+
+// Enqueue keys, stable sort, sorted SoA, per-cell counts / max h and cell
+// offsets for n float4 particles p4 with smoothing lengths h.  Out-of-grid
+// keys are counted into err (device uint32, not cleared here).
+void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
+                        uint64_t n, const grid_params& g, int32_t kz0, float max_h, int kernel,
+                        uint32_t* err, hipStream_t s);
+
+float kernel_norm(int kernel);
+
+}  // namespace ilc

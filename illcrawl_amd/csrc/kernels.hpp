@@ -26,6 +26,39 @@ void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
                       uint32_t* count, uint32_t* maxh_bits, hipStream_t s);
 void launch_export_cells(const uint32_t* start, const uint32_t* maxh_bits, uint32_t ncells,
                          int32_t* cells2, float* maxh, hipStream_t s);
+void launch_bbox_final(const float* partial, int nparts, const uint32_t* bad, bbox_result* out,
+                       hipStream_t s);
+
+// ---- batch.hip
+// Per-view particle pass: quantity functor (mass or bolometric X-ray), camera
+// projection with depth mask, and either bounding-box partials (bbox mode) or
+// the projected float4 particles (write mode).
+struct view_proj {
+  float m[9];
+  float c[3];
+  float depth;
+  int quantity;  // 0 mass, 1 X-ray
+  double scale;  // quantity scaling applied to the mass (w = float(scale * m))
+};
+struct particle_soa {
+  const float* pos3;  // x, y, z interleaved
+  const float* mass;
+  const float* hsml;
+  const float* rho;
+  const float* temp;
+  const float* xe;
+  uint64_t n;
+};
+struct gaunt_table {
+  const float* y;  // device
+  int n;
+  float x0, dx;
+};
+int view_bbox_parts(uint64_t n);
+void launch_view_bbox(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
+                      float* partial, uint32_t* bad, bbox_result* out, hipStream_t s);
+void launch_view_project(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
+                         float4* out, hipStream_t s);
 
 // ---- gather.hip
 // Device view of one built grid (sorted SoA + cells).
@@ -61,6 +94,24 @@ struct tile_shape {
 void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,
                    const tile_shape& ts, double* acc, unsigned long long* counts,
                    const float* cubic_table, hipStream_t s);
+
+// One view of a batch: its grid, its points, its accumulator, its tiling.
+struct view_desc {
+  grid_view g;
+  point_view p;
+  double* acc;
+  uint32_t tiles_x;
+  uint32_t ntiles;
+};
+
+// tiles of a point image for a tile shape
+void tile_grid(const tile_shape& ts, uint32_t lnx, uint32_t lny, uint32_t* tiles_x,
+               uint32_t* ntiles);
+// workgroups of the persistent batch gather that fit on the device at once
+int gather_batch_grid(int op, int kernel, const tile_shape& ts);
+void launch_gather_batch(const view_desc* views, const uint2* items, uint32_t nitems,
+                         uint32_t* counter, int op, int kernel, const tile_shape& ts,
+                         const float* cubic_table, int grid_blocks, hipStream_t s);
 
 // ---- project.hip
 void launch_project_particles(const camera_frame& f, float depth, const float4* in, uint64_t n,

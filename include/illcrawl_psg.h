@@ -211,6 +211,72 @@ ilc_status ilc_quantity_xray_bolometric(const float* rho_dev, const float* temp_
                                         const float* table_host, int32_t ntab, float x0,
                                         float dx, float* w_dev, void* stream);
 
+
+/* ------------------------------------------------- batched multi-view path */
+
+/* Many projections of one resident snapshot in one persistent gather launch,
+ * optionally replayed from a hipGraph.  Replaces, on one GPU, the frame loop
+ * that the reference distributes over MPI ranks (animation.cpp:44-100 with
+ * uniform_work_partitioner.cpp:36-52; illcrawl_render.cpp:189-229) and its
+ * per-frame projection::create_projection (projection.cpp:49-113). */
+
+typedef struct ilc_batch ilc_batch;
+
+typedef enum {
+  ILC_QUANTITY_MASS = 0, /* w = particle mass: column density
+                            (unprocessed_quantity, quantities.cl:701-711) */
+  ILC_QUANTITY_XRAY = 1  /* w = bolometric free-free emissivity (see
+                            ilc_quantity_xray_bolometric) */
+} ilc_quantity_kind;
+
+typedef struct {
+  ilc_camera cam;
+  int32_t quantity; /* ilc_quantity_kind */
+  float scale;      /* effective dA applied at retrieval
+                       (projective_smoothing_reconstruction.cpp:49-84) */
+} ilc_view;
+
+/* ilc_batch_render flags */
+#define ILC_RENDER_PLAIN 1 /* one gather launch per view (the plain path) */
+#define ILC_RENDER_GRAPH 2 /* replay the device work from a hipGraph while the
+                              plan (every view's grid) is unchanged */
+
+ilc_status ilc_batch_create(const ilc_psg_config* cfg, ilc_batch** out);
+ilc_status ilc_batch_destroy(ilc_batch* b);
+/* Resident snapshot, borrowed until the next call: pos3_dev (n x, y, z
+ * interleaved), mass_dev, hsml_dev; rho/temp/xe (may be NULL unless an
+ * X-ray view is rendered).  The crawler's per-block particle setup
+ * (reconstructing_data_crawler.cpp:164-255) with the whole snapshot as one
+ * block. */
+ilc_status ilc_batch_set_particles(ilc_batch* b, const float* pos3_dev, const float* mass_dev,
+                                   const float* hsml_dev, const float* rho_dev,
+                                   const float* temp_dev, const float* xe_dev, uint64_t n);
+/* 1-D Gaunt table for X-ray views (node i at x0 + i*dx on ln T). */
+ilc_status ilc_batch_set_gaunt_table(ilc_batch* b, const float* table_host, int32_t ntab,
+                                     float x0, float dx);
+/* Views; computes each view's pixel points (camera.cpp:157-189). */
+ilc_status ilc_batch_set_views(ilc_batch* b, const ilc_view* views, int32_t nviews,
+                               void* stream);
+/* Render every view (synchronises the stream before returning). */
+ilc_status ilc_batch_render(ilc_batch* b, int32_t flags, void* stream);
+/* Image of a view: nx*ny floats (ix + nx*iy), scaled by the view's dA. */
+ilc_status ilc_batch_image(const ilc_batch* b, int32_t view, float* out_dev, void* stream);
+/* Its FP64 accumulator (unscaled). */
+ilc_status ilc_batch_image_f64(const ilc_batch* b, int32_t view, double* out_dev, void* stream);
+/* Exact particle-kernel evaluation count of each view (after a render). */
+ilc_status ilc_batch_count_evals(ilc_batch* b, uint64_t* per_view_host, void* stream);
+/* Device time (ms, HIP events on the render stream) of the gather launch(es)
+ * of the last non-graph render: the one persistent launch, or the sum of the
+ * per-view launches of a plain render.  Measurement only. */
+ilc_status ilc_batch_last_gather_ms(const ilc_batch* b, float* ms);
+/* Gather tile (points); tile_w * tile_h must be a multiple of 256. */
+ilc_status ilc_batch_set_tile(ilc_batch* b, int32_t tile_w, int32_t tile_h);
+/* Times the batched render for each supported tile shape, keeps the
+ * fastest and returns it (no reference counterpart: measurement-driven
+ * tuning). */
+ilc_status ilc_batch_autotune(ilc_batch* b, int32_t* tile_w, int32_t* tile_h, double* best_ms,
+                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif
