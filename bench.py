@@ -1,6 +1,6 @@
 """Benchmark of the projective SPH smoothing path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1|c4]
 
 Workloads (synthetic, seeded; see illcrawl_amd/synth.py):
   c3 (default)  the batched multi-view workload: the 16M-cell X-ray snapshot
@@ -9,7 +9,10 @@ Workloads (synthetic, seeded; see illcrawl_amd/synth.py):
                 widths 1000-5000 kpc/h, quantity alternating column density /
                 X-ray), one persistent gather launch per batch;
   c2            one 2048^2 X-ray view of the same snapshot (plain path);
-  c1            the 2^20-cell column-density view, 512^2 (plain path).
+  c1            the 2^20-cell column-density view, 512^2 (plain path);
+  c4            forward and adjoint of a 1024^2 column-density image of 4M
+                cells (exact mode, quartic): adjoint cost relative to forward
+                and gradient error against the FP64 oracle on a sample.
 
 A step is one pass of the hot path over the workload, with the snapshot
 already resident in HBM: per view the quantity functor, the camera projection
@@ -158,6 +161,86 @@ def workload(name):
     return snap, views, desc
 
 
+def adjoint_main(args):
+    """c4: forward and adjoint of L = 1/2 |I(m) - I(m')|^2, m' = m * lognormal(0.05),
+    for the 4M-cell column-density snapshot at 1024^2 (exact mode, quartic).
+    Forward = projection + grid build + gather + retrieval; adjoint =
+    ilc_psg_backward (gradient clear, adjoint gather, un-permute)."""
+    import numpy as np
+    import torch
+    from illcrawl_amd import backend as B, synth
+    t0 = time.time()
+    snap = synth.cached("c4")
+    view = synth.centred_view(snap.box, width=2000.0, nx=1024)
+    log(f"[bench] c4: {snap.n} cells ({time.time() - t0:.1f}s)")
+    dev = torch.device("cuda", 0)
+    p4 = torch.empty((snap.n, 4), dtype=torch.float32, device=dev)
+    p4[:, :3] = torch.from_numpy(snap.pos).to(dev)
+    p4[:, 3] = torch.from_numpy(snap.mass).to(dev)
+    hs = torch.from_numpy(snap.hsml).to(dev)
+    pts = B.camera_pixel_points(view)
+    rng = np.random.default_rng(7)
+    m_t = torch.from_numpy((snap.mass * rng.lognormal(0.0, 0.05, snap.n)).astype(np.float32))
+    b = B.ProjectiveSmoothingGrid(mode="exact", kernel="quartic")
+    b.init_backend(snap.n)
+
+    def forward(w=None):
+        q = p4 if w is None else torch.cat([p4[:, :3], w.to(dev)[:, None]], 1)
+        proj = B.project_particles(view, q)
+        b.setup_projected_particles(proj, hs)
+        b.setup_evaluation_points(pts, layout_nx=view["nx"])
+        b.run()
+        return b.retrieve_results(1.0), proj
+
+    target, _ = forward(m_t)
+    img, proj = forward()
+    G = (img - target).contiguous()
+    total = b.count_evals()
+    for _ in range(args.warmup):
+        forward()
+        b.backward(G)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        forward()
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        grads = b.backward(G)
+    torch.cuda.synchronize()
+    bwd_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"[bench] c4 forward {fwd_ms:.2f} ms, adjoint {bwd_ms:.2f} ms, evals {total:.6g}")
+    line = {
+        "metric": "adjoint vs forward cost, projected SPH image (exact quartic)",
+        "value": total / (fwd_ms * 1e-3), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": fwd_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 per-pixel accumulation)",
+        "data": "synthetic (seeded: uniform + NFW halo, 32-NN density, lognormal masses)",
+        "config": dict(workload="adjoint of a 1024^2 column-density image of 4M cells",
+                       n_cells=snap.n, box_kpc_h=snap.box, resolution=[1024, 1024],
+                       width_kpc_h=2000.0, mode="exact", kernel="quartic",
+                       evals_per_step=total, parallelism="single GPU"),
+        "adjoint": dict(ms=bwd_ms, forward_ms=fwd_ms, ratio=bwd_ms / fwd_ms,
+                        evals_per_s=total / (bwd_ms * 1e-3)),
+    }
+    if not args.no_parity:
+        from oracle import backward_exact
+        log("[bench] c4 gradient check vs FP64 oracle on a particle sample ...")
+        sel = np.sort(rng.choice(snap.n, size=2000, replace=False))
+        pp = proj.cpu().numpy()[sel]
+        ref = backward_exact(pp, snap.hsml[sel], pts.cpu().numpy(),
+                             G.cpu().numpy().astype(np.float64), 1)
+        errs = {}
+        for name, g, r in zip(("m", "h", "x", "y"), grads, ref):
+            got = g.cpu().numpy()[sel].astype(np.float64)
+            errs[name] = float(np.max(np.abs(got - r)) / max(np.max(np.abs(r)), 1e-300))
+        line["adjoint"]["grad_rel_to_max_err"] = errs
+        line["adjoint"]["grad_sample"] = "2000 random particles, all 1024^2 pixels, FP64"
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +265,8 @@ def main():
     from illcrawl_amd import backend as B, synth
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
+    if args.workload == "c4":
+        return adjoint_main(args)
 
     t0 = time.time()
     snap, views, desc = workload(args.workload)

@@ -73,6 +73,7 @@ SIGNATURES = {
     "ilc_psg_run": (_ST, [_P, _P]),
     "ilc_psg_retrieve_results": (_ST, [_P, ctypes.c_float, _P, _P]),
     "ilc_psg_retrieve_results_f64": (_ST, [_P, _P, _P]),
+    "ilc_psg_backward": (_ST, [_P, _P, ctypes.c_float, _P, _P, _P, _P, _P]),
     "ilc_psg_grid_info": (_ST, [_P, ctypes.POINTER(GridInfo)]),
     "ilc_psg_export_grid": (_ST, [_P, _P, _P, _P, _P]),
     "ilc_psg_count_evals": (_ST, [_P, _P, ctypes.POINTER(_U64), _P]),
@@ -80,6 +81,8 @@ SIGNATURES = {
                                ctypes.POINTER(ctypes.c_double * 3),
                                ctypes.POINTER(ctypes.c_double * 3),
                                ctypes.POINTER(ctypes.c_double)]),
+    "ilc_camera_projection": (_ST, [ctypes.POINTER(CameraC), ctypes.POINTER(ctypes.c_float * 9),
+                                    ctypes.POINTER(ctypes.c_float * 3)]),
     "ilc_project_particles": (_ST, [ctypes.POINTER(CameraC), _P, _U64, _P, _P]),
     "ilc_camera_pixel_points": (_ST, [ctypes.POINTER(CameraC), _P, _P]),
     "ilc_quantity_scaled": (_ST, [_P, ctypes.c_double, _U64, _P, _P]),

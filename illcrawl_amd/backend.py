@@ -114,6 +114,19 @@ class ProjectiveSmoothingGrid:
         N.call("ilc_psg_retrieve_results_f64", self._h, _ptr(out), _stream(stream))
         return out
 
+    def backward(self, dl_dvalue, scale=1.0, stream=None):
+        """Adjoint of run + retrieve_results(scale) (exact mode, quartic
+        kernels): returns (dL/dw, dL/dh, dL/dx, dL/dy) per particle in input
+        order, x and y in the projected frame."""
+        g = _dev_f32(dl_dvalue, self.device).reshape(-1)
+        if g.shape[0] != self._npoints:
+            raise ValueError(f"expected {self._npoints} point gradients, got {g.shape[0]}")
+        n = self.grid_info()["num_particles"]
+        out = torch.empty((4, n), dtype=torch.float32, device=self.device)
+        N.call("ilc_psg_backward", self._h, _ptr(g), ctypes.c_float(scale), _ptr(out[0]),
+               _ptr(out[1]), _ptr(out[2]), _ptr(out[3]), _stream(stream))
+        return out[0], out[1], out[2], out[3]
+
     # inspection ------------------------------------------------------------
     def grid_info(self):
         gi = N.GridInfo()
@@ -149,6 +162,15 @@ def camera_basis(cam):
     N.call("ilc_camera_basis", ctypes.byref(c), ctypes.byref(b0), ctypes.byref(b1),
            ctypes.byref(mp), ctypes.byref(ps))
     return np.array(b0[:]), np.array(b1[:]), np.array(mp[:]), ps.value
+
+
+def camera_projection(cam):
+    """The FP32 matrix and centre of project_particles: x' = minv (x - center)."""
+    c = N.camera_struct(cam)
+    m, ctr = (ctypes.c_float * 9)(), (ctypes.c_float * 3)()
+    N.call("ilc_camera_projection", ctypes.byref(c), ctypes.byref(m), ctypes.byref(ctr))
+    return (np.array(m[:], dtype=np.float32).reshape(3, 3),
+            np.array(ctr[:], dtype=np.float32))
 
 
 def camera_pixel_points(cam, device=0, stream=None):

@@ -24,7 +24,7 @@ SOURCES = ["host.cpp", "gridbuild.cpp", "capi.cpp", "batch.cpp", "grid.hip", "ga
 ARCH = "gfx950"
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
 
 
 def hipcc():

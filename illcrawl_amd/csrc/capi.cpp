@@ -41,6 +41,7 @@ struct ilc_psg {
   dbuf<float> cubic;
   bool cubic_ready = false;
   dbuf<unsigned long long> cnt, cnt_scratch;
+  dbuf<float> grad_sorted;  // adjoint: 4 x n gradients in grid order
 
   ~ilc_psg() {
     if (bbox_host) (void)hipHostFree(bbox_host);
@@ -226,6 +227,33 @@ ilc_status ilc_psg_run(ilc_psg* h, void* stream) {
   });
 }
 
+ilc_status ilc_psg_backward(ilc_psg* h, const float* dl_dvalue_dev, float scale, float* dl_dw_dev,
+                            float* dl_dh_dev, float* dl_dx_dev, float* dl_dy_dev, void* stream) {
+  return guarded([&] {
+    check_handle(h);
+    if (!h->grid.ready) fail(ILC_ENOTREADY, "backward before setup_projected_particles");
+    if (!h->pts) fail(ILC_ENOTREADY, "backward before setup_evaluation_points");
+    if (h->mode != ILC_MODE_EXACT)
+      fail(ILC_EINVAL, "backward needs exact mode (the reference-mode search box is not smooth)");
+    if (h->kernel != ILC_KERNEL_QUARTIC_REF && h->kernel != ILC_KERNEL_QUARTIC)
+      fail(ILC_EINVAL, "backward is implemented for the quartic projected kernels");
+    ILC_REQUIRE(dl_dvalue_dev && dl_dw_dev && dl_dh_dev && dl_dx_dev && dl_dy_dev,
+                "null gradient pointer");
+    device_guard dg(h->device);
+    hipStream_t s = as_stream(stream);
+    const uint64_t n = h->grid.n;
+    if (!n) return;
+    h->grad_sorted.ensure(4 * n);
+    float* gs = h->grad_sorted.p;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(gs), 0u, 4 * n, s);
+    point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
+    launch_adjoint(h->grid.view(), p, dl_dvalue_dev, scale, kernel_norm(h->kernel), gs, gs + n,
+                   gs + 2 * n, gs + 3 * n, s);
+    launch_unpermute4(h->grid.perm.p, static_cast<uint32_t>(n), gs, gs + n, gs + 2 * n, gs + 3 * n,
+                      dl_dw_dev, dl_dh_dev, dl_dx_dev, dl_dy_dev, s);
+  });
+}
+
 ilc_status ilc_psg_retrieve_results(ilc_psg* h, float scale, float* out_dev, void* stream) {
   return guarded([&] {
     check_handle(h);
@@ -294,6 +322,17 @@ ilc_status ilc_psg_count_evals(ilc_psg* h, uint64_t* counts_dev, uint64_t* total
                              hipMemcpyDeviceToDevice, s));
     const unsigned long long tot = launch_sum_u64(h->cnt.p, h->npts, h->cnt_scratch.p, s);
     if (total) *total = tot;
+  });
+}
+
+ilc_status ilc_camera_projection(const ilc_camera* cam, float minv[9], float center[3]) {
+  return guarded([&] {
+    ILC_REQUIRE(cam, "null camera");
+    const camera_frame f = make_camera_frame(*cam);
+    for (int i = 0; i < 9; ++i)
+      if (minv) minv[i] = f.minv[i];
+    for (int i = 0; i < 3; ++i)
+      if (center) center[i] = f.center[i];
   });
 }
 

@@ -8,21 +8,33 @@
 // The workgroup
 //   1. loads its points and, in reference mode, each point's search box
 //      exactly as the reference computes it (cell of the point, radius =
-//      min(max_h[cell] + cell_radius, max_h), index box of p -/+ radius, with
+//      min(max_h[cell] + cell_radius, max_h), index box of p -/+ radius, from
 //      the unclamped 64-bit key of the reference);
 //   2. reduces the tile's bounding box and the union / intersection of the
 //      points' index boxes;
-//   3. walks the candidate cells row by row: a row of cells with consecutive
-//      x is one contiguous span of the cell-sorted particle array, streamed
-//      256 particles at a time with coalesced 16-byte loads;
-//   4. culls each streamed particle against the tile (support disk vs tile
-//      box) and compacts the survivors into LDS with wave ballots.  In
-//      reference mode a survivor whose cell lies inside every point's index
-//      box goes to the "full" list, the others to the "masked" list, which
-//      carries the cell coordinates for a per-point integer test;
-//   5. when a list holds FLUSH particles, every thread evaluates all listed
-//      particles for its points (LDS broadcast reads), accumulating in FP32
-//      over 32 particles and then into a per-point FP64 accumulator.
+//   3. visits the candidate cells 256 at a time, skips every cell whose max h
+//      cannot reach the tile, and streams the particles of the others with a
+//      block-wide offset table (coalesced 16-byte loads);
+//   4. culls each particle against the tile (support disk vs tile box) and
+//      compacts the survivors into LDS lists with wave ballots:
+//        big    - every point may use it and its support is at least twice
+//                 the tile's half-diagonal: stored as the coefficients of
+//                 its kernel argument expanded about the tile centre;
+//        small  - every point may use it, stored by position;
+//        masked - reference mode, its cell is inside some points' index box
+//                 and outside others': stored by position and cell;
+//   5. when a list is full, every thread evaluates the listed particles for
+//      its points from LDS (broadcast reads), FP32 over 32 particles, then
+//      into a per-point FP64 accumulator.
+//
+// Kernel arithmetic.  With c = coef^0.4, the quartic term coef*s^2.5,
+// s = 1 - r^2/h^2, equals (c*s)^2.5, and c*s is affine in the point's
+// coordinates u (relative to the tile centre t) and |u|^2:
+//   c*s = c(1 - |x-t|^2/h^2) + (2c/h^2)(x-t).u - (c/h^2)|u|^2,
+// three FMAs per point for big particles (their expansion stays well
+// conditioned: |x-t| <= 1.5 h), then max, sqrt, mul, fma.  Small and masked
+// particles use c*s = c - (c/h^2) r^2 directly.
+//
 // The cell-distance cull of the reference (smoothing_particle_grid.cl:95-100)
 // never removes a particle whose support covers the point (a cell's centre
 // is within the cell half-diagonal of every particle in it), so it is not
@@ -30,10 +42,10 @@
 // reference and are culled here.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
-
-#include <type_traits>
 
 namespace ilc {
 
@@ -51,7 +63,7 @@ __device__ inline int32_t floor_to_int(float f) {
   return static_cast<int32_t>(fl);
 }
 
-// grid3d_get_cell_indices of p - r (or p + r) on one axis, FP32, no FMA.
+// grid3d_get_cell_indices of p + r on one axis, FP32, no FMA.
 __device__ inline int32_t cell_of(float p, float r, float mn, float cs) {
 #pragma clang fp contract(off)
   const float a = p + r;
@@ -104,57 +116,70 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
   return off + incl - v;
 }
 
-// Kernel shape times coefficient.  KERNEL 0: (1 - r^2/h^2)^2.5 (both
-// quartic normalisations); KERNEL 2: tabulated projected cubic spline on
-// u = r^2/h^2 in [0, 4), linear interpolation.
-template <int KERNEL>
-__device__ inline float shape(float r2, float ih2, const float* tab) {
-  if constexpr (KERNEL == 2) {
-    const float u = r2 * ih2;
-    const float x = fminf(u * (float(kCubicTable - 1) / 4.0f), float(kCubicTable - 1));
-    const int i = min(int(x), kCubicTable - 2);
-    const float f = x - float(i);
-    const float a = tab[i], b = tab[i + 1];
-    return fmaf(f, b - a, a);
-  } else {
-    float s = fmaf(-r2, ih2, 1.0f);
-    s = fmaxf(s, 0.0f);
-    return (s * s) * __builtin_amdgcn_sqrtf(s);
-  }
+// (c s)^2.5 for a scaled kernel argument cs = c*s, clamped at the support.
+__device__ inline float pow25(float cs) {
+  cs = fmaxf(cs, 0.0f);
+  return (cs * cs) * __builtin_amdgcn_sqrtf(cs);
 }
 
-template <bool REF, int KERNEL>
+// Tabulated projected cubic spline on u = r^2/h^2 in [0, 4), linear.
+__device__ inline float cubic_shape(float u, const float* tab) {
+  const float x = fminf(u * (float(kCubicTable - 1) / 4.0f), float(kCubicTable - 1));
+  const int i = min(int(x), kCubicTable - 2);
+  const float f = x - float(i);
+  const float a = tab[i], b = tab[i + 1];
+  return fmaf(f, b - a, a);
+}
+
+// Adjoint inputs and outputs (OP_ADJOINT only).
+struct adjoint_io {
+  const float* grad;  // dL/d(point value), one per point
+  float scale;        // points' value = scale * accumulator
+  float norm;         // kernel normalisation K (W = K s^2.5 / h^2)
+  float* gw;          // per sorted particle: dL/dw, dL/dh, dL/dx, dL/dy (atomic adds)
+  float* gh;
+  float* gx;
+  float* gy;
+};
+
+template <bool REF, int KERNEL, bool ADJ = false>
 struct gather_smem {
-  float4 Lf[kCap];
-  float4 Lm[REF ? kCap : 1];
-  int2 Lmc[REF ? kCap : 1];
-  uint32_t wcnt[2][8];
+  float4 Lb[KERNEL == 0 && !ADJ ? kCap : 1];  // big: expansion coefficients
+  float4 Ls[kCap];                    // small: position form
+  float4 Lm[REF ? kCap : 1];          // masked: position form
+  int Lmc[REF ? kCap : 1];            // masked: packed cell (cx | cy << 16)
+  uint32_t wcnt[2][12];
   float redf[4][4];
   int redi[4][10];
   float tab[KERNEL == 2 ? kCubicTable : 1];
   uint32_t cell_b[kBlock];
   uint32_t cell_off[kBlock];
-  int2 cell_info[kBlock];
+  int cell_info[kBlock];  // packed cell (cx | cy << 16) << 1 | full
   uint32_t scan_ws[4];
+  uint32_t Lidx[ADJ ? kCap : 1];    // adjoint: sorted particle index
+  float4 Pt[ADJ ? kBlock * 4 : 1];  // adjoint: tile points (x, y, dL/dvalue, -), row-major
+  float4 Sb[ADJ ? 16 : 1];          // adjoint: 8x8 point blocks' boxes (x0, y0, x1, y1)
 };
 
 // One tile (tx, ty) of the point image: everything between the point loads
 // and the accumulator update.  All 256 threads of the workgroup call it with
 // the same arguments.
 template <int TW, int PPT, int OP, int KERNEL>
-__device__ inline void gather_tile(const grid_view& g, const point_view& p, uint32_t tx,
-                                   uint32_t ty, double* __restrict__ acc_out,
-                                   unsigned long long* __restrict__ counts, float support,
-                                   gather_smem<OP == OP_REFERENCE, KERNEL>& sm) {
+__device__ __attribute__((always_inline)) inline void gather_tile(
+    const grid_view& g, const point_view& p, uint32_t tx, uint32_t ty, double* __restrict__ acc_out,
+    unsigned long long* __restrict__ counts, float support,
+    gather_smem<OP == OP_REFERENCE, KERNEL, OP == OP_ADJOINT>& sm,
+    const adjoint_io* adj = nullptr) {
   constexpr int ROWS = kBlock / TW;
   constexpr int TH = ROWS * PPT;
   constexpr bool REF = OP == OP_REFERENCE;
   constexpr bool COUNT = OP == OP_COUNT;
+  constexpr bool ADJ = OP == OP_ADJOINT;
+  constexpr bool QUARTIC = KERNEL == 0 && !COUNT && !ADJ;
+  static_assert(!ADJ || PPT * kBlock <= kBlock * 4, "adjoint tile table");
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-
-
   const int col = t % TW, rg = t / TW;
 
   float ex[PPT], ey[PPT];
@@ -238,7 +263,7 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
       sm.redf[wave][2] = v2;
       sm.redf[wave][3] = v3;
       for (int j = 0; j < 10; ++j) sm.redi[wave][j] = iv[j];
-      // waves without a valid point must not widen the intersection
+      // waves without a valid point must not narrow the intersection
       if (!av) {
         sm.redi[wave][4] = INT32_MIN;
         sm.redi[wave][5] = INT32_MIN;
@@ -246,15 +271,15 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
         sm.redi[wave][7] = INT32_MAX;
         sm.redi[wave][9] = 1;
       }
+      sm.wcnt[0][wave] = av;
     }
-    if (lane == 0) sm.wcnt[0][4 + wave] = av;
   }
   __syncthreads();
   float tbx0 = sm.redf[0][0], tby0 = sm.redf[0][1], tbx1 = sm.redf[0][2], tby1 = sm.redf[0][3];
   int r_ulx = sm.redi[0][0], r_uly = sm.redi[0][1], r_uhx = sm.redi[0][2], r_uhy = sm.redi[0][3];
   int r_ilx = sm.redi[0][4], r_ily = sm.redi[0][5], r_ihx = sm.redi[0][6], r_ihy = sm.redi[0][7];
   int r_anyz = sm.redi[0][8], r_allz = sm.redi[0][9];
-  int any_valid = sm.wcnt[0][4];
+  int any_valid = sm.wcnt[0][0];
   for (int w = 1; w < 4; ++w) {
     tbx0 = fminf(tbx0, sm.redf[w][0]);
     tby0 = fminf(tby0, sm.redf[w][1]);
@@ -270,10 +295,47 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
     r_ihy = min(r_ihy, sm.redi[w][7]);
     r_anyz = max(r_anyz, sm.redi[w][8]);
     r_allz = min(r_allz, sm.redi[w][9]);
-    any_valid |= sm.wcnt[0][4 + w];
+    any_valid |= sm.wcnt[0][w];
   }
   __syncthreads();
   if (!uniform(any_valid)) return;
+
+  if constexpr (ADJ) {
+    // the tile's points with their incoming gradients (0 outside the image),
+    // then the bounding box of the points with a nonzero gradient in each
+    // 8x8 block of the tile (one wave per block)
+    static_assert(TW == 32 && TH == 32, "adjoint tile is 32 x 32");
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const float gk = pv[k] ? adj->grad[pid[k]] * adj->scale : 0.0f;
+      sm.Pt[(rg * PPT + k) * TW + col] = make_float4(ex[k], ey[k], gk, 0.0f);
+    }
+    __syncthreads();
+    for (int sb = wave; sb < 16; sb += 4) {
+      const float4 q = sm.Pt[((sb >> 2) * 8 + (lane >> 3)) * TW + (sb & 3) * 8 + (lane & 7)];
+      const bool on = q.z != 0.0f;
+      const float x0 = wmin(on ? q.x : INFINITY), y0 = wmin(on ? q.y : INFINITY);
+      const float x1 = wmax(on ? q.x : -INFINITY), y1 = wmax(on ? q.y : -INFINITY);
+      if (lane == 0) sm.Sb[sb] = make_float4(x0, y0, x1, y1);
+    }
+    __syncthreads();
+    int live = 0;
+    for (int sb = 0; sb < 16; ++sb) live |= sm.Sb[sb].x <= sm.Sb[sb].z ? 1 : 0;
+    if (!uniform(live)) return;
+  }
+
+  // tile centre and the points relative to it (big-particle expansion)
+  const float tcx = 0.5f * (tbx0 + tbx1), tcy = 0.5f * (tby0 + tby1);
+  float lx[PPT], ly[PPT], l2[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    lx[k] = ex[k] - tcx;
+    ly[k] = ey[k] - tcy;
+    l2[k] = fmaf(ly[k], ly[k], lx[k] * lx[k]);
+  }
+  // a particle is "big" when h >= 2 * tile half-diagonal
+  const float hd_x = 0.5f * (tbx1 - tbx0), hd_y = 0.5f * (tby1 - tby0);
+  const float big_r2 = 4.0f * fmaf(hd_y, hd_y, hd_x * hd_x) * 1.0001f + 1e-30f;
 
   // ---- candidate cell box
   int cx0, cx1, cy0, cy1, cz0, cz1;
@@ -307,12 +369,41 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
   const bool allz_u = uniform(r_allz) != 0;
   const float cull2 = COUNT ? support * support * 1.0001f : support * support;
 
-  int nf = 0, nm = 0;  // list sizes (identical in every thread)
+  int nb = 0, ns = 0, nm = 0;  // list sizes (identical in every thread)
   int slot = 0;
 
-  // Evaluate every listed particle for this thread's points: FP32 partial
-  // sums over 32 particles, then FP64.  Loop bounds are wave-uniform.
-  auto eval_full = [&](int cnt) {
+  // ---- evaluation of the lists (loop bounds are wave-uniform)
+  auto eval_big = [&](int cnt) {
+    if constexpr (QUARTIC) {
+      cnt = uniform(cnt);
+      for (int j0 = 0; j0 < cnt; j0 += 32) {
+        const int jn = min(32, cnt - j0);
+        float a[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
+        int j = 0;
+        for (; j + 4 <= jn; j += 4) {
+          float4 q[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) q[u] = sm.Lb[j0 + j + u];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < PPT; ++k)
+              a[k] += pow25(fmaf(q[u].y, lx[k], fmaf(q[u].z, ly[k], fmaf(q[u].w, l2[k], q[u].x))));
+        }
+        for (; j < jn; ++j) {
+          const float4 q = sm.Lb[j0 + j];
+#pragma unroll
+          for (int k = 0; k < PPT; ++k)
+            a[k] += pow25(fmaf(q.y, lx[k], fmaf(q.z, ly[k], fmaf(q.w, l2[k], q.x))));
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
+      }
+    }
+  };
+  auto eval_small = [&](int cnt) {
     cnt = uniform(cnt);
     for (int j0 = 0; j0 < cnt; j0 += 32) {
       const int jn = min(32, cnt - j0);
@@ -324,20 +415,23 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
         for (; j + 4 <= jn; j += 4) {
           float4 q[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) q[u] = sm.Lf[j0 + j + u];
+          for (int u = 0; u < 4; ++u) q[u] = sm.Ls[j0 + j + u];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
               const float dx = q[u].x - ex[k], dy = q[u].y - ey[k];
               const float r2 = fmaf(dy, dy, dx * dx);
-              a[k] = fmaf(q[u].z, shape<KERNEL>(r2, q[u].w, sm.tab), a[k]);
+              if constexpr (KERNEL == 2)
+                a[k] = fmaf(q[u].z, cubic_shape(r2 * q[u].w, sm.tab), a[k]);
+              else
+                a[k] += pow25(fmaf(-r2, q[u].z, q[u].w));
             }
           }
         }
       }
       for (; j < jn; ++j) {
-        const float4 q = sm.Lf[j0 + j];
+        const float4 q = sm.Ls[j0 + j];
         if constexpr (COUNT) {
 #pragma clang fp contract(off)
           const double hs = double(support) * double(q.z);
@@ -354,7 +448,10 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
           for (int k = 0; k < PPT; ++k) {
             const float dx = q.x - ex[k], dy = q.y - ey[k];
             const float r2 = fmaf(dy, dy, dx * dx);
-            a[k] = fmaf(q.z, shape<KERNEL>(r2, q.w, sm.tab), a[k]);
+            if constexpr (KERNEL == 2)
+              a[k] = fmaf(q.z, cubic_shape(r2 * q.w, sm.tab), a[k]);
+            else
+              a[k] += pow25(fmaf(-r2, q.z, q.w));
           }
         }
       }
@@ -364,9 +461,15 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
       }
     }
   };
+  // masked entries come in runs of one cell: the per-point membership test
+  // is recomputed only when the (wave-uniform) cell changes
   auto eval_masked = [&](int cnt) {
     if constexpr (REF) {
       cnt = uniform(cnt);
+      int cur = -1;
+      float keepf[PPT];
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) keepf[k] = 0.0f;
       for (int j0 = 0; j0 < cnt; j0 += 32) {
         const int jn = min(32, cnt - j0);
         float a[PPT];
@@ -374,15 +477,20 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
         for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
         for (int j = 0; j < jn; ++j) {
           const float4 q = sm.Lm[j0 + j];
-          const int2 c = sm.Lmc[j0 + j];
+          const int c = uniform(sm.Lmc[j0 + j]);
+          if (c != cur) {
+            cur = c;
+            const int cx = c & 0xffff, cy = c >> 16;
+#pragma unroll
+            for (int k = 0; k < PPT; ++k)
+              keepf[k] = (zin[k] & (cx >= lox[k]) & (cx <= hix[k]) & (cy >= loy[k]) &
+                          (cy <= hiy[k])) ? 1.0f : 0.0f;
+          }
 #pragma unroll
           for (int k = 0; k < PPT; ++k) {
-            const bool in = zin[k] & (c.x >= lox[k]) & (c.x <= hix[k]) & (c.y >= loy[k]) &
-                            (c.y <= hiy[k]);
             const float dx = q.x - ex[k], dy = q.y - ey[k];
             const float r2 = fmaf(dy, dy, dx * dx);
-            const float v = shape<KERNEL>(r2, q.w, sm.tab);
-            a[k] = fmaf(q.z, in ? v : 0.0f, a[k]);
+            a[k] = fmaf(keepf[k], pow25(fmaf(-r2, q.z, q.w)), a[k]);
           }
         }
 #pragma unroll
@@ -391,28 +499,71 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
     }
   };
 
-  // Candidate cells are the box [cx0,cx1] x [cy0,cy1] x [cz0,cz1], visited
-  // 256 at a time (one per thread, x fastest).  A cell is skipped unless
-  // its max h can reach the tile (the conservative form of the reference's
-  // cell cull); the particles of the reached cells are then streamed with a
-  // block-wide offset table, culled one by one against the tile, and
-  // appended to the LDS lists.
+  // adjoint: one listed particle per thread.  The thread visits the 8x8
+  // point blocks its support reaches and sums, over their points, five
+  // moments of g = dL/dvalue * s^1.5 (s = 1 - r^2/h^2); the weights are
+  // recomputed, not stored.  One atomic add per gradient component per
+  // particle and tile:
+  //   dL/dw = K/h^2        sum g s
+  //   dL/dh = w K/h^3      sum g (7 r^2/h^2 - 2)
+  //   dL/dx = -5 w K/h^4   sum g (x - x_point)     (same for y)
+  auto eval_adjoint = [&](int cnt) {
+    if constexpr (ADJ) {
+      cnt = uniform(cnt);
+      for (int j0 = 0; j0 < cnt; j0 += kBlock) {
+        const int j = j0 + t;
+        const bool act = j < cnt;
+        const float4 q = act ? sm.Ls[j] : make_float4(0.f, 0.f, 0.f, 0.f);  // (x, y, 1/h^2, coef)
+        float A0 = 0.f, A1 = 0.f, A2 = 0.f, A3 = 0.f, A4 = 0.f;
+        for (int sb = 0; sb < 16; ++sb) {
+          const float4 bx = sm.Sb[sb];
+          const float bdx = fmaxf(fmaxf(bx.x - q.x, q.x - bx.z), 0.0f);
+          const float bdy = fmaxf(fmaxf(bx.y - q.y, q.y - bx.w), 0.0f);
+          if (!(act && fmaf(bdy, bdy, bdx * bdx) * q.z < 1.0f)) continue;
+          const int base = (sb >> 2) * 8 * TW + (sb & 3) * 8;
+          for (int rr = 0; rr < 8; ++rr) {
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc) {
+              const float4 pt = sm.Pt[base + rr * TW + cc];
+              const float dx = q.x - pt.x, dy = q.y - pt.y;
+              const float r2 = fmaf(dy, dy, dx * dx);
+              const float sv = fmaxf(fmaf(-r2, q.z, 1.0f), 0.0f);
+              const float gq = pt.z * (sv * __builtin_amdgcn_sqrtf(sv));
+              A0 = fmaf(gq, sv, A0);
+              A1 += gq;
+              A2 = fmaf(gq, r2, A2);
+              A3 = fmaf(gq, dx, A3);
+              A4 = fmaf(gq, dy, A4);
+            }
+          }
+        }
+        if (act) {
+          const uint32_t si = sm.Lidx[j];
+          const float c5 = -5.0f * q.w * q.z;  // -5 w K / h^4
+          atomicAdd(adj->gw + si, adj->norm * q.z * A0);
+          atomicAdd(adj->gh + si, q.w * __builtin_amdgcn_sqrtf(q.z) * fmaf(7.0f * q.z, A2, -2.0f * A1));
+          atomicAdd(adj->gx + si, c5 * A3);
+          atomicAdd(adj->gy + si, c5 * A4);
+        }
+      }
+    }
+  };
+
+  // ---- candidate cells: 256 at a time (x fastest), reach cull, streaming
   const int ncols = cx1 >= cx0 ? cx1 - cx0 + 1 : 0;
   const int nrows = cy1 >= cy0 ? cy1 - cy0 + 1 : 0;
   const int nlays = cz1 >= cz0 ? cz1 - cz0 + 1 : 0;
   const int64_t ncand = int64_t(ncols) * nrows * nlays;
   const uint32_t nxny = uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
-  // reached cells must overlap the tile box grown by the support; float
-  // rounding of particle coordinates vs cell edges is absorbed by a margin
   const float sup_h2 = support * support * 1.00002f;
   for (int64_t c0 = 0; c0 < ncand; c0 += kBlock) {
     const int64_t ci = c0 + t;
     uint32_t cb = 0, cn = 0;
     int ccx = 0, ccy = 0, cfull = 0;
     if (ci < ncand) {
-      const int col = int(ci % ncols);
+      const int col_c = int(ci % ncols);
       const int64_t rest = ci / ncols;
-      ccx = cx0 + col;
+      ccx = cx0 + col_c;
       ccy = cy0 + int(rest % nrows);
       const int ccz = cz0 + int(rest / nrows);
       const uint32_t key = uint32_t(ccx) + uint32_t(ccy) * uint32_t(g.nc[0]) + uint32_t(ccz) * nxny;
@@ -433,21 +584,20 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
           cfull = (allz_u && ccy >= r_ily && ccy <= r_ihy && ccx >= r_ilx && ccx <= r_ihx) ? 1 : 0;
       }
     }
-    // block exclusive scan of the reached cells' particle counts
     uint32_t tot;
     const uint32_t coff = block_scan_excl(cn, &tot, sm.scan_ws);
     sm.cell_b[t] = cb;
     sm.cell_off[t] = coff;
-    sm.cell_info[t] = make_int2(ccx, (ccy << 1) | cfull);
+    sm.cell_info[t] = ((ccx | (ccy << 16)) << 1) | cfull;
     __syncthreads();
     tot = uint32_t(uniform(int(tot)));
     for (uint32_t v0 = 0; v0 < tot; v0 += kBlock) {
       const uint32_t v = v0 + uint32_t(t);
-      const bool valid = v < tot;
-      float4 q = make_float4(0.f, 0.f, 0.f, 1.f);
-      bool keep = false, full = false;
-      int2 cinf = make_int2(0, 0);
-      if (valid) {
+      float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+      bool keep = false, full = false, big = false;
+      int cinf = 0;
+      uint32_t adj_idx = 0;
+      if (v < tot) {
         // last cell whose offset is <= v (it has a nonzero count)
         int lo = 0;
 #pragma unroll
@@ -455,62 +605,103 @@ __device__ inline void gather_tile(const grid_view& g, const point_view& p, uint
           if (sm.cell_off[lo + step] <= v) lo += step;
         const uint32_t sidx = sm.cell_b[lo] + (v - sm.cell_off[lo]);
         cinf = sm.cell_info[lo];
-        q = g.part[sidx];
-        if (q.z != 0.0f) {
+        const float4 q = g.part[sidx];  // (x, y, coef, 1/h^2)
+        if (ADJ || q.z != 0.0f) {
           const float dx = fmaxf(fmaxf(tbx0 - q.x, q.x - tbx1), 0.0f);
           const float dy = fmaxf(fmaxf(tby0 - q.y, q.y - tby1), 0.0f);
           const float d2 = fmaf(dy, dy, dx * dx);
           keep = d2 * q.w < cull2;
         }
-        full = keep && (!REF || (cinf.y & 1));
-        if constexpr (COUNT) {
-          if (keep) q.z = g.hs[sidx];
+        full = keep && (!REF || (cinf & 1));
+        if (keep) {
+          if constexpr (COUNT) {
+            e = make_float4(q.x, q.y, g.hs[sidx], 0.0f);
+          } else if constexpr (ADJ) {
+            e = make_float4(q.x, q.y, q.w, q.z);  // (x, y, 1/h^2, coef)
+            adj_idx = sidx;
+          } else if constexpr (KERNEL == 2) {
+            e = q;
+          } else {
+            const float c = powf(q.z, 0.4f);  // coef^0.4
+            const float ci2 = c * q.w;        // c / h^2
+            big = full && big_r2 * q.w <= 1.0f;
+            if (big) {
+              const float xr = q.x - tcx, yr = q.y - tcy;
+              const float d2r = fmaf(yr, yr, xr * xr);
+              e = make_float4(c * fmaf(-q.w, d2r, 1.0f), 2.0f * ci2 * xr, 2.0f * ci2 * yr, -ci2);
+            } else {
+              e = make_float4(q.x, q.y, ci2, c);
+            }
+          }
         }
       }
-      const uint64_t bf = __ballot(full);
+      const uint64_t bb = __ballot(big);
+      const uint64_t bs = __ballot(full && !big);
       const uint64_t bm = __ballot(keep && !full);
       if (lane == 0) {
-        sm.wcnt[slot][wave] = __popcll(bf);
-        sm.wcnt[slot][4 + wave] = __popcll(bm);
+        sm.wcnt[slot][wave] = __popcll(bb);
+        sm.wcnt[slot][4 + wave] = __popcll(bs);
+        sm.wcnt[slot][8 + wave] = __popcll(bm);
       }
       __syncthreads();
-      int basef = nf, basem = nm, totf = 0, totm = 0;
+      int baseb = nb, bases = ns, basem = nm, totb = 0, tots = 0, totm = 0;
       for (int w = 0; w < 4; ++w) {
-        const int cf = sm.wcnt[slot][w], cm = sm.wcnt[slot][4 + w];
+        const int cb_ = sm.wcnt[slot][w], cs_ = sm.wcnt[slot][4 + w],
+                  cm_ = sm.wcnt[slot][8 + w];
         if (w < wave) {
-          basef += cf;
-          basem += cm;
+          baseb += cb_;
+          bases += cs_;
+          basem += cm_;
         }
-        totf += cf;
-        totm += cm;
+        totb += cb_;
+        tots += cs_;
+        totm += cm_;
       }
-      if (full) sm.Lf[basef + prefix_count(bf)] = q;
+      if constexpr (QUARTIC) {
+        if (big) sm.Lb[baseb + prefix_count(bb)] = e;
+      }
+      if (full && !big) {
+        const int pos = bases + prefix_count(bs);
+        sm.Ls[pos] = e;
+        if constexpr (ADJ) sm.Lidx[pos] = adj_idx;
+      }
       if constexpr (REF) {
         if (keep && !full) {
           const int pos = basem + prefix_count(bm);
-          sm.Lm[pos] = q;
-          sm.Lmc[pos] = make_int2(cinf.x, cinf.y >> 1);
+          sm.Lm[pos] = e;
+          sm.Lmc[pos] = cinf >> 1;
         }
       }
-      nf += totf;
+      nb += totb;
+      ns += tots;
       nm += totm;
       slot ^= 1;
-      if (nf >= kFlush || nm >= kFlush) {
+      if (nb >= kFlush || ns >= kFlush || nm >= kFlush) {
         __syncthreads();
-        eval_full(nf);
-        eval_masked(nm);
+        if constexpr (ADJ) {
+          eval_adjoint(ns);
+        } else {
+          eval_big(nb);
+          eval_small(ns);
+          eval_masked(nm);
+        }
         __syncthreads();
-        nf = 0;
-        nm = 0;
+        nb = ns = nm = 0;
       }
     }
     __syncthreads();  // the cell table is rewritten by the next chunk
   }
-  if (nf > 0 || nm > 0) {
+  if (nb > 0 || ns > 0 || nm > 0) {
     __syncthreads();
-    eval_full(nf);
-    eval_masked(nm);
+    if constexpr (ADJ) {
+      eval_adjoint(ns);
+    } else {
+      eval_big(nb);
+      eval_small(ns);
+      eval_masked(nm);
+    }
   }
+  if constexpr (ADJ) return;
 
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
@@ -529,7 +720,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
                                                    unsigned long long* __restrict__ counts,
                                                    const float* __restrict__ cubic_table,
                                                    float support) {
-  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL> sm;
+  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false> sm;
   if constexpr (KERNEL == 2) {
     for (int i = threadIdx.x; i < kCubicTable; i += kBlock) sm.tab[i] = cubic_table[i];
   }
@@ -545,7 +736,6 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
 
 // Persistent form over many views: every workgroup takes the next
 // (view, tile) item from a device counter until the list is exhausted.
-// Items are ordered by estimated cost, largest first.
 template <int TW, int PPT, int OP, int KERNEL>
 __global__ __launch_bounds__(kBlock) void k_gather_batch(const view_desc* __restrict__ views,
                                                          const uint2* __restrict__ items,
@@ -553,7 +743,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_batch(const view_desc* __rest
                                                          uint32_t* __restrict__ counter,
                                                          const float* __restrict__ cubic_table,
                                                          float support) {
-  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL> sm;
+  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false> sm;
   __shared__ uint32_t next;
   if constexpr (KERNEL == 2) {
     for (int i = threadIdx.x; i < kCubicTable; i += kBlock) sm.tab[i] = cubic_table[i];
@@ -573,55 +763,75 @@ __global__ __launch_bounds__(kBlock) void k_gather_batch(const view_desc* __rest
   }
 }
 
-template <int TW, int PPT, int OP, int KERNEL>
-void launch_one(const grid_view& g, const point_view& p, double* acc, unsigned long long* counts,
-                const float* cubic_table, float support, hipStream_t s) {
-  constexpr int TH = (kBlock / TW) * PPT;
-  const uint32_t tiles_x = (p.lnx + TW - 1) / TW;
-  const uint32_t tiles_y = (p.lny + TH - 1) / TH;
-  const uint32_t ntiles = tiles_x * tiles_y;
-  if (!ntiles) return;
-  hipLaunchKernelGGL((k_gather<TW, PPT, OP, KERNEL>), dim3(ntiles), dim3(kBlock), 0, s, g, p,
-                     tiles_x, ntiles, acc, counts, cubic_table, support);
-  ILC_HIP(hipGetLastError());
+// Adjoint of the exact-mode quartic gather over one point image.
+template <int TW, int PPT>
+__global__ __launch_bounds__(kBlock) void k_adjoint(grid_view g, point_view p, uint32_t tiles_x,
+                                                    uint32_t ntiles, adjoint_io io) {
+  __shared__ gather_smem<false, 0, true> sm;
+  const uint32_t bid = blockIdx.x;
+  const uint32_t per = ntiles / 8u;
+  const uint32_t tile = bid < per * 8u ? (bid % 8u) * per + bid / 8u : bid;
+  gather_tile<TW, PPT, OP_ADJOINT, 0>(g, p, tile % tiles_x, tile / tiles_x, nullptr, nullptr,
+                                      1.0f, sm, &io);
 }
 
-template <int OP, int KERNEL>
-void launch_shape(const grid_view& g, const point_view& p, const tile_shape& ts, double* acc,
-                  unsigned long long* counts, const float* tab, float support, hipStream_t s) {
-  if (ts.tw == 16 && ts.ppt == 1)
-    launch_one<16, 1, OP, KERNEL>(g, p, acc, counts, tab, support, s);
-  else if (ts.tw == 32 && ts.ppt == 2)
-    launch_one<32, 2, OP, KERNEL>(g, p, acc, counts, tab, support, s);
-  else if (ts.tw == 32 && ts.ppt == 4)
-    launch_one<32, 4, OP, KERNEL>(g, p, acc, counts, tab, support, s);
-  else if (ts.tw == 64 && ts.ppt == 4)
-    launch_one<64, 4, OP, KERNEL>(g, p, acc, counts, tab, support, s);
-  else
-    fail(ILC_EINVAL, "gather: unsupported tile shape tw=%d ppt=%d", ts.tw, ts.ppt);
-}
-
-template <int TW, int PPT, int OP, int KERNEL>
-void launch_batch_one(const view_desc* views, const uint2* items, uint32_t nitems,
-                      uint32_t* counter, const float* tab, float support, int grid,
-                      hipStream_t s) {
-  hipLaunchKernelGGL((k_gather_batch<TW, PPT, OP, KERNEL>), dim3(grid), dim3(kBlock), 0, s, views,
-                     items, nitems, counter, tab, support);
-  ILC_HIP(hipGetLastError());
+// gradients in sorted order -> input order
+__global__ __launch_bounds__(kBlock) void k_unpermute4(const uint32_t* __restrict__ perm, uint32_t n,
+                                                        const float* __restrict__ a,
+                                                        const float* __restrict__ b,
+                                                        const float* __restrict__ c,
+                                                        const float* __restrict__ d, float* oa,
+                                                        float* ob, float* oc, float* od) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t i = perm[s];
+  oa[i] = a[s];
+  ob[i] = b[s];
+  oc[i] = c[s];
+  od[i] = d[s];
 }
 
 template <int TW, int PPT, int OP, int KERNEL>
-int batch_occupancy() {
-  int per_cu = 0;
-  ILC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_batch<TW, PPT, OP, KERNEL>,
-                                                      kBlock, 0));
-  return per_cu;
-}
+struct plain_launcher {
+  static int run(const grid_view& g, const point_view& p, double* acc,
+                 unsigned long long* counts, const float* tab, float support, hipStream_t s) {
+    constexpr int TH = (kBlock / TW) * PPT;
+    const uint32_t tiles_x = (p.lnx + TW - 1) / TW;
+    const uint32_t tiles_y = (p.lny + TH - 1) / TH;
+    const uint32_t ntiles = tiles_x * tiles_y;
+    if (!ntiles) return 0;
+    hipLaunchKernelGGL((k_gather<TW, PPT, OP, KERNEL>), dim3(ntiles), dim3(kBlock), 0, s, g, p,
+                       tiles_x, ntiles, acc, counts, tab, support);
+    ILC_HIP(hipGetLastError());
+    return 0;
+  }
+};
 
-// dispatch over (tile shape, op, kernel) to F<TW, PPT, OP, KERNEL>
+template <int TW, int PPT, int OP, int KERNEL>
+struct batch_launcher {
+  static int run(const view_desc* v, const uint2* it, uint32_t ni, uint32_t* c, const float* tab,
+                 float sup, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_gather_batch<TW, PPT, OP, KERNEL>), dim3(grid), dim3(kBlock), 0, s, v,
+                       it, ni, c, tab, sup);
+    ILC_HIP(hipGetLastError());
+    return 0;
+  }
+};
+
+template <int TW, int PPT, int OP, int KERNEL>
+struct batch_occ {
+  static int run() {
+    int per_cu = 0;
+    ILC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, k_gather_batch<TW, PPT, OP, KERNEL>, kBlock, 0));
+    return per_cu;
+  }
+};
+
+// dispatch over (tile shape, op, kernel) to F<TW, PPT, OP, KERNEL>::run
 template <template <int, int, int, int> class F, class... A>
-auto dispatch(const tile_shape& ts, int op, int kernel, A&&... a) {
-  auto by_kernel = [&](auto tw, auto ppt) {
+int dispatch(const tile_shape& ts, int op, int kernel, A&&... a) {
+  auto by_op = [&](auto tw, auto ppt) -> int {
     constexpr int TW = decltype(tw)::value, PPT = decltype(ppt)::value;
     if (op == OP_REFERENCE) {
       if (kernel == ILC_KERNEL_CUBIC) fail(ILC_EINVAL, "reference mode uses the quartic kernel");
@@ -636,28 +846,35 @@ auto dispatch(const tile_shape& ts, int op, int kernel, A&&... a) {
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
   using I4 = std::integral_constant<int, 4>;
-  if (ts.tw == 16 && ts.ppt == 1) return by_kernel(I16{}, I1{});
-  if (ts.tw == 32 && ts.ppt == 2) return by_kernel(I32{}, I2{});
-  if (ts.tw == 32 && ts.ppt == 4) return by_kernel(I32{}, I4{});
-  if (ts.tw == 64 && ts.ppt == 4) return by_kernel(I64{}, I4{});
+  if (ts.tw == 16 && ts.ppt == 1) return by_op(I16{}, I1{});
+  if (ts.tw == 32 && ts.ppt == 2) return by_op(I32{}, I2{});
+  if (ts.tw == 32 && ts.ppt == 4) return by_op(I32{}, I4{});
+  if (ts.tw == 64 && ts.ppt == 4) return by_op(I64{}, I4{});
   fail(ILC_EINVAL, "gather: unsupported tile shape tw=%d ppt=%d", ts.tw, ts.ppt);
 }
 
-template <int TW, int PPT, int OP, int KERNEL>
-struct batch_launcher {
-  static int run(const view_desc* v, const uint2* it, uint32_t ni, uint32_t* c, const float* tab,
-                 float sup, int grid, hipStream_t s) {
-    launch_batch_one<TW, PPT, OP, KERNEL>(v, it, ni, c, tab, sup, grid, s);
-    return 0;
-  }
-};
-
-template <int TW, int PPT, int OP, int KERNEL>
-struct batch_occ {
-  static int run() { return batch_occupancy<TW, PPT, OP, KERNEL>(); }
-};
-
 }  // namespace
+
+void launch_adjoint(const grid_view& g, const point_view& p, const float* grad, float scale,
+                    float norm, float* gw, float* gh, float* gx, float* gy, hipStream_t s) {
+  constexpr int TW = 32, PPT = 4, TH = (kBlock / TW) * PPT;
+  const uint32_t tiles_x = (p.lnx + TW - 1) / TW;
+  const uint32_t ntiles = tiles_x * ((p.lny + TH - 1) / TH);
+  if (!ntiles) return;
+  adjoint_io io{grad, scale, norm, gw, gh, gx, gy};
+  hipLaunchKernelGGL((k_adjoint<TW, PPT>), dim3(ntiles), dim3(kBlock), 0, s, g, p, tiles_x,
+                     ntiles, io);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_unpermute4(const uint32_t* perm, uint32_t n, const float* a, const float* b,
+                       const float* c, const float* d, float* oa, float* ob, float* oc, float* od,
+                       hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_unpermute4, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, perm, n, a,
+                     b, c, d, oa, ob, oc, od);
+  ILC_HIP(hipGetLastError());
+}
 
 void tile_grid(const tile_shape& ts, uint32_t lnx, uint32_t lny, uint32_t* tiles_x,
                uint32_t* ntiles) {
@@ -687,17 +904,11 @@ void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,
                    const float* cubic_table, hipStream_t s) {
   const float support = kernel == ILC_KERNEL_CUBIC ? 2.0f : 1.0f;
   if (op == OP_COUNT) {
-    launch_one<32, 4, OP_COUNT, 0>(g, p, nullptr, counts, nullptr, support, s);
+    // the count has its own tile shape: its cost is independent of the render's
+    plain_launcher<32, 4, OP_COUNT, 0>::run(g, p, nullptr, counts, nullptr, support, s);
     return;
   }
-  if (op == OP_REFERENCE) {
-    if (kernel == ILC_KERNEL_CUBIC) fail(ILC_EINVAL, "reference mode uses the quartic kernel");
-    launch_shape<OP_REFERENCE, 0>(g, p, ts, acc, counts, cubic_table, support, s);
-  } else if (kernel == ILC_KERNEL_CUBIC) {
-    launch_shape<OP_EXACT, 2>(g, p, ts, acc, counts, cubic_table, support, s);
-  } else {
-    launch_shape<OP_EXACT, 0>(g, p, ts, acc, counts, cubic_table, support, s);
-  }
+  dispatch<plain_launcher>(ts, op, kernel, g, p, acc, counts, cubic_table, support, s);
 }
 
 }  // namespace ilc

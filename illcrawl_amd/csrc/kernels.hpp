@@ -84,7 +84,7 @@ struct point_view {
   uint32_t lnx, lny;
 };
 
-enum gather_op { OP_REFERENCE = 0, OP_EXACT = 1, OP_COUNT = 2 };
+enum gather_op { OP_REFERENCE = 0, OP_EXACT = 1, OP_COUNT = 2, OP_ADJOINT = 3 };
 
 struct tile_shape {
   int tw;   // tile width (points)
@@ -94,6 +94,15 @@ struct tile_shape {
 void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,
                    const tile_shape& ts, double* acc, unsigned long long* counts,
                    const float* cubic_table, hipStream_t s);
+
+// Adjoint of the exact-mode quartic gather: adds into per-sorted-particle
+// gradient arrays (dL/dw, dL/dh, dL/dx, dL/dy) for dL/d(point value) = grad,
+// point value = scale * accumulator, W = norm * s^2.5 / h^2.
+void launch_adjoint(const grid_view& g, const point_view& p, const float* grad, float scale,
+                    float norm, float* gw, float* gh, float* gx, float* gy, hipStream_t s);
+void launch_unpermute4(const uint32_t* perm, uint32_t n, const float* a, const float* b,
+                       const float* c, const float* d, float* oa, float* ob, float* oc, float* od,
+                       hipStream_t s);
 
 // One view of a batch: its grid, its points, its accumulator, its tiling.
 struct view_desc {

@@ -150,6 +150,21 @@ ilc_status ilc_psg_run(ilc_psg* h, void* stream);
  * out_dev[i] = float(acc[i] * scale).  Unlike the reference, the stored
  * accumulator is not modified, so retrieving twice gives the same image. */
 ilc_status ilc_psg_retrieve_results(ilc_psg* h, float scale, float* out_dev, void* stream);
+
+/* Adjoint of run + retrieve_results(scale); the reference has no
+ * counterpart (it renders only).  For point values
+ *   value[i] = scale * sum_j w_j K s_ij^2.5 / h_j^2,  s = 1 - r_ij^2 / h_j^2,
+ * with the projected particles of setup_projected_particles (x, y, -, w) and
+ * their h, and dl_dvalue_dev[i] = dL/dvalue[i] (npts floats, device), writes
+ * per particle, in input order (n floats each, device):
+ *   dl_dw = dL/dw, dl_dh = dL/dh, dl_dx = dL/dx, dl_dy = dL/dy
+ * (x, y in the projected frame).  Weights are recomputed, not stored.
+ * Exact mode and the quartic kernels only (ILC_EINVAL otherwise): the
+ * reference-mode search box makes the image a non-smooth function of the
+ * particles.  Gradients are FP32 sums whose order varies from call to call
+ * (atomic adds), so repeated calls agree to rounding, not bit for bit. */
+ilc_status ilc_psg_backward(ilc_psg* h, const float* dl_dvalue_dev, float scale, float* dl_dw_dev,
+                            float* dl_dh_dev, float* dl_dx_dev, float* dl_dy_dev, void* stream);
 /* The FP64 accumulator itself (n doubles). */
 ilc_status ilc_psg_retrieve_results_f64(ilc_psg* h, double* out_dev, void* stream);
 
@@ -181,6 +196,12 @@ ilc_status ilc_psg_count_evals(ilc_psg* h, uint64_t* counts_dev, uint64_t* total
  * (camera.cpp:124-155). */
 ilc_status ilc_camera_basis(const ilc_camera* cam, double b0[3], double b1[3],
                             double min_position[3], double* pixel_size);
+
+/* The FP32 projection ilc_project_particles applies, x' = minv (x - center)
+ * (projective_smoothing_reconstruction.cpp:175-193, math.cl:45-91): minv is
+ * row-major.  Its first two rows are the Jacobian that carries projected-frame
+ * gradients back to world positions. */
+ilc_status ilc_camera_projection(const ilc_camera* cam, float minv[9], float center[3]);
 
 /* projective_smoothing::project_particles (projective_smoothing_reconstruction.cpp:175-205;
  * projection.cl:56-95): out4 = camera-frame float4 with z = 0 and w = 0 for

@@ -49,6 +49,7 @@ PINNING = {
     "orc_tabulated_2d": "hand-computed bilinear half-texel answers",
     "orc_temperature": "closed form (gamma-1) u mu 121.1475502",
     "orc_xray_bolometric": "closed form on table-node-free inputs",
+    "orc_backward_exact": "FP64 central differences of orc_gather_exact in w, h, x, y",
 }
 
 
@@ -125,6 +126,9 @@ def _declare(L):
     L.orc_xray_bolometric.argtypes = [_f32p, _f32p, _f32p, _u64, _f32p, ctypes.c_int32,
                                       ctypes.c_float, ctypes.c_float, _f32p]
     L.orc_xray_bolometric.restype = None
+    L.orc_backward_exact.argtypes = [_f32p, _f32p, _u64, _f32p, _u64, _f64p, ctypes.c_int, _f64p,
+                                     _f64p, _f64p, _f64p]
+    L.orc_backward_exact.restype = None
     L.orc_num_threads.argtypes = []
     L.orc_num_threads.restype = ctypes.c_int
     L.orc_set_num_threads.argtypes = [ctypes.c_int]
@@ -294,6 +298,18 @@ def xray_bolometric(rho, temp, xe, gtab, gx0, gdx):
     lib().orc_xray_bolometric(rho, temp, xe, rho.shape[0], gtab, gtab.shape[0],
                               np.float32(gx0), np.float32(gdx), out)
     return out
+
+
+def backward_exact(p4, h, pts4, G, kind=K_QUARTIC_REF):
+    """FP64 gradients (dL/dw, dL/dh, dL/dx, dL/dy) of sum_t G[t] out[t]."""
+    p4 = _c(p4, np.float32)
+    h = _c(h, np.float32)
+    pts4 = _c(pts4, np.float32)
+    G = _c(G, np.float64)
+    n = p4.shape[0]
+    out = [np.zeros(n, np.float64) for _ in range(4)]
+    lib().orc_backward_exact(p4, h, n, pts4, pts4.shape[0], G, int(kind), *out)
+    return tuple(out)
 
 
 def num_threads():

@@ -719,3 +719,47 @@ void orc_set_num_threads(int n)
   (void)n;
 #endif
 }
+
+/* ------------------------------------------------------------------ */
+/* FP64 adjoint of orc_gather_exact (quartic kernels)                  */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Gradients of L with dL/d out[t] = G[t] (out as in orc_gather_exact, with
+ * kind QUARTIC_REF or QUARTIC) w.r.t. each particle's w, h and camera-frame
+ * x, y.  With s = 1 - r^2/h^2, W = K s^2.5 / h^2:
+ *   dW/dh = K s^1.5 (7 r^2/h^2 - 2) / h^3,
+ *   dW/dx = -5 K s^1.5 (x - p_x) / h^4   (and y alike).
+ * No reference counterpart (the reference has no adjoint); the formulas are
+ * pinned by central differences of orc_gather_exact in tests/.
+ */
+void orc_backward_exact(const float* p4, const float* h, uint64_t n, const float* pts4,
+                        uint64_t npts, const double* G, int kind, double* gw, double* gh,
+                        double* gx, double* gy)
+{
+  const double K = kind == ORC_K_QUARTIC_REF ? 2.0 * (double)1.04445f * (double)1.066667f
+                                             : 3.5 / M_PI;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t j = 0; j < (int64_t)n; ++j) {
+    const double w = p4[4 * j + 3];
+    const double px = p4[4 * j], py = p4[4 * j + 1], hh = h[j];
+    const double h2 = hh * hh;
+    double sw = 0, sh = 0, sx = 0, sy = 0;
+    for (uint64_t t = 0; t < npts; ++t) {
+      const double dx = px - (double)pts4[4 * t], dy = py - (double)pts4[4 * t + 1];
+      const double r2 = dx * dx + dy * dy;
+      if (!(r2 < h2)) continue;
+      const double s = 1.0 - r2 / h2;
+      const double s15 = s * sqrt(s);
+      const double g = G[t];
+      sw += g * K * s15 * s / h2;
+      sh += g * K * s15 * (7.0 * r2 / h2 - 2.0) / (h2 * hh);
+      sx += g * (-5.0) * K * s15 * dx / (h2 * h2);
+      sy += g * (-5.0) * K * s15 * dy / (h2 * h2);
+    }
+    gw[j] = sw;
+    gh[j] = w * sh;
+    gx[j] = w * sx;
+    gy[j] = w * sy;
+  }
+}

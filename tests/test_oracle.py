@@ -348,3 +348,40 @@ def test_xray_bolometric_closed_form(orc):
     w = orc.xray_bolometric([rho], [T], [xe], gy, x0, dx)[0]
     exp = (4.046263 * 2e-6 * 1.1) ** 2 * math.sqrt(float(T)) * float(gy[i])
     assert w == pytest.approx(exp, rel=3e-5)
+
+
+# ---------------------------------------------------------------- adjoint
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_backward_exact_vs_central_differences(orc, kind):
+    rng = np.random.default_rng(17)
+    n = 40
+    p = np.zeros((n, 4), np.float32)
+    p[:, :2] = rng.uniform(-3, 3, (n, 2))
+    p[:, 3] = rng.lognormal(0, 0.3, n)
+    h = rng.uniform(0.8, 2.5, n).astype(np.float32)
+    pts = np.zeros((30 * 30, 4), np.float32)
+    xs = np.linspace(-4, 4, 30, dtype=np.float32)
+    X, Y = np.meshgrid(xs, xs)
+    pts[:, 0], pts[:, 1] = X.reshape(-1), Y.reshape(-1)
+    G = rng.normal(size=pts.shape[0])
+    gw, gh, gx, gy = orc.backward_exact(p, h, pts, G, kind)
+
+    def L(pp, hh):
+        return float(np.dot(G, orc.gather_exact(pp, hh, pts, kind)))
+
+    for j in (0, 7, 19, 33):
+        # central differences in float32-representable steps (inputs are float32)
+        for comp, g in ((3, gw), (0, gx), (1, gy)):
+            eps = 1e-3 * max(abs(float(p[j, comp])), 1.0)
+            a, b = p.copy(), p.copy()
+            a[j, comp] += eps
+            b[j, comp] -= eps
+            fd = (L(a, h) - L(b, h)) / (float(a[j, comp]) - float(b[j, comp]))
+            assert g[j] == pytest.approx(fd, rel=2e-3, abs=1e-6 * abs(g).max())
+        eps = 1e-3 * float(h[j])
+        ha, hb = h.copy(), h.copy()
+        ha[j] += eps
+        hb[j] -= eps
+        fd = (L(p, ha) - L(p, hb)) / (float(ha[j]) - float(hb[j]))
+        assert gh[j] == pytest.approx(fd, rel=2e-3, abs=1e-6 * abs(gh).max())
