@@ -85,6 +85,9 @@ SIGNATURES = {
                                     ctypes.POINTER(ctypes.c_float * 3)]),
     "ilc_project_particles": (_ST, [ctypes.POINTER(CameraC), _P, _U64, _P, _P]),
     "ilc_camera_pixel_points": (_ST, [ctypes.POINTER(CameraC), _P, _P]),
+    "ilc_camera_generate_pixel_coordinates": (_ST, [ctypes.POINTER(CameraC), _P, _P]),
+    "ilc_project_evaluation_points": (_ST, [ctypes.POINTER(CameraC), _P, _U64, _P, _P]),
+    "ilc_vector_scale": (_ST, [_P, _U64, ctypes.c_float, _P]),
     "ilc_quantity_scaled": (_ST, [_P, ctypes.c_double, _U64, _P, _P]),
     "ilc_quantity_xray_bolometric": (_ST, [_P, _P, _P, _U64, _P, ctypes.c_int32, ctypes.c_float,
                                            ctypes.c_float, _P, _P]),

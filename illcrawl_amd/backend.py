@@ -181,6 +181,33 @@ def camera_pixel_points(cam, device=0, stream=None):
     return out
 
 
+def camera_generate_pixel_coordinates(cam, device=0, stream=None):
+    """camera::generate_pixel_coordinates: world-frame pixel corners (nx*ny, 4)."""
+    c = N.camera_struct(cam)
+    out = torch.empty((c.nx * c.ny, 4), dtype=torch.float32, device=torch.device("cuda", device))
+    N.call("ilc_camera_generate_pixel_coordinates", ctypes.byref(c), _ptr(out), _stream(stream))
+    return out
+
+
+def project_evaluation_points(cam, points4, device=0, stream=None, out=None):
+    """projective_smoothing::project_evaluation_points (z and w kept)."""
+    c = N.camera_struct(cam)
+    p = _dev_f32(points4, torch.device("cuda", device), 4)
+    if out is None:
+        out = torch.empty_like(p)
+    N.call("ilc_project_evaluation_points", ctypes.byref(c), _ptr(p), p.shape[0], _ptr(out),
+           _stream(stream))
+    return out
+
+
+def vector_scale(x, alpha, stream=None):
+    """vector_scale: x *= alpha in FP32, in place (x: contiguous float32 CUDA tensor)."""
+    if x.dtype != torch.float32 or not x.is_cuda or not x.is_contiguous():
+        raise ValueError("vector_scale needs a contiguous float32 CUDA tensor")
+    N.call("ilc_vector_scale", _ptr(x), x.numel(), ctypes.c_float(alpha), _stream(stream))
+    return x
+
+
 def project_particles(cam, particles4, device=0, stream=None, out=None):
     """projective_smoothing::project_particles (depth mask, z := 0)."""
     c = N.camera_struct(cam)

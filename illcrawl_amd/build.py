@@ -17,6 +17,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libilcpsg.so")
+# C++ program driving the library through include/illcrawl_backend.hpp (GPU tests run it)
+DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "backend_demo.cpp")
+DEMO = os.path.join(LIBDIR, "backend_demo")
 
 SOURCES = ["host.cpp", "gridbuild.cpp", "capi.cpp", "batch.cpp", "grid.hip", "gather.hip",
            "project.hip", "batch.hip"]
@@ -67,6 +70,11 @@ def build(verbose=False, jobs=None):
         cmd = [cc, "-shared", f"--offload-arch={ARCH}", "-o", tmp_lib] + objs
         subprocess.run(cmd, check=True)
         os.replace(tmp_lib, LIB)
+        if os.path.exists(DEMO_SRC):
+            cmd = [cc, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+                   DEMO_SRC, "-L", LIBDIR, "-lilcpsg", "-Wl,-rpath,$ORIGIN", "-o", DEMO + ".tmp"]
+            subprocess.run(cmd, check=True)
+            os.replace(DEMO + ".tmp", DEMO)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return LIB

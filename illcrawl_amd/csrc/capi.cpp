@@ -370,6 +370,33 @@ ilc_status ilc_camera_pixel_points(const ilc_camera* cam, float* out4_dev, void*
   });
 }
 
+ilc_status ilc_camera_generate_pixel_coordinates(const ilc_camera* cam, float* out4_dev,
+                                                 void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(cam && out4_dev, "null argument");
+    const camera_frame f = make_camera_frame(*cam);
+    launch_world_pixels(f, cam->nx, cam->ny, reinterpret_cast<float4*>(out4_dev),
+                        as_stream(stream));
+  });
+}
+
+ilc_status ilc_project_evaluation_points(const ilc_camera* cam, const float* in4_dev, uint64_t n,
+                                         float* out4_dev, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(cam && in4_dev && out4_dev, "null argument");
+    const camera_frame f = make_camera_frame(*cam);
+    launch_project_points(f, reinterpret_cast<const float4*>(in4_dev), n,
+                          reinterpret_cast<float4*>(out4_dev), as_stream(stream));
+  });
+}
+
+ilc_status ilc_vector_scale(float* x_dev, uint64_t n, float alpha, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(x_dev || n == 0, "null argument");
+    launch_vector_scale(x_dev, alpha, n, as_stream(stream));
+  });
+}
+
 ilc_status ilc_quantity_scaled(const float* in_dev, double scale, uint64_t n, float* w_dev,
                                void* stream) {
   return guarded([&] {

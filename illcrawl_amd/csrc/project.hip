@@ -75,6 +75,44 @@ __global__ __launch_bounds__(kBlock) void k_pixel_points(pix_args a, uint32_t nx
                        dot3(a.pa.m + 6, vx, vy, vz), 0.0f);
 }
 
+// camera_generate_pixel_coordinates (camera.cl:23-41): world-frame corners,
+// min + ix*ps*b0 + iy*ps*b1 evaluated left to right in FP32, w = 0.
+__global__ __launch_bounds__(kBlock) void k_world_pixels(pix_args a, uint32_t nx, uint32_t ny,
+                                                          float4* out) {
+#pragma clang fp contract(off)
+  const uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+  if (i >= uint64_t(nx) * ny) return;
+  const uint32_t iy = uint32_t(i / nx), ix = uint32_t(i - uint64_t(iy) * nx);
+  const float sx = float(ix) * a.ps;
+  const float sy = float(iy) * a.ps;
+  float q[3];
+  for (int c = 0; c < 3; ++c) {
+    const float t0 = sx * a.b0[c];
+    const float t1 = a.mn[c] + t0;
+    const float t2 = sy * a.b1[c];
+    q[c] = t1 + t2;
+  }
+  out[i] = make_float4(q[0], q[1], q[2], 0.0f);
+}
+
+// project_evaluation_points (projection.cl:97-122): z and w kept.
+__global__ __launch_bounds__(kBlock) void k_project_points(proj_args pa, const float4* in,
+                                                            uint64_t n, float4* out) {
+#pragma clang fp contract(off)
+  const uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = in[i];
+  const float vx = q.x - pa.c[0], vy = q.y - pa.c[1], vz = q.z - pa.c[2];
+  out[i] = make_float4(dot3(pa.m + 0, vx, vy, vz), dot3(pa.m + 3, vx, vy, vz),
+                       dot3(pa.m + 6, vx, vy, vz), q.w);
+}
+
+// vector_scale (math.cl:94-102): x *= alpha in FP32.
+__global__ __launch_bounds__(kBlock) void k_vector_scale(float* x, float alpha, uint64_t n) {
+  const uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+  if (i < n) x[i] *= alpha;
+}
+
 __global__ __launch_bounds__(kBlock) void k_retrieve(const double* acc, uint64_t n, float scale,
                                                       float* out) {
   const uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
@@ -181,6 +219,36 @@ void launch_pixel_points(const camera_frame& f, uint32_t nx, uint32_t ny, float4
   a.ps = f.psf;
   hipLaunchKernelGGL(k_pixel_points, dim3(nblocks(uint64_t(nx) * ny)), dim3(kBlock), 0, s, a, nx,
                      ny, out);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_world_pixels(const camera_frame& f, uint32_t nx, uint32_t ny, float4* out,
+                         hipStream_t s) {
+  if (!nx || !ny) return;
+  pix_args a;
+  a.pa = make_proj(f);
+  for (int c = 0; c < 3; ++c) {
+    a.mn[c] = f.min_corner[c];
+    a.b0[c] = f.b0f[c];
+    a.b1[c] = f.b1f[c];
+  }
+  a.ps = f.psf;
+  hipLaunchKernelGGL(k_world_pixels, dim3(nblocks(uint64_t(nx) * ny)), dim3(kBlock), 0, s, a, nx,
+                     ny, out);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_project_points(const camera_frame& f, const float4* in, uint64_t n, float4* out,
+                           hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_project_points, dim3(nblocks(n)), dim3(kBlock), 0, s, make_proj(f), in, n,
+                     out);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_vector_scale(float* x, float alpha, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_vector_scale, dim3(nblocks(n)), dim3(kBlock), 0, s, x, alpha, n);
   ILC_HIP(hipGetLastError());
 }
 

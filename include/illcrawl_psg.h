@@ -214,6 +214,23 @@ ilc_status ilc_project_particles(const ilc_camera* cam, const float* in4_dev, ui
  * pixel-corner points in the camera frame. */
 ilc_status ilc_camera_pixel_points(const ilc_camera* cam, float* out4_dev, void* stream);
 
+/* camera::generate_pixel_coordinates alone (camera.cpp:157-189,
+ * camera.cl:23-41): nx*ny world-frame pixel-corner float4 (w = 0). */
+ilc_status ilc_camera_generate_pixel_coordinates(const ilc_camera* cam, float* out4_dev,
+                                                 void* stream);
+
+/* projective_smoothing::project_evaluation_points (projective_smoothing_reconstruction.cpp:144-172,
+ * projection.cl:97-122): n world-frame float4 points into the camera frame, z
+ * and w kept.  Unlike the reference (which overwrites the caller's buffer),
+ * the result goes to out4_dev; in4_dev may equal out4_dev. */
+ilc_status ilc_project_evaluation_points(const ilc_camera* cam, const float* in4_dev, uint64_t n,
+                                         float* out4_dev, void* stream);
+
+/* vector_scale (math.cl:94-102), as projective_smoothing::retrieve_results
+ * applies the quantity's effective dA (projective_smoothing_reconstruction.cpp:49-84):
+ * x[i] *= alpha in FP32, in place. */
+ilc_status ilc_vector_scale(float* x_dev, uint64_t n, float alpha, void* stream);
+
 /* ------------------------------------------------- quantity transformation */
 
 /* quantity_transformation with unprocessed_quantity / constant_quantity
