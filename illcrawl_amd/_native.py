@@ -50,6 +50,11 @@ class CameraC(ctypes.Structure):
                 ("ny", ctypes.c_uint32), ("depth", ctypes.c_double)]
 
 
+class VolumeC(ctypes.Structure):
+    _fields_ = [("center", ctypes.c_double * 3), ("extent", ctypes.c_double * 3),
+                ("periodic_wraparound", ctypes.c_double * 3), ("border", ctypes.c_double)]
+
+
 class ViewC(ctypes.Structure):
     _fields_ = [("cam", CameraC), ("quantity", ctypes.c_int32), ("scale", ctypes.c_float)]
 
@@ -88,6 +93,8 @@ SIGNATURES = {
     "ilc_camera_generate_pixel_coordinates": (_ST, [ctypes.POINTER(CameraC), _P, _P]),
     "ilc_project_evaluation_points": (_ST, [ctypes.POINTER(CameraC), _P, _U64, _P, _P]),
     "ilc_vector_scale": (_ST, [_P, _U64, ctypes.c_float, _P]),
+    "ilc_crawl_block": (_ST, [ctypes.POINTER(VolumeC), _P, _U64, _P, ctypes.c_int32, _P, _P,
+                              _P, ctypes.POINTER(_U64), _P]),
     "ilc_quantity_scaled": (_ST, [_P, ctypes.c_double, _U64, _P, _P]),
     "ilc_quantity_xray_bolometric": (_ST, [_P, _P, _P, _U64, _P, ctypes.c_int32, ctypes.c_float,
                                            ctypes.c_float, _P, _P]),
@@ -100,6 +107,7 @@ SIGNATURES = {
     "ilc_batch_image": (_ST, [_P, ctypes.c_int32, _P, _P]),
     "ilc_batch_image_f64": (_ST, [_P, ctypes.c_int32, _P, _P]),
     "ilc_batch_count_evals": (_ST, [_P, ctypes.POINTER(_U64), _P]),
+    "ilc_batch_view_info": (_ST, [_P, ctypes.c_int32, ctypes.POINTER(GridInfo)]),
     "ilc_batch_set_tile": (_ST, [_P, ctypes.c_int32, ctypes.c_int32]),
     "ilc_batch_last_gather_ms": (_ST, [_P, ctypes.POINTER(ctypes.c_float)]),
     "ilc_batch_autotune": (_ST, [_P, ctypes.POINTER(ctypes.c_int32),

@@ -219,6 +219,34 @@ def project_particles(cam, particles4, device=0, stream=None, out=None):
     return out
 
 
+def crawl_block(center, extent, periodic_wraparound, border, xyz, columns=(), device=0,
+                stream=None, with_index=False):
+    """Block body of the data crawler (reconstructing_data_crawler.cpp:180-213):
+    periodic correction towards the render volume, cutout, compaction in input
+    order.  Returns (particles4 (m, 4) with w = 0, [columns (m,)], [index (m,)])."""
+    dev = torch.device("cuda", device)
+    x = _dev_f32(xyz, dev, 3)
+    n = x.shape[0]
+    cols = [_dev_f32(c, dev).reshape(-1) for c in columns]
+    if any(c.shape[0] != n for c in cols):
+        raise ValueError("every column needs one value per row")
+    out4 = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    outs = [torch.empty(n, dtype=torch.float32, device=dev) for _ in cols]
+    idx = torch.empty(n, dtype=torch.int64, device=dev) if with_index else None
+    vol = N.VolumeC((ctypes.c_double * 3)(*center), (ctypes.c_double * 3)(*extent),
+                    (ctypes.c_double * 3)(*periodic_wraparound), float(border))
+    k = max(len(cols), 1)
+    cin = (ctypes.c_void_p * k)(*[c.data_ptr() for c in cols])
+    cout = (ctypes.c_void_p * k)(*[c.data_ptr() for c in outs])
+    m = ctypes.c_uint64()
+    N.call("ilc_crawl_block", ctypes.byref(vol), _ptr(x), n, ctypes.cast(cin, ctypes.c_void_p),
+           len(cols), _ptr(out4), ctypes.cast(cout, ctypes.c_void_p), _ptr(idx), ctypes.byref(m),
+           _stream(stream))
+    m = m.value
+    res = (out4[:m], [o[:m] for o in outs])
+    return res + (idx[:m],) if with_index else res
+
+
 def quantity_scaled(inp, scale, n=None, device=0, stream=None):
     """quantity_transformation with unprocessed_quantity (inp given) or
     constant_quantity (inp None): w = float(scale * inp)."""
@@ -364,6 +392,14 @@ class BatchRenderer:
         arr = (ctypes.c_uint64 * len(self.views))()
         N.call("ilc_batch_count_evals", self._h, arr, _stream(stream))
         return [int(x) for x in arr]
+
+    def view_info(self, view):
+        """Grid of one view after a render (num_particles: particles it holds)."""
+        gi = N.GridInfo()
+        N.call("ilc_batch_view_info", self._h, int(view), ctypes.byref(gi))
+        return dict(num_cells=list(gi.num_cells), min_corner=list(gi.min_corner),
+                    cell_size=list(gi.cell_size), max_h=gi.max_h,
+                    num_particles=gi.num_particles)
 
     def set_tile(self, w, h):
         N.call("ilc_batch_set_tile", self._h, int(w), int(h))

@@ -21,7 +21,8 @@ LIB = os.path.join(LIBDIR, "libilcpsg.so")
 DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "backend_demo.cpp")
 DEMO = os.path.join(LIBDIR, "backend_demo")
 
-SOURCES = ["host.cpp", "gridbuild.cpp", "capi.cpp", "batch.cpp", "grid.hip", "gather.hip",
+SOURCES = ["host.cpp", "gridbuild.cpp", "capi.cpp", "batch.cpp", "crawl.hip", "grid.hip",
+           "gather.hip",
            "project.hip", "batch.hip"]
 
 ARCH = "gfx950"

@@ -50,6 +50,9 @@ struct view_state {
   grid_params gp{};
   int32_t kz0 = 0;
   float max_h = 0.f;
+  bool windowed = false;  // grid built from the particles in `win` only
+  window_f win{};
+  uint32_t nwin = 0;      // particles in the window (from the plan)
 };
 
 template <class T>
@@ -90,6 +93,9 @@ struct ilc_batch {
   std::vector<std::unique_ptr<view_state>> views;
   grid_scratch scratch;
   dbuf<float4> p4tmp;
+  dbuf<float> htmp;
+  dbuf<uint32_t> wcounts, woffs, wtotal;
+  pinned<uint32_t> wtotal_host;
   dbuf<float> bbox_parts;
   dbuf<bbox_result> bbox_dev;
   dbuf<uint32_t> bad;
@@ -215,6 +221,59 @@ void plan(ilc_batch* b, hipStream_t s) {
     for (uint32_t x : bits) sig.push_back(x);
     sig.push_back(uint64_t(vs.kz0));
   }
+  // windows: the screen grown by the largest reach of any point (F3 radius
+  // <= max h; support up to 2 h) plus whole cells.  Reference mode needs every
+  // point's own cell inside the grid (an out-of-grid point's unclamped key
+  // can alias any cell), else the view keeps all particles.
+  bool any_win = false;
+  for (size_t v = 0; v < nv; ++v) {
+    view_state& vs = *b->views[v];
+    const double ps = vs.v.cam.width / double(vs.v.cam.nx);
+    const double hx = (0.5 * vs.v.cam.nx + 2.0) * ps, hy = (0.5 * vs.v.cam.ny + 2.0) * ps;
+    const double cs = std::max(vs.gp.cs[0], vs.gp.cs[1]);
+    const double reach = 2.0 * double(vs.max_h) + 4.0 * cs;
+    bool ok = std::isfinite(reach) && std::isfinite(hx) && std::isfinite(hy);
+    if (ok && b->mode == ILC_MODE_REFERENCE) {
+      for (int a = 0; a < 2; ++a) {
+        const double e = a == 0 ? hx : hy;
+        const double lo = std::floor((-e - double(vs.gp.minc[a])) / double(vs.gp.cs[a]));
+        const double hi = std::floor((e - double(vs.gp.minc[a])) / double(vs.gp.cs[a]));
+        ok = ok && lo >= 1.0 && hi <= double(vs.gp.nc[a]) - 2.0;
+      }
+    }
+    const bbox_result& bb = b->bbox_host.p[v];
+    // no gain when the window holds the whole bounding box
+    if (ok && -hx - reach <= bb.mn[0] && hx + reach >= bb.mx[0] && -hy - reach <= bb.mn[1] &&
+        hy + reach >= bb.mx[1])
+      ok = false;
+    vs.windowed = ok;
+    if (ok) {
+      vs.win = window_f{std::nextafter(float(-hx - reach), -INFINITY),
+                        std::nextafter(float(hx + reach), INFINITY),
+                        std::nextafter(float(-hy - reach), -INFINITY),
+                        std::nextafter(float(hy + reach), INFINITY)};
+      any_win = true;
+    }
+  }
+  if (any_win) {
+    const uint32_t nt = window_tiles(b->ps.n);
+    b->wcounts.ensure(size_t(nt) + 1);
+    b->wtotal.ensure(nv);
+    b->wtotal_host.ensure(nv);
+    launch_fill_u32(b->wtotal.p, 0u, nv, s);
+    for (size_t v = 0; v < nv; ++v)
+      if (b->views[v]->windowed)
+        launch_window_count(b->ps, proj_of(b, *b->views[v]), b->views[v]->win, b->wcounts.p,
+                            b->wtotal.p + v, s);
+    ILC_HIP(hipMemcpyAsync(b->wtotal_host.p, b->wtotal.p, sizeof(uint32_t) * nv,
+                           hipMemcpyDeviceToHost, s));
+    ILC_HIP(hipStreamSynchronize(s));
+  }
+  for (size_t v = 0; v < nv; ++v) {
+    view_state& vs = *b->views[v];
+    vs.nwin = vs.windowed ? b->wtotal_host.p[v] : 0u;
+    sig.push_back(vs.windowed ? uint64_t(vs.nwin) : ~uint64_t(0));
+  }
   sig.push_back(uint64_t(b->ts.tw) << 32 | uint64_t(b->ts.ppt));
   // work items: largest views first, tiles in image order
   std::vector<uint32_t> order(nv);
@@ -268,9 +327,28 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
   launch_fill_u32(b->counter.p, 0u, 2, s);
   for (size_t v = 0; v < nv; ++v) {
     view_state& vs = *b->views[v];
-    launch_view_project(b->ps, proj_of(b, vs), gaunt_of(b), b->p4tmp.p, s);
-    enqueue_grid_build(vs.grid, b->scratch, b->p4tmp.p, b->ps.hsml, b->ps.n, vs.gp, vs.kz0,
-                       vs.max_h, b->kernel, b->counter.p + 1, s);
+    if (vs.windowed) {
+      // kept particles in input order, then the grid from their keys
+      const uint32_t nt = window_tiles(b->ps.n);
+      const uint64_t m = vs.nwin;
+      b->wcounts.ensure(size_t(nt) + 1);
+      b->woffs.ensure(size_t(nt) + 1);
+      b->htmp.ensure(std::max<uint64_t>(m, 1));
+      vs.grid.key.ensure(std::max<uint64_t>(m, 1));
+      vs.grid.perm.ensure(std::max<uint64_t>(m, 1));
+      b->scratch.temp.ensure(scan_temp_words(uint64_t(nt) + 1));
+      launch_window_count(b->ps, proj_of(b, vs), vs.win, b->wcounts.p, nullptr, s);
+      launch_exclusive_scan(b->wcounts.p, b->woffs.p, nt, b->scratch.temp.p, s);
+      launch_window_scatter(b->ps, proj_of(b, vs), gaunt_of(b), vs.win, b->woffs.p, vs.gp,
+                            b->p4tmp.p, b->htmp.p, vs.grid.key.p, vs.grid.perm.p,
+                            b->counter.p + 1, s);
+      enqueue_grid_from_keys(vs.grid, b->scratch, b->p4tmp.p, b->htmp.p, m, vs.gp, vs.kz0,
+                             vs.max_h, b->kernel, s);
+    } else {
+      launch_view_project(b->ps, proj_of(b, vs), gaunt_of(b), b->p4tmp.p, s);
+      enqueue_grid_build(vs.grid, b->scratch, b->p4tmp.p, b->ps.hsml, b->ps.n, vs.gp, vs.kz0,
+                         vs.max_h, b->kernel, b->counter.p + 1, s);
+    }
     const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
     launch_fill_f64(vs.acc.p, 0.0, P, s);
     view_desc& d = b->desc_host.p[v];
@@ -472,6 +550,22 @@ ilc_status ilc_batch_render(ilc_batch* b, int32_t flags, void* stream) {
     ILC_HIP(hipStreamWaitEvent(s, b->ev_out, 0));
     ILC_HIP(hipStreamSynchronize(s));
     check_flags(b);
+  });
+}
+
+ilc_status ilc_batch_view_info(const ilc_batch* b, int32_t view, ilc_grid_info* out) {
+  return guarded([&] {
+    ILC_REQUIRE(b && out, "null argument");
+    ILC_REQUIRE(view >= 0 && size_t(view) < b->views.size(), "view %d out of range", view);
+    const view_state& vs = *b->views[view];
+    if (!vs.grid.ready) fail(ILC_ENOTREADY, "view %d has no grid (render first)", view);
+    for (int a = 0; a < 3; ++a) {
+      out->num_cells[a] = vs.grid.gp.nc[a];
+      out->min_corner[a] = vs.grid.gp.minc[a];
+      out->cell_size[a] = vs.grid.gp.cs[a];
+    }
+    out->max_h = vs.grid.max_h;
+    out->num_particles = vs.grid.n;
   });
 }
 

@@ -153,7 +153,151 @@ __global__ __launch_bounds__(kBlock) void k_view_pass(particle_soa ps, view_proj
   }
 }
 
+// ---- windowed grid input
+//
+// Only the cells that some point of the view can reach matter to its image
+// (the gather never visits the others), so the batch builds each view's grid
+// from the particles whose projection falls in a window around the screen:
+// the screen grown by more than the largest reach plus whole cells, so that
+// every cell a point can reach keeps all of its particles.  Kept particles
+// stay in input order (order-preserving compaction), so the within-cell order
+// of the reachable cells, their counts and max h, and hence the images, are
+// the ones of the full grid.
+
+// as grid.hip's (grid3d_get_cell_indices)
+__device__ inline int32_t floor_to_int(float f) {
+  const float fl = floorf(f);
+  if (!(fl == fl)) return 0;
+  if (fl >= 2147483647.0f) return INT32_MAX;
+  if (fl <= -2147483648.0f) return INT32_MIN;
+  return static_cast<int32_t>(fl);
+}
+
+constexpr int kWinItems = 16;                   // particles per thread
+constexpr int kWinTile = kBlock * kWinItems;    // particles per workgroup
+
+__device__ inline bool in_window(const float* pos3, const view_proj& vp, const window_f& win,
+                                 uint64_t i, float* px, float* py) {
+#pragma clang fp contract(off)
+  const float x = pos3[3 * i], y = pos3[3 * i + 1], z = pos3[3 * i + 2];
+  const float vx = x - vp.c[0], vy = y - vp.c[1], vz = z - vp.c[2];
+  *px = dot3(vp.m + 0, vx, vy, vz);
+  *py = dot3(vp.m + 3, vx, vy, vz);
+  return *px >= win.x0 && *px <= win.x1 && *py >= win.y0 && *py <= win.y1;
+}
+
+// counts[b] = kept particles of workgroup b's tile; *total += the same
+__global__ __launch_bounds__(kBlock) void k_window_count(const float* __restrict__ pos3,
+                                                          uint64_t n, view_proj vp, window_f win,
+                                                          uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ total) {
+  __shared__ uint32_t wc[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
+  uint32_t c = 0;
+  for (int r = 0; r < kWinItems; ++r) {
+    const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
+    float px, py;
+    const bool keep = i < n && in_window(pos3, vp, win, i, &px, &py);
+    c += __popcll(__ballot(keep));
+  }
+  if (lane == 0) wc[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = wc[0] + wc[1] + wc[2] + wc[3];
+    counts[blockIdx.x] = t;
+    if (total && t) atomicAdd(total, t);
+  }
+}
+
+// Kept particles in input order: offsets = exclusive scan of the counts;
+// writes the projected float4 (x', y', 0, w with the depth mask), h, the cell
+// key (as k_keys computes it) and the compacted index.
+__global__ __launch_bounds__(kBlock) void k_window_scatter(
+    particle_soa ps, view_proj vp, gaunt_table gt, window_f win,
+    const uint32_t* __restrict__ offsets, grid_params g, float4* __restrict__ p4c,
+    float* __restrict__ hc, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+    uint32_t* __restrict__ err) {
+#pragma clang fp contract(off)
+  __shared__ float tab[kMaxGaunt];
+  __shared__ uint32_t wc[2][4];
+  if (vp.quantity == 1) {
+    for (int i = threadIdx.x; i < gt.n; i += kBlock) tab[i] = gt.y[i];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
+  uint32_t out = offsets[blockIdx.x];
+  __syncthreads();
+  for (int r = 0; r < kWinItems; ++r) {
+    const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
+    float px = 0.f, py = 0.f;
+    const bool keep = i < ps.n && in_window(ps.pos3, vp, win, i, &px, &py);
+    const uint64_t bal = __ballot(keep);
+    const int slot = r & 1;
+    if (lane == 0) wc[slot][wave] = __popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t cw = wc[slot][w];
+      if (w < wave) before += cw;
+      tot += cw;
+    }
+    if (keep) {
+      const uint32_t j = out + before +
+                         __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
+      float pc[3], w;
+      particle_of(ps, vp, tab, gt.n, gt.x0, gt.dx, i, pc, &w);
+      if (pc[2] < 0.0f || pc[2] > vp.depth) w = 0.0f;
+      p4c[j] = make_float4(pc[0], pc[1], 0.0f, w);
+      hc[j] = ps.hsml[i];
+      // cell key of (x', y', 0): grid3d_get_cell_indices, particle_grid.cl:45-53
+      const float c3[3] = {pc[0], pc[1], 0.0f};
+      int32_t idx[3];
+      bool ok = true;
+      for (int a = 0; a < 3; ++a) {
+        const float d = c3[a] - g.minc[a];
+        idx[a] = floor_to_int(d / g.cs[a]);
+        ok = ok && idx[a] >= 0 && idx[a] < g.nc[a];
+      }
+      uint32_t key = 0;
+      if (ok)
+        key = uint32_t(idx[0]) + uint32_t(idx[1]) * uint32_t(g.nc[0]) +
+              uint32_t(idx[2]) * uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
+      else
+        atomicAdd(err, 1u);
+      keys[j] = key;
+      vals[j] = j;
+    }
+    out += tot;
+  }
+}
+
 }  // namespace
+
+uint32_t window_tiles(uint64_t n) { return static_cast<uint32_t>((n + kWinTile - 1) / kWinTile); }
+
+void launch_window_count(const particle_soa& ps, const view_proj& vp, const window_f& win,
+                         uint32_t* counts, uint32_t* total, hipStream_t s) {
+  const uint32_t nb = window_tiles(ps.n);
+  if (!nb) return;
+  hipLaunchKernelGGL(k_window_count, dim3(nb), dim3(kBlock), 0, s, ps.pos3, ps.n, vp, win, counts,
+                     total);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_window_scatter(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
+                           const window_f& win, const uint32_t* offsets, const grid_params& g,
+                           float4* p4c, float* hc, uint32_t* keys, uint32_t* vals, uint32_t* err,
+                           hipStream_t s) {
+  if (vp.quantity == 1 && (gt.n < 1 || gt.n > kMaxGaunt || !gt.y))
+    fail(ILC_EINVAL, "X-ray view without a Gaunt table (1..%d entries)", kMaxGaunt);
+  const uint32_t nb = window_tiles(ps.n);
+  if (!nb) return;
+  hipLaunchKernelGGL(k_window_scatter, dim3(nb), dim3(kBlock), 0, s, ps, vp, gt, win, offsets, g,
+                     p4c, hc, keys, vals, err);
+  ILC_HIP(hipGetLastError());
+}
 
 int view_bbox_parts(uint64_t n) {
   return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(1024, (n + kBlock - 1) / kBlock)));

@@ -397,6 +397,54 @@ ilc_status ilc_vector_scale(float* x_dev, uint64_t n, float alpha, void* stream)
   });
 }
 
+ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t n,
+                           const float* const* columns_dev, int32_t ncolumns,
+                           float* particles4_dev, float* const* columns_out_dev,
+                           uint64_t* index_out_dev, uint64_t* n_out, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(vol && n_out, "null argument");
+    ILC_REQUIRE(ncolumns >= 0 && ncolumns <= kMaxCrawlColumns, "ncolumns %d not in [0, %d]",
+                ncolumns, kMaxCrawlColumns);
+    ILC_REQUIRE(n == 0 || (xyz_dev && particles4_dev), "null argument");
+    ILC_REQUIRE(ncolumns == 0 || (columns_dev && columns_out_dev), "null column list");
+    crawl_columns cols{};
+    cols.n = ncolumns;
+    for (int k = 0; k < ncolumns; ++k) {
+      ILC_REQUIRE(columns_dev[k] && columns_out_dev[k], "null column %d", k);
+      cols.in[k] = columns_dev[k];
+      cols.out[k] = columns_out_dev[k];
+    }
+    double vmin[3], vmax[3], gmin[3], gmax[3];
+    for (int j = 0; j < 3; ++j) {
+      vmin[j] = vol->center[j] - 0.5 * vol->extent[j];
+      vmax[j] = vol->center[j] + 0.5 * vol->extent[j];
+      gmin[j] = vmin[j] - vol->border;
+      gmax[j] = vmax[j] + vol->border;
+    }
+    *n_out = 0;
+    if (!n) return;
+    hipStream_t s = as_stream(stream);
+    const uint64_t nt = crawl_tiles(n);
+    const uint64_t words = 2 * (nt + 1) + scan_temp_words(nt + 1);
+    uint32_t* scratch = nullptr;
+    ILC_HIP(hipMalloc(&scratch, sizeof(uint32_t) * words));
+    struct freer {
+      uint32_t* p;
+      ~freer() { (void)hipFree(p); }
+    } fr{scratch};
+    uint32_t* counts = scratch;
+    uint32_t* offsets = scratch + nt + 1;
+    launch_fill_u32(counts + nt, 0u, 1, s);
+    launch_crawl(vol->periodic_wraparound, gmin, gmax, vmin, vmax, xyz_dev, n, cols, counts,
+                 offsets, scratch + 2 * (nt + 1), reinterpret_cast<float4*>(particles4_dev),
+                 index_out_dev, s);
+    uint32_t m = 0;
+    ILC_HIP(hipMemcpyAsync(&m, offsets + nt, sizeof m, hipMemcpyDeviceToHost, s));
+    ILC_HIP(hipStreamSynchronize(s));
+    *n_out = m;
+  });
+}
+
 ilc_status ilc_quantity_scaled(const float* in_dev, double scale, uint64_t n, float* w_dev,
                                void* stream) {
   return guarded([&] {

@@ -389,6 +389,7 @@ void launch_bbox_final(const float* partial, int nparts, const uint32_t* bad, bb
 
 void launch_keys(const float4* p4, uint32_t n, const grid_params& g, uint32_t* keys,
                  uint32_t* vals, uint32_t* err, hipStream_t s) {
+  if (!n) return;
   hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, n, g, keys,
                      vals, err);
   ILC_HIP(hipGetLastError());

@@ -66,6 +66,15 @@ grid_params plan_grid(const bbox_result& bb, uint64_t n, uint64_t target, int mo
 void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                         uint64_t n, const grid_params& g, int32_t kz0, float max_h, int kernel,
                         uint32_t* err, hipStream_t s) {
+  gs.key.ensure(n);
+  gs.perm.ensure(n);
+  launch_keys(p4, uint32_t(n), g, gs.key.p, gs.perm.p, err, s);
+  enqueue_grid_from_keys(gs, sc, p4, h, n, g, kz0, max_h, kernel, s);
+}
+
+void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
+                            uint64_t n, const grid_params& g, int32_t kz0, float max_h,
+                            int kernel, hipStream_t s) {
   const uint64_t ncells = g.ncells();
   gs.ready = false;
   gs.key.ensure(n);
@@ -76,7 +85,6 @@ void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, cons
   gs.hs.ensure(n);
   sc.counts.ensure(ncells + 1);
   sc.temp.ensure(std::max(radix_temp_words(n), scan_temp_words(ncells + 1)));
-  launch_keys(p4, uint32_t(n), g, gs.key.p, gs.perm.p, err, s);
   launch_radix_sort(gs.key.p, gs.perm.p, uint32_t(n), bits_for(ncells - 1), sc.temp.p, s);
   launch_fill_u32(sc.counts.p, 0u, ncells + 1, s);
   launch_fill_u32(gs.maxh_bits.p, 0u, ncells, s);

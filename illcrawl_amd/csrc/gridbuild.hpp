@@ -43,6 +43,12 @@ void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, cons
                         uint64_t n, const grid_params& g, int32_t kz0, float max_h, int kernel,
                         uint32_t* err, hipStream_t s);
 
+// The same from keys already in gs.key (cell keys) and gs.perm (indices into
+// p4 / h), n of them, in the order the within-cell order must follow.
+void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
+                            uint64_t n, const grid_params& g, int32_t kz0, float max_h,
+                            int kernel, hipStream_t s);
+
 float kernel_norm(int kernel);
 
 }  // namespace ilc

@@ -54,6 +54,31 @@ struct gaunt_table {
   int n;
   float x0, dx;
 };
+// Crawler block body (periodic correction + cutout + compaction).
+constexpr int kMaxCrawlColumns = 8;
+struct crawl_columns {
+  const float* in[kMaxCrawlColumns];
+  float* out[kMaxCrawlColumns];
+  int n;
+};
+uint64_t crawl_tiles(uint64_t n);
+// counts/offsets: crawl_tiles(n) + 1 words each; scan_temp: scan_temp_words(tiles + 1)
+void launch_crawl(const double period[3], const double gmin[3], const double gmax[3],
+                  const double vmin[3], const double vmax[3], const float* xyz, uint64_t n,
+                  const crawl_columns& cols, uint32_t* counts, uint32_t* offsets,
+                  uint32_t* scan_temp, float4* out4, uint64_t* index_out, hipStream_t s);
+
+// Window of camera-frame (x', y') around a view's screen (batched grids).
+struct window_f {
+  float x0, x1, y0, y1;
+};
+uint32_t window_tiles(uint64_t n);
+void launch_window_count(const particle_soa& ps, const view_proj& vp, const window_f& win,
+                         uint32_t* counts, uint32_t* total, hipStream_t s);
+void launch_window_scatter(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
+                           const window_f& win, const uint32_t* offsets, const grid_params& g,
+                           float4* p4c, float* hc, uint32_t* keys, uint32_t* vals, uint32_t* err,
+                           hipStream_t s);
 int view_bbox_parts(uint64_t n);
 void launch_view_bbox(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
                       float* partial, uint32_t* bad, bbox_result* out, hipStream_t s);

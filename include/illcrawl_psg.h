@@ -231,6 +231,33 @@ ilc_status ilc_project_evaluation_points(const ilc_camera* cam, const float* in4
  * x[i] *= alpha in FP32, in place. */
 ilc_status ilc_vector_scale(float* x_dev, uint64_t n, float alpha, void* stream);
 
+/* ------------------------------------------------------- crawler block body */
+
+/* volume_cutout (reconstructing_data_crawler.hpp:49-76): an axis-aligned
+ * render volume, the period of the simulation box, and the crawler's
+ * additional border region (reconstructing_data_crawler.cpp:188-193). */
+typedef struct {
+  double center[3];
+  double extent[3];
+  double periodic_wraparound[3];
+  double border;
+} ilc_volume;
+
+/* The per-row body of reconstructing_data_crawler::run
+ * (reconstructing_data_crawler.cpp:180-213): row i's float coordinates
+ * xyz_dev[3i..3i+2] are moved by one period towards the volume grown by the
+ * border (coordinate_system.hpp:24-47, in double), and the row is kept if the
+ * volume contains the result (reconstructing_data_crawler.cpp:59-71).  Kept
+ * rows are written in input order: particles4_dev gets float4 (x, y, z, 0),
+ * each columns_out_dev[k] gets columns_dev[k][i] (ncolumns <= 8: the
+ * backend's additional datasets and the quantity inputs), index_out_dev (may
+ * be NULL) the input row.  *n_out = rows kept.  Output buffers hold n
+ * entries.  Synchronises `stream`. */
+ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t n,
+                           const float* const* columns_dev, int32_t ncolumns,
+                           float* particles4_dev, float* const* columns_out_dev,
+                           uint64_t* index_out_dev, uint64_t* n_out, void* stream);
+
 /* ------------------------------------------------- quantity transformation */
 
 /* quantity_transformation with unprocessed_quantity / constant_quantity
@@ -301,6 +328,12 @@ ilc_status ilc_batch_render(ilc_batch* b, int32_t flags, void* stream);
 ilc_status ilc_batch_image(const ilc_batch* b, int32_t view, float* out_dev, void* stream);
 /* Its FP64 accumulator (unscaled). */
 ilc_status ilc_batch_image_f64(const ilc_batch* b, int32_t view, double* out_dev, void* stream);
+/* Grid of a view after a render (the particle_grid getters,
+ * particle_grid.cpp:121-134, :342-346).  num_particles counts the particles
+ * the view's grid holds: a view whose screen lies well inside the particle
+ * bounding box keeps only the particles of the cells its points can reach
+ * (the gather never visits the others, so its image is unchanged). */
+ilc_status ilc_batch_view_info(const ilc_batch* b, int32_t view, ilc_grid_info* out);
 /* Exact particle-kernel evaluation count of each view (after a render). */
 ilc_status ilc_batch_count_evals(ilc_batch* b, uint64_t* per_view_host, void* stream);
 /* Device time (ms, HIP events on the render stream) of the gather launch(es)

@@ -50,6 +50,9 @@ PINNING = {
     "orc_temperature": "closed form (gamma-1) u mu 121.1475502",
     "orc_xray_bolometric": "closed form on table-node-free inputs",
     "orc_backward_exact": "FP64 central differences of orc_gather_exact in w, h, x, y",
+    "orc_crawl_block": "hand-built rows: periodic images moved in by one period (each "
+                       "axis, both signs), border tolerance, inclusive volume faces, "
+                       "input order kept; numpy brute force on random rows",
 }
 
 
@@ -129,6 +132,10 @@ def _declare(L):
     L.orc_backward_exact.argtypes = [_f32p, _f32p, _u64, _f32p, _u64, _f64p, ctypes.c_int, _f64p,
                                      _f64p, _f64p, _f64p]
     L.orc_backward_exact.restype = None
+    L.orc_crawl_block.argtypes = [ctypes.c_double * 3, ctypes.c_double * 3, ctypes.c_double * 3,
+                                  ctypes.c_double, _f32p, _u64, ctypes.c_void_p, ctypes.c_int32,
+                                  _f32p, ctypes.c_void_p, _u64p]
+    L.orc_crawl_block.restype = ctypes.c_uint64
     L.orc_num_threads.argtypes = []
     L.orc_num_threads.restype = ctypes.c_int
     L.orc_set_num_threads.argtypes = [ctypes.c_int]
@@ -310,6 +317,23 @@ def backward_exact(p4, h, pts4, G, kind=K_QUARTIC_REF):
     out = [np.zeros(n, np.float64) for _ in range(4)]
     lib().orc_backward_exact(p4, h, n, pts4, pts4.shape[0], G, int(kind), *out)
     return tuple(out)
+
+
+def crawl_block(center, extent, period, border, xyz, cols=()):
+    """Kept rows of one crawler block: (particles4, [columns], input indices)."""
+    xyz = _c(xyz, np.float32).reshape(-1, 3)
+    n = xyz.shape[0]
+    cols = [_c(c, np.float32).reshape(-1) for c in cols]
+    out4 = np.zeros((n, 4), np.float32)
+    outs = [np.zeros(n, np.float32) for _ in cols]
+    idx = np.zeros(n, np.uint64)
+    d3 = ctypes.c_double * 3
+    cin = (ctypes.c_void_p * max(len(cols), 1))(*[c.ctypes.data for c in cols])
+    cout = (ctypes.c_void_p * max(len(cols), 1))(*[c.ctypes.data for c in outs])
+    m = lib().orc_crawl_block(d3(*center), d3(*extent), d3(*period), ctypes.c_double(border),
+                              xyz, n, ctypes.cast(cin, ctypes.c_void_p), len(cols), out4,
+                              ctypes.cast(cout, ctypes.c_void_p), idx)
+    return out4[:m], [o[:m] for o in outs], idx[:m]
 
 
 def num_threads():

@@ -763,3 +763,50 @@ void orc_backward_exact(const float* p4, const float* h, uint64_t n, const float
     gy[j] = w * sy;
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* Block body of reconstructing_data_crawler::run                      */
+/* (reconstructing_data_crawler.cpp:180-213): each row's coordinates,  */
+/* read as float (:183-185), are moved by one period towards the       */
+/* render volume grown by `border` (coordinate_system.hpp:24-47), and   */
+/* kept if the volume contains them (reconstructing_data_crawler.cpp:  */
+/* 59-71, tolerance 0); kept rows go out in input order as float4      */
+/* (x, y, z, 0) (to_device_vector4) with their extra columns.           */
+/* Returns the number kept.                                            */
+/* ------------------------------------------------------------------ */
+uint64_t orc_crawl_block(const double center[3], const double extent[3],
+                         const double period[3], double border, const float* xyz,
+                         uint64_t n, const float* const* cols, int32_t ncols, float* out4,
+                         float* const* cols_out, uint64_t* index_out)
+{
+  double bmin[3], bmax[3];
+  for (int j = 0; j < 3; ++j) {
+    bmin[j] = center[j] - 0.5 * extent[j];
+    bmax[j] = center[j] + 0.5 * extent[j];
+  }
+  uint64_t m = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    double c[3];
+    for (int j = 0; j < 3; ++j) c[j] = (double)xyz[3 * i + j];
+    for (int j = 0; j < 3; ++j) {
+      const double gmin = bmin[j] - border, gmax = bmax[j] + border;
+      const double lo = c[j] - period[j], hi = c[j] + period[j];
+      if (gmin <= lo && lo <= gmax)
+        c[j] = lo;
+      else if (gmin <= hi && hi <= gmax)
+        c[j] = hi;
+    }
+    int inside = 1;
+    for (int j = 0; j < 3; ++j) {
+      if (c[j] < center[j] - 0.5 * extent[j] - 0.0) inside = 0;
+      if (c[j] > center[j] + 0.5 * extent[j] + 0.0) inside = 0;
+    }
+    if (!inside) continue;
+    for (int j = 0; j < 3; ++j) out4[4 * m + j] = (float)c[j];
+    out4[4 * m + 3] = 0.0f;
+    for (int k = 0; k < ncols; ++k) cols_out[k][m] = cols[k][i];
+    if (index_out) index_out[m] = i;
+    ++m;
+  }
+  return m;
+}

@@ -161,3 +161,50 @@ def test_batch_counts_and_autotune(B, scene, orc):
     assert tw * th % 256 == 0 and ms > 0
     r.render()
     assert r.count_evals() == counts
+
+
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+def test_windowed_grids_equal_single_view(B, mode):
+    """Narrow views deep inside the snapshot build their grids from the
+    particles around the screen only; their images must stay bit-identical to
+    the single-view backend's full grid.  A view reaching past the particle
+    bounding box keeps every particle (reference mode: a point outside the grid
+    can alias any cell)."""
+    from illcrawl_amd import synth
+    say(f"case: windowed batch grids, mode={mode}")
+    snap = synth.make_snapshot(1 << 16, 9, 10000.0, n_halos=2, rs_range=(200.0, 400.0),
+                               gas=True)
+    tab = synth.gaunt_table_1d()
+    centre = snap.halos[0][0]
+    views = []
+    for k, (w, nx) in enumerate(((300.0, 64), (600.0, 96), (900.0, 128), (450.0, 80))):
+        c = np.asarray(centre, np.float64) + 150.0 * k
+        v = synth.centred_view(snap.box, look_at=(0.1 * k, -0.05 * k, 1.0), width=w, nx=nx,
+                               roll=0.2 * k, centre=c)
+        v["quantity"] = k % 2
+        v["scale"] = 1.0
+        views.append(v)
+    edge = synth.centred_view(snap.box, width=6000.0, nx=64, centre=(0.0, 0.0, 5000.0))
+    edge["quantity"], edge["scale"] = 0, 1.0
+    views.append(edge)
+    r = make_batch(B, snap, views, mode, tab)
+    r.render()
+    n_windowed = 0
+    for k, v in enumerate(views):
+        ref, _ = single_view(B, snap, v, mode, tab)
+        got = r.image_f64(k).cpu().numpy()
+        info = r.view_info(k)
+        say(f"  view {k}: {info['num_particles']} of {snap.n} particles, "
+            f"max {float(ref.max()):.3e}")
+        assert np.array_equal(got, ref), f"view {k}"
+        assert ref.max() > 0
+        n_windowed += info["num_particles"] < snap.n
+    assert n_windowed >= 3
+    if mode == "reference":
+        assert r.view_info(len(views) - 1)["num_particles"] == snap.n
+    # graph replays give the same images
+    r.render(graph=True)
+    r.render(graph=True)
+    for k, v in enumerate(views):
+        ref, _ = single_view(B, snap, v, mode, tab)
+        assert np.array_equal(r.image_f64(k).cpu().numpy(), ref), f"graph view {k}"

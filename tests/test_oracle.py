@@ -385,3 +385,54 @@ def test_backward_exact_vs_central_differences(orc, kind):
         hb[j] -= eps
         fd = (L(p, ha) - L(p, hb)) / (float(ha[j]) - float(hb[j]))
         assert gh[j] == pytest.approx(fd, rel=2e-3, abs=1e-6 * abs(gh).max())
+
+
+def test_crawl_block_known_rows(orc):
+    """Periodic images moved in by one period per axis and sign, the border
+    tolerance, inclusive faces, and input order (reconstructing_data_crawler
+    block body)."""
+    c, e, L = (50.0, 50.0, 50.0), (20.0, 20.0, 20.0), (100.0, 100.0, 100.0)
+    rows = np.array([
+        [50, 50, 50],      # inside
+        [150, 50, 50],     # x - L inside
+        [50, -55, 50],     # y + L = 45 inside
+        [50, 50, 141],     # z - L = 41 inside
+        [40, 50, 50],      # inside, on the min face
+        [60, 60, 60],      # inside, on the max faces
+        [39.5, 50, 50],    # just outside
+        [50, 50, 61],      # outside (z)
+        [161, 50, 50],     # x - L = 61: outside
+    ], np.float32)
+    h = np.arange(len(rows), dtype=np.float32) + 1
+    p4, (hk,), idx = orc.crawl_block(c, e, L, 0.0, rows, [h])
+    assert list(idx) == [0, 1, 2, 3, 4, 5]
+    assert np.array_equal(p4[:, :3], np.array([[50, 50, 50], [50, 50, 50], [50, 45, 50],
+                                               [50, 50, 41], [40, 50, 50], [60, 60, 60]],
+                                              np.float32))
+    assert np.all(p4[:, 3] == 0) and np.array_equal(hk, h[:6])
+    # a border lets a row move into the grown box, but the volume test has no border:
+    # 138.5 - 100 = 38.5 is within [40 - 2, 60 + 2] -> moved, then outside the volume
+    p4b, _, idxb = orc.crawl_block(c, e, L, 2.0, np.array([[138.5, 50, 50]], np.float32))
+    assert len(idxb) == 0
+    p4c, _, idxc = orc.crawl_block(c, e, L, 2.0, np.array([[140.0, 50, 50]], np.float32))
+    assert list(idxc) == [0] and p4c[0, 0] == 40.0
+
+
+def test_crawl_block_vs_numpy(orc):
+    rng = np.random.default_rng(3)
+    n = 20000
+    xyz = rng.uniform(-120, 220, (n, 3)).astype(np.float32)
+    col = rng.random(n).astype(np.float32)
+    c, e, L, bd = (37.5, 61.25, 80.0), (30.0, 45.0, 70.0), (100.0, 100.0, 120.0), 3.0
+    p4, (ck,), idx = orc.crawl_block(c, e, L, bd, xyz, [col])
+    x = xyz.astype(np.float64)
+    c, e, L = np.array(c), np.array(e), np.array(L)
+    bmin, bmax = c - 0.5 * e, c + 0.5 * e
+    lo, hi = x - L, x + L
+    in_lo = (bmin - bd <= lo) & (lo <= bmax + bd)
+    in_hi = (bmin - bd <= hi) & (hi <= bmax + bd)
+    xc = np.where(in_lo, lo, np.where(in_hi, hi, x))
+    keep = np.all((xc >= c - 0.5 * e) & (xc <= c + 0.5 * e), axis=1)
+    assert np.array_equal(idx, np.flatnonzero(keep))
+    assert np.array_equal(p4[:, :3], xc[keep].astype(np.float32))
+    assert np.array_equal(ck, col[keep])
