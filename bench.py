@@ -1,6 +1,6 @@
 """Benchmark of the projective SPH smoothing path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1|c4]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1|c4|c5]
 
 Workloads (synthetic, seeded; see illcrawl_amd/synth.py):
   c3 (default)  the batched multi-view workload: the 16M-cell X-ray snapshot
@@ -10,6 +10,8 @@ Workloads (synthetic, seeded; see illcrawl_amd/synth.py):
                 X-ray), one persistent gather launch per batch;
   c2            one 2048^2 X-ray view of the same snapshot (plain path);
   c1            the 2^20-cell column-density view, 512^2 (plain path);
+  c5            32 X-ray slabs (500 kpc/h, centre mask) of a 64M-cell snapshot at
+                4096^2 each, batched;
   c4            forward and adjoint of a 1024^2 column-density image of 4M
                 cells (exact mode, quartic): adjoint cost relative to forward
                 and gradient error against the FP64 oracle on a sample.
@@ -156,6 +158,14 @@ def workload(name):
                     box_kpc_h=snap.box, halos=8, views=64,
                     resolutions={"256^2": 16, "512^2": 16, "1024^2": 16, "2048^2": 16},
                     width_kpc_h=[1000.0, 5000.0], quantity="alternating column density / X-ray")
+    elif name == "c5":
+        snap = synth.cached("c5")
+        views = synth.slab_views(snap.box, n_slabs=32, thickness=500.0, nx=4096)
+        desc = dict(workload="slab tomography: 32 slabs of 500 kpc/h along z through the "
+                             "64M-cell X-ray snapshot, 4096^2 each, one persistent launch",
+                    n_cells=snap.n, box_kpc_h=snap.box, halos=32, views=32,
+                    resolution=[4096, 4096], width_kpc_h=snap.box, slab_kpc_h=500.0,
+                    quantity="X-ray", depth_cut="centre mask")
     else:
         raise SystemExit(f"unknown workload {name}")
     return snap, views, desc

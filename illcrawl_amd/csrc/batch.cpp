@@ -51,8 +51,10 @@ struct view_state {
   int32_t kz0 = 0;
   float max_h = 0.f;
   bool windowed = false;  // grid built from the particles in `win` only
+  bool compact = false;   // particle lists from the depth cut only; max h from the group
+  int group = -1;         // slab group (views with identical projected x', y')
   window_f win{};
-  uint32_t nwin = 0;      // particles in the window (from the plan)
+  uint32_t nwin = 0;      // particles kept by the filter (from the plan)
 };
 
 template <class T>
@@ -94,6 +96,8 @@ struct ilc_batch {
   grid_scratch scratch;
   dbuf<float4> p4tmp;
   dbuf<float> htmp;
+  std::vector<std::unique_ptr<grid_store>> group_grids;  // full grid per slab group
+  std::vector<size_t> group_leader;
   dbuf<uint32_t> wcounts, woffs, wtotal;
   pinned<uint32_t> wtotal_host;
   dbuf<float> bbox_parts;
@@ -176,6 +180,14 @@ view_proj proj_of(const ilc_batch* b, const view_state& vs) {
   return vp;
 }
 
+// A view's grid as the gather sees it: a depth-compacted view takes the
+// per-cell max h of its group's full grid.
+grid_view grid_of(const ilc_batch* b, const view_state& vs) {
+  grid_view g = vs.grid.view();
+  if (vs.compact) g.maxh_bits = b->group_grids[size_t(vs.group)]->maxh_bits.p;
+  return g;
+}
+
 gaunt_table gaunt_of(const ilc_batch* b) {
   return gaunt_table{b->gaunt.p, b->gaunt_n, b->gx0, b->gdx};
 }
@@ -221,6 +233,38 @@ void plan(ilc_batch* b, hipStream_t s) {
     for (uint32_t x : bits) sig.push_back(x);
     sig.push_back(uint64_t(vs.kz0));
   }
+  // slab groups: views whose projection rows for x' and y' and centre x, y are
+  // bitwise equal, with no z term in those rows and no x, y term in the depth
+  // row, project every particle to the same (x', y') and differ only in the
+  // depth cut; they share one full grid (same bounding box, cells and per-cell
+  // max h) and keep only their depth-cut particles in their lists.
+  b->group_leader.clear();
+  for (size_t v = 0; v < nv; ++v) {
+    b->views[v]->group = -1;
+    b->views[v]->compact = false;
+  }
+  for (size_t v = 0; v < nv; ++v) {
+    view_state& vs = *b->views[v];
+    const float* m = vs.f.minv;
+    if (vs.group >= 0 || m[2] != 0.0f || m[5] != 0.0f || m[6] != 0.0f || m[7] != 0.0f) continue;
+    for (size_t u = v + 1; u < nv; ++u) {
+      view_state& us = *b->views[u];
+      if (us.group >= 0 || us.v.quantity != vs.v.quantity) continue;
+      if (std::memcmp(us.f.minv, m, 6 * sizeof(float)) != 0 ||
+          std::memcmp(us.f.center, vs.f.center, 2 * sizeof(float)) != 0 ||
+          us.f.minv[6] != 0.0f || us.f.minv[7] != 0.0f)
+        continue;
+      if (vs.group < 0) {
+        vs.group = static_cast<int>(b->group_leader.size());
+        b->group_leader.push_back(v);
+      }
+      us.group = vs.group;
+    }
+  }
+  for (size_t v = 0; v < nv; ++v) b->views[v]->compact = b->views[v]->group >= 0;
+  while (b->group_grids.size() < b->group_leader.size())
+    b->group_grids.push_back(std::make_unique<grid_store>());
+
   // windows: the screen grown by the largest reach of any point (F3 radius
   // <= max h; support up to 2 h) plus whole cells.  Reference mode needs every
   // point's own cell inside the grid (an out-of-grid point's unclamped key
@@ -246,12 +290,15 @@ void plan(ilc_batch* b, hipStream_t s) {
     if (ok && -hx - reach <= bb.mn[0] && hx + reach >= bb.mx[0] && -hy - reach <= bb.mn[1] &&
         hy + reach >= bb.mx[1])
       ok = false;
-    vs.windowed = ok;
-    if (ok) {
+    vs.windowed = ok && !vs.compact;
+    if (vs.windowed) {
       vs.win = window_f{std::nextafter(float(-hx - reach), -INFINITY),
                         std::nextafter(float(hx + reach), INFINITY),
                         std::nextafter(float(-hy - reach), -INFINITY),
-                        std::nextafter(float(hy + reach), INFINITY)};
+                        std::nextafter(float(hy + reach), INFINITY), 1, 0};
+      any_win = true;
+    } else if (vs.compact) {
+      vs.win = window_f{0.f, 0.f, 0.f, 0.f, 0, 1};
       any_win = true;
     }
   }
@@ -262,7 +309,7 @@ void plan(ilc_batch* b, hipStream_t s) {
     b->wtotal_host.ensure(nv);
     launch_fill_u32(b->wtotal.p, 0u, nv, s);
     for (size_t v = 0; v < nv; ++v)
-      if (b->views[v]->windowed)
+      if (b->views[v]->windowed || b->views[v]->compact)
         launch_window_count(b->ps, proj_of(b, *b->views[v]), b->views[v]->win, b->wcounts.p,
                             b->wtotal.p + v, s);
     ILC_HIP(hipMemcpyAsync(b->wtotal_host.p, b->wtotal.p, sizeof(uint32_t) * nv,
@@ -271,8 +318,10 @@ void plan(ilc_batch* b, hipStream_t s) {
   }
   for (size_t v = 0; v < nv; ++v) {
     view_state& vs = *b->views[v];
-    vs.nwin = vs.windowed ? b->wtotal_host.p[v] : 0u;
-    sig.push_back(vs.windowed ? uint64_t(vs.nwin) : ~uint64_t(0));
+    const bool filtered = vs.windowed || vs.compact;
+    vs.nwin = filtered ? b->wtotal_host.p[v] : 0u;
+    sig.push_back(filtered ? uint64_t(vs.nwin) : ~uint64_t(0));
+    sig.push_back(uint64_t(int64_t(vs.group)));
   }
   sig.push_back(uint64_t(b->ts.tw) << 32 | uint64_t(b->ts.ppt));
   // work items: largest views first, tiles in image order
@@ -325,9 +374,17 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
   const int op = b->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT;
   b->p4tmp.ensure(b->ps.n);
   launch_fill_u32(b->counter.p, 0u, 2, s);
+  // full grid of each slab group (bounding box, cells and per-cell max h of
+  // every particle, as each member's reference grid has them)
+  for (size_t gi = 0; gi < b->group_leader.size(); ++gi) {
+    view_state& ls = *b->views[b->group_leader[gi]];
+    launch_view_project(b->ps, proj_of(b, ls), gaunt_of(b), b->p4tmp.p, s);
+    enqueue_grid_build(*b->group_grids[gi], b->scratch, b->p4tmp.p, b->ps.hsml, b->ps.n, ls.gp,
+                       ls.kz0, ls.max_h, b->kernel, b->counter.p + 1, s);
+  }
   for (size_t v = 0; v < nv; ++v) {
     view_state& vs = *b->views[v];
-    if (vs.windowed) {
+    if (vs.windowed || vs.compact) {
       // kept particles in input order, then the grid from their keys
       const uint32_t nt = window_tiles(b->ps.n);
       const uint64_t m = vs.nwin;
@@ -352,7 +409,7 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
     const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
     launch_fill_f64(vs.acc.p, 0.0, P, s);
     view_desc& d = b->desc_host.p[v];
-    d.g = vs.grid.view();
+    d.g = grid_of(b, vs);
     d.p = point_view{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
     d.acc = vs.acc.p;
     d.tiles_x = vs.tiles_x;
@@ -606,7 +663,7 @@ ilc_status ilc_batch_count_evals(ilc_batch* b, uint64_t* per_view_host, void* st
       cnt.ensure(P);
       launch_fill_u32(reinterpret_cast<uint32_t*>(cnt.p), 0u, 2 * P, s);
       point_view p{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
-      launch_gather(vs.grid.view(), p, OP_COUNT, b->kernel, b->ts, nullptr, cnt.p, nullptr, s);
+      launch_gather(grid_of(b, vs), p, OP_COUNT, b->kernel, b->ts, nullptr, cnt.p, nullptr, s);
       per_view_host[v] = launch_sum_u64(cnt.p, P, tmp.p, s);
     }
   });

@@ -153,16 +153,21 @@ __global__ __launch_bounds__(kBlock) void k_view_pass(particle_soa ps, view_proj
   }
 }
 
-// ---- windowed grid input
+// ---- filtered grid input
 //
-// Only the cells that some point of the view can reach matter to its image
-// (the gather never visits the others), so the batch builds each view's grid
-// from the particles whose projection falls in a window around the screen:
-// the screen grown by more than the largest reach plus whole cells, so that
-// every cell a point can reach keeps all of its particles.  Kept particles
-// stay in input order (order-preserving compaction), so the within-cell order
-// of the reachable cells, their counts and max h, and hence the images, are
-// the ones of the full grid.
+// Window: only the cells that some point of the view can reach matter to its
+// image (the gather never visits the others), so the batch can build a view's
+// grid from the particles whose projection falls in a window around the
+// screen: the screen grown by more than the largest reach plus whole cells,
+// so that every cell a point can reach keeps all of its particles.  Kept
+// particles stay in input order (order-preserving compaction), so the
+// within-cell order of the reachable cells, their counts and max h, and hence
+// the images, are the ones of the full grid.
+//
+// Depth: views that project every particle to the same (x', y') (slabs of one
+// stack) share one full grid for the per-cell max h and the bounding box, and
+// each keeps for its particle lists only the particles inside its depth cut
+// (the others have w = 0 and add nothing).
 
 // as grid.hip's (grid3d_get_cell_indices)
 __device__ inline int32_t floor_to_int(float f) {
@@ -183,7 +188,14 @@ __device__ inline bool in_window(const float* pos3, const view_proj& vp, const w
   const float vx = x - vp.c[0], vy = y - vp.c[1], vz = z - vp.c[2];
   *px = dot3(vp.m + 0, vx, vy, vz);
   *py = dot3(vp.m + 3, vx, vy, vz);
-  return *px >= win.x0 && *px <= win.x1 && *py >= win.y0 && *py <= win.y1;
+  bool keep = true;
+  if (win.use_window)
+    keep = *px >= win.x0 && *px <= win.x1 && *py >= win.y0 && *py <= win.y1;
+  if (win.use_depth) {
+    const float pz = dot3(vp.m + 6, vx, vy, vz);
+    keep = keep && !(pz < 0.0f || pz > vp.depth);
+  }
+  return keep;
 }
 
 // counts[b] = kept particles of workgroup b's tile; *total += the same

@@ -68,9 +68,11 @@ void launch_crawl(const double period[3], const double gmin[3], const double gma
                   const crawl_columns& cols, uint32_t* counts, uint32_t* offsets,
                   uint32_t* scan_temp, float4* out4, uint64_t* index_out, hipStream_t s);
 
-// Window of camera-frame (x', y') around a view's screen (batched grids).
+// Row filter of the batched grids: camera-frame (x', y') inside a window
+// around the view's screen, and/or the view's depth cut (z' in [0, depth]).
 struct window_f {
   float x0, x1, y0, y1;
+  int use_window, use_depth;
 };
 uint32_t window_tiles(uint64_t n);
 void launch_window_count(const particle_soa& ps, const view_proj& vp, const window_f& win,

@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import sys
 
 import numpy as np
 
@@ -67,10 +68,17 @@ def knn_density(pos, mass, box, k=32, workers=-1):
     from scipy.spatial import cKDTree
     p = np.asarray(pos, np.float64) % box
     tree = cKDTree(p, boxsize=box, leafsize=32, balanced_tree=False, compact_nodes=False)
-    d, idx = tree.query(p, k=k + 1, workers=workers)
-    rk = d[:, k]
-    msum = np.asarray(mass, np.float64)[idx[:, 1:]].sum(axis=1)
-    return (msum / (4.0 / 3.0 * math.pi * rk ** 3)).astype(np.float32)
+    m64 = np.asarray(mass, np.float64)
+    rho = np.empty(p.shape[0], np.float32)
+    chunk = 1 << 21   # bounded memory for large snapshots; queries are independent
+    for a in range(0, p.shape[0], chunk):
+        if p.shape[0] > (1 << 23):
+            print(f"[synth] density {a}/{p.shape[0]}", file=sys.stderr, flush=True)
+        d, idx = tree.query(p[a:a + chunk], k=k + 1, workers=workers)
+        rk = d[:, k]
+        msum = m64[idx[:, 1:]].sum(axis=1)
+        rho[a:a + chunk] = (msum / (4.0 / 3.0 * math.pi * rk ** 3)).astype(np.float32)
+    return rho
 
 
 def make_snapshot(n, seed, box, halos=None, n_halos=1, rs_range=(300.0, 300.0),
@@ -147,6 +155,27 @@ def config4(n=1 << 22, seed=3):
     return make_snapshot(n, seed, 10000.0, n_halos=1, rs_range=(300.0, 300.0))
 
 
+def config5(n=1 << 26, seed=4, n_halos=32):
+    """Slab tomography workload: 64M cells of the X-ray generator at seed 4 with
+    32 halos in the 75000 kpc/h box."""
+    return make_snapshot(n, seed, 75000.0, n_halos=n_halos, rs_range=(100.0, 500.0), gas=True)
+
+
+def slab_views(box, n_slabs=32, thickness=500.0, nx=4096, width=None, quantity=1):
+    """Tomography slabs along +z through the box centre: slab k is a view of
+    depth ``thickness`` whose screen sits at z0 + k * thickness (z0 = centre -
+    n_slabs * thickness / 2); the depth cut is the reference's centre mask."""
+    c = 0.5 * box
+    z0 = c - 0.5 * n_slabs * thickness
+    width = box if width is None else width
+    views = []
+    for k in range(n_slabs):
+        views.append(dict(position=np.array([c, c, z0 + k * thickness]),
+                          look_at=np.array([0.0, 0.0, 1.0]), roll=0.0, width=float(width),
+                          nx=int(nx), ny=int(nx), depth=float(thickness), quantity=quantity))
+    return views
+
+
 def centred_view(box, look_at=(0.0, 0.0, 1.0), width=2000.0, nx=512, ny=None, roll=0.0,
                  centre=None):
     """Camera dict for a view through the box centre: position = centre -
@@ -193,13 +222,13 @@ def gaunt_table_1d(n=512, t_min=1e4, t_max=1e9):
 
 
 def cached(name, cache_dir=None):
-    """Workload snapshot by name (``c1``, ``c2``, ``c4``), cached as an .npz
+    """Workload snapshot by name (``c1``, ``c2``, ``c4``, ``c5``), cached as an .npz
     under ``cache_dir`` (default $ILC_CACHE or /tmp/ilc_cache) so that one
     session generates it once.  Loaded with numpy's pickle-free loader."""
     import os
     cache_dir = cache_dir or os.environ.get("ILC_CACHE", "/tmp/ilc_cache")
     path = os.path.join(cache_dir, f"{name}.npz")
-    makers = {"c1": config1, "c2": config2, "c4": config4}
+    makers = {"c1": config1, "c2": config2, "c4": config4, "c5": config5}
     if os.path.exists(path):
         z = np.load(path)
         halos = [(z["halo_c"][i], float(z["halo_rs"][i])) for i in range(z["halo_c"].shape[0])]
