@@ -263,6 +263,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one device copy of MIB MiB (a known-traffic kernel for "
+                         "scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -371,6 +374,14 @@ def main():
                      "kernel": "k_gather_batch (persistent gather, all views)",
                      "flop_per_eval": FLOP_PER_EVAL, "evals_per_launch": total_evals},
     }
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        line["copy_calibration_bytes"] = nb
+        del src, dst
     big = max(range(len(views)), key=lambda k: views[k]["nx"] * views[k]["ny"])
     if not args.no_parity:
         log("[bench] sampled parity vs oracle ...")

@@ -29,6 +29,7 @@ class ProjectedImage(torch.autograd.Function):
         kept = project_particles(camera, ones, dev.index)[:, 3] != 0
         psg = ProjectiveSmoothingGrid(dev.index, target_particles_per_cell, mode="exact",
                                       kernel=kernel)
+        psg.init_backend(proj.shape[0])
         psg.setup_projected_particles(proj, h.detach().float())
         pts = camera_pixel_points(camera, dev.index)
         psg.setup_evaluation_points(pts, layout_nx=int(camera["nx"]))

@@ -401,10 +401,12 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
                             b->counter.p + 1, s);
       enqueue_grid_from_keys(vs.grid, b->scratch, b->p4tmp.p, b->htmp.p, m, vs.gp, vs.kz0,
                              vs.max_h, b->kernel, s);
+      vs.grid.density_n = vs.windowed ? b->ps.n : 0;
     } else {
       launch_view_project(b->ps, proj_of(b, vs), gaunt_of(b), b->p4tmp.p, s);
       enqueue_grid_build(vs.grid, b->scratch, b->p4tmp.p, b->ps.hsml, b->ps.n, vs.gp, vs.kz0,
                          vs.max_h, b->kernel, b->counter.p + 1, s);
+      vs.grid.density_n = 0;
     }
     const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
     launch_fill_f64(vs.acc.p, 0.0, P, s);

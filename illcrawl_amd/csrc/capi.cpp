@@ -407,6 +407,8 @@ ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t
                 ncolumns, kMaxCrawlColumns);
     ILC_REQUIRE(n == 0 || (xyz_dev && particles4_dev), "null argument");
     ILC_REQUIRE(ncolumns == 0 || (columns_dev && columns_out_dev), "null column list");
+    *n_out = 0;
+    if (!n) return;
     crawl_columns cols{};
     cols.n = ncolumns;
     for (int k = 0; k < ncolumns; ++k) {
@@ -421,8 +423,6 @@ ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t
       gmin[j] = vmin[j] - vol->border;
       gmax[j] = vmax[j] + vol->border;
     }
-    *n_out = 0;
-    if (!n) return;
     hipStream_t s = as_stream(stream);
     const uint64_t nt = crawl_tiles(n);
     const uint64_t words = 2 * (nt + 1) + scan_temp_words(nt + 1);

@@ -154,7 +154,8 @@ struct gather_smem {
   float tab[KERNEL == 2 ? kCubicTable : 1];
   uint32_t cell_b[kBlock];
   uint32_t cell_off[kBlock];
-  int cell_info[kBlock];  // packed cell (cx | cy << 16) << 1 | full
+  int cell_info[kBlock];  // cells: packed cell (cx | cy << 16) << 1 | full; rows: base key
+  int cell_y[kBlock];     // rows: the row's cy
   uint32_t scan_ws[4];
   uint32_t Lidx[ADJ ? kCap : 1];    // adjoint: sorted particle index
   float4 Pt[ADJ ? kBlock * 4 : 1];  // adjoint: tile points (x, y, dL/dvalue, -), row-major
@@ -549,18 +550,36 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
     }
   };
 
-  // ---- candidate cells: 256 at a time (x fastest), reach cull, streaming
+  // ---- candidates.  The particles of the cells cx0..cx1 of one row (y, z)
+  // are contiguous in the sorted order, so the box's particles form nrows *
+  // nlays ranges.  Dense grids visit the box's cells 256 at a time (x
+  // fastest) and skip the cells whose max h cannot reach the tile; sparse
+  // grids (well under one particle per cell, e.g. the depth-cut lists of a
+  // slab) stream the row ranges and cull per particle.  Both give the same
+  // candidates in the same order (rows, cells by x, particles in sorted
+  // order).
   const int ncols = cx1 >= cx0 ? cx1 - cx0 + 1 : 0;
   const int nrows = cy1 >= cy0 ? cy1 - cy0 + 1 : 0;
   const int nlays = cz1 >= cz0 ? cz1 - cz0 + 1 : 0;
   const int64_t ncand = int64_t(ncols) * nrows * nlays;
+  const int64_t nrowz = ncols > 0 ? int64_t(nrows) * nlays : 0;
   const uint32_t nxny = uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
   const float sup_h2 = support * support * 1.00002f;
-  for (int64_t c0 = 0; c0 < ncand; c0 += kBlock) {
+  const bool rows_mode = g.sparse != 0 && ncand > int64_t(kBlock);
+  const int64_t nitems = rows_mode ? nrowz : ncand;
+  for (int64_t c0 = 0; c0 < nitems; c0 += kBlock) {
     const int64_t ci = c0 + t;
     uint32_t cb = 0, cn = 0;
     int ccx = 0, ccy = 0, cfull = 0;
-    if (ci < ncand) {
+    if (rows_mode) {
+      if (ci < nrowz) {
+        ccy = cy0 + int(ci % nrows);
+        const uint32_t kb = uint32_t(ccy) * uint32_t(g.nc[0]) + uint32_t(cz0 + int(ci / nrows)) * nxny;
+        cb = g.start[kb + uint32_t(cx0)];
+        cn = g.start[kb + uint32_t(cx1) + 1] - cb;
+        ccx = int(kb);  // stored as the row's base key
+      }
+    } else if (ci < ncand) {
       const int col_c = int(ci % ncols);
       const int64_t rest = ci / ncols;
       ccx = cx0 + col_c;
@@ -588,7 +607,12 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
     const uint32_t coff = block_scan_excl(cn, &tot, sm.scan_ws);
     sm.cell_b[t] = cb;
     sm.cell_off[t] = coff;
-    sm.cell_info[t] = ((ccx | (ccy << 16)) << 1) | cfull;
+    if (rows_mode) {
+      sm.cell_info[t] = ccx;
+      sm.cell_y[t] = ccy;
+    } else {
+      sm.cell_info[t] = ((ccx | (ccy << 16)) << 1) | cfull;
+    }
     __syncthreads();
     tot = uint32_t(uniform(int(tot)));
     for (uint32_t v0 = 0; v0 < tot; v0 += kBlock) {
@@ -605,6 +629,16 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
           if (sm.cell_off[lo + step] <= v) lo += step;
         const uint32_t sidx = sm.cell_b[lo] + (v - sm.cell_off[lo]);
         cinf = sm.cell_info[lo];
+        if constexpr (REF) {
+          if (rows_mode) {
+            // the particle's cell from its key and the row's base key
+            const int px_c = int(g.key[sidx] - uint32_t(cinf));
+            const int py_c = sm.cell_y[lo];
+            const int pf = (allz_u && py_c >= r_ily && py_c <= r_ihy && px_c >= r_ilx &&
+                            px_c <= r_ihx) ? 1 : 0;
+            cinf = ((px_c | (py_c << 16)) << 1) | pf;
+          }
+        }
         const float4 q = g.part[sidx];  // (x, y, coef, 1/h^2)
         if (ADJ || q.z != 0.0f) {
           const float dx = fmaxf(fmaxf(tbx0 - q.x, q.x - tbx1), 0.0f);

@@ -45,6 +45,8 @@ grid_view grid_store::view() const {
   g.cell_radius = cell_radius;
   g.kz0 = kz0;
   g.ncells = static_cast<uint32_t>(gp.ncells());
+  const uint64_t dn = density_n ? density_n : n;
+  g.sparse = dn * 5 < uint64_t(g.ncells) * 2 ? 1 : 0;
   return g;
 }
 

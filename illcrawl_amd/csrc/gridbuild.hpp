@@ -20,6 +20,10 @@ struct grid_store {
   float max_h = 0.f, cell_radius = 0.f;
   int32_t kz0 = 0;
   uint64_t n = 0;
+  // particles per cell that decides how the gather enumerates candidates;
+  // 0 = n (a windowed grid sets the count of the full set it stands for, so
+  // that it is enumerated like the full grid)
+  uint64_t density_n = 0;
   bool ready = false;
   grid_view view() const;
 };

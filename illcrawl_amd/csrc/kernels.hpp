@@ -92,6 +92,7 @@ void launch_view_project(const particle_soa& ps, const view_proj& vp, const gaun
 struct grid_view {
   const float4* part;         // (x, y, coef, 1/h^2) sorted by cell
   const uint32_t* key;        // sorted cell keys
+  int sparse;                 // fewer than 0.4 particles per cell: stream row ranges
   const float* hs;            // sorted h
   const uint32_t* start;      // ncells + 1 offsets
   const uint32_t* maxh_bits;  // per-cell max h (float bits)

@@ -68,6 +68,7 @@ def test_backward_vs_fp64_oracle(B, orc, kernel, kind, seed):
     G = rng.normal(size=nx * ny).astype(np.float32)
     scale = 1.7
     b = B.ProjectiveSmoothingGrid(mode="exact", kernel=kernel, target_particles_per_cell=50)
+    b.init_backend(p4.shape[0])
     b.setup_projected_particles(p4, h)
     b.setup_evaluation_points(pts, layout_nx=nx)
     gw, gh, gx, gy = (t.cpu().numpy() for t in b.backward(torch.from_numpy(G).cuda(), scale))
@@ -90,6 +91,7 @@ def test_backward_repeats_and_sparse_gradient(B, orc):
     G = np.zeros((ny, nx), np.float32)
     G[40:48, 8:16] = 1.0
     b = B.ProjectiveSmoothingGrid(mode="exact", kernel="quartic")
+    b.init_backend(p4.shape[0])
     b.setup_projected_particles(p4, h)
     b.setup_evaluation_points(pts, layout_nx=nx)
     g1 = [t.cpu().numpy() for t in b.backward(torch.from_numpy(G.ravel()).cuda())]
@@ -110,6 +112,7 @@ def test_backward_errors(B):
     G = torch.zeros(32 * 32, device="cuda")
     for mode, kernel in (("reference", "quartic_ref"), ("exact", "cubic")):
         b = B.ProjectiveSmoothingGrid(mode=mode, kernel=kernel)
+        b.init_backend(p4.shape[0])
         b.setup_projected_particles(p4, h)
         b.setup_evaluation_points(pts, layout_nx=32)
         with pytest.raises(IllcrawlError):
@@ -117,6 +120,7 @@ def test_backward_errors(B):
     b = B.ProjectiveSmoothingGrid(mode="exact", kernel="quartic")
     with pytest.raises(IllcrawlError):
         b.backward(torch.zeros(0, device="cuda"))
+    b.init_backend(p4.shape[0])
     b.setup_projected_particles(p4, h)
     b.setup_evaluation_points(pts, layout_nx=32)
     with pytest.raises(ValueError):
