@@ -39,6 +39,10 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-kernel evals/s, SPH projection: batched vs plain on MI355X @1e-5 rel err"
 PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (the spec figure counts packed FP32)
 FLOP_PER_EVAL = 12
+# HBM bytes per k_gather_batch launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs of this workload scaled by a 1 GiB copy (profiles/k_gather_batch_xray_16M_64views.md);
+# keyed by (workload, mode, tile)
+MEASURED_TRAFFIC = {("c3", "reference", (32, 32)): dict(read=387.0e9, write=0.716e9)}
 
 
 def log(msg):
@@ -343,6 +347,7 @@ def main():
         r.render()
 
     value = total_evals / (ms_per_step * 1e-3)
+    traffic = MEASURED_TRAFFIC.get((args.workload, args.mode, tuple(tile)))
     achieved = total_evals * FLOP_PER_EVAL / (gather_ms * 1e-3) / 1e12
     npix = sum(v["nx"] * v["ny"] for v in views)
     line = {
@@ -370,7 +375,9 @@ def main():
                              evals_per_s=total_evals / (graph_ms * 1e-3)),
         "gather_ms": gather_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic["read"] + traffic["write"] if traffic else None,
+                     "traffic_split": traffic,
                      "kernel": "k_gather_batch (persistent gather, all views)",
                      "flop_per_eval": FLOP_PER_EVAL, "evals_per_launch": total_evals},
     }
