@@ -39,6 +39,21 @@ using namespace ilc;
 
 namespace {
 
+// Scratch of one stream building grids.
+struct build_lane {
+  hipStream_t st = nullptr;
+  hipEvent_t done = nullptr;
+  grid_scratch scratch;
+  dbuf<float4> p4tmp;
+  dbuf<float> htmp;
+  dbuf<uint32_t> wcounts, woffs;
+  ~build_lane() {
+    if (done) (void)hipEventDestroy(done);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+constexpr int kLanes = 4;
+
 struct view_state {
   ilc_view v{};
   camera_frame f{};
@@ -96,6 +111,10 @@ struct ilc_batch {
   grid_scratch scratch;
   dbuf<float4> p4tmp;
   dbuf<float> htmp;
+  // extra streams (and their scratch) for the per-view grid builds of a
+  // persistent render; the builds of different views are independent
+  std::vector<std::unique_ptr<build_lane>> lanes;
+  hipEvent_t ev_fork = nullptr;
   std::vector<std::unique_ptr<grid_store>> group_grids;  // full grid per slab group
   std::vector<size_t> group_leader;
   dbuf<uint32_t> wcounts, woffs, wtotal;
@@ -109,7 +128,9 @@ struct ilc_batch {
   dbuf<uint2> items_dev;
   pinned<uint2> items_host;
   uint32_t nitems = 0;
-  dbuf<uint32_t> counter;  // [0] work counter, [1] out-of-grid particles
+  queue_segs segs{};
+  dbuf<uint32_t> qcounters;  // per-queue work counters (kQueueStride apart)
+  dbuf<uint32_t> counter;  // [0] unused, [1] out-of-grid particles
   pinned<uint32_t> flags_host;
 
   bool planned = false;
@@ -132,6 +153,8 @@ struct ilc_batch {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    lanes.clear();
     if (ev_out) (void)hipEventDestroy(ev_out);
     if (own) (void)hipStreamDestroy(own);
   }
@@ -339,10 +362,19 @@ void plan(ilc_batch* b, hipStream_t s) {
     nitems += vs.ntiles;
   }
   ILC_REQUIRE(nitems < (uint64_t(1) << 32), "too many work items");
+  // queue q: the q-th eighth (contiguous tiles, image order) of every view,
+  // largest views first
   uint2* it = b->items_host.ensure(nitems);
   uint64_t k = 0;
-  for (uint32_t v : order)
-    for (uint32_t t = 0; t < b->views[v]->ntiles; ++t) it[k++] = make_uint2(v, t);
+  for (int q = 0; q < kQueues; ++q) {
+    b->segs.seg[q] = static_cast<uint32_t>(k);
+    for (uint32_t v : order) {
+      const uint64_t nt = b->views[v]->ntiles;
+      const uint64_t t0 = nt * uint64_t(q) / kQueues, t1 = nt * uint64_t(q + 1) / kQueues;
+      for (uint64_t t = t0; t < t1; ++t) it[k++] = make_uint2(v, static_cast<uint32_t>(t));
+    }
+  }
+  b->segs.seg[kQueues] = static_cast<uint32_t>(k);
   b->nitems = static_cast<uint32_t>(nitems);
   b->items_dev.ensure(nitems);
   b->desc_dev.ensure(nv);
@@ -357,8 +389,7 @@ void plan(ilc_batch* b, hipStream_t s) {
                                      b->kernel, b->ts);
 }
 
-// 2. device work of a planned render, persistent (one gather launch over
-// all views) or plain (one gather launch per view).
+// An event for timing the gather launch(es) of an eager render.
 hipEvent_t timing_event(ilc_batch* b) {
   if (b->tev_used == int(b->tev.size())) {
     hipEvent_t e;
@@ -368,68 +399,125 @@ hipEvent_t timing_event(ilc_batch* b) {
   return b->tev[b->tev_used++];
 }
 
+// Everything a grid build writes besides the view's own grid.
+struct build_ctx {
+  grid_scratch* sc;
+  dbuf<float4>* p4tmp;
+  dbuf<float>* htmp;
+  dbuf<uint32_t>* wcounts;
+  dbuf<uint32_t>* woffs;
+  hipStream_t st;
+};
+
+// the full grid of slab group gi
+void build_group(ilc_batch* b, size_t gi, const build_ctx& c) {
+  view_state& ls = *b->views[b->group_leader[gi]];
+  c.p4tmp->ensure(b->ps.n);
+  launch_view_project(b->ps, proj_of(b, ls), gaunt_of(b), c.p4tmp->p, c.st);
+  enqueue_grid_build(*b->group_grids[gi], *c.sc, c.p4tmp->p, b->ps.hsml, b->ps.n, ls.gp, ls.kz0,
+                     ls.max_h, b->kernel, b->counter.p + 1, c.st);
+}
+
+// view v's grid and cleared accumulator
+void build_view(ilc_batch* b, view_state& vs, const build_ctx& c) {
+  const hipStream_t st = c.st;
+  c.p4tmp->ensure(b->ps.n);
+  if (vs.windowed || vs.compact) {
+    // kept particles in input order, then the grid from their keys
+    const uint32_t nt = window_tiles(b->ps.n);
+    const uint64_t m = vs.nwin;
+    c.wcounts->ensure(size_t(nt) + 1);
+    c.woffs->ensure(size_t(nt) + 1);
+    c.htmp->ensure(std::max<uint64_t>(m, 1));
+    vs.grid.key.ensure(std::max<uint64_t>(m, 1));
+    vs.grid.perm.ensure(std::max<uint64_t>(m, 1));
+    c.sc->temp.ensure(scan_temp_words(uint64_t(nt) + 1));
+    launch_window_count(b->ps, proj_of(b, vs), vs.win, c.wcounts->p, nullptr, st);
+    launch_exclusive_scan(c.wcounts->p, c.woffs->p, nt, c.sc->temp.p, st);
+    launch_window_scatter(b->ps, proj_of(b, vs), gaunt_of(b), vs.win, c.woffs->p, vs.gp,
+                          c.p4tmp->p, c.htmp->p, vs.grid.key.p, vs.grid.perm.p,
+                          b->counter.p + 1, st);
+    enqueue_grid_from_keys(vs.grid, *c.sc, c.p4tmp->p, c.htmp->p, m, vs.gp, vs.kz0, vs.max_h,
+                           b->kernel, st);
+    vs.grid.density_n = vs.windowed ? b->ps.n : 0;
+  } else {
+    launch_view_project(b->ps, proj_of(b, vs), gaunt_of(b), c.p4tmp->p, st);
+    enqueue_grid_build(vs.grid, *c.sc, c.p4tmp->p, b->ps.hsml, b->ps.n, vs.gp, vs.kz0, vs.max_h,
+                       b->kernel, b->counter.p + 1, st);
+    vs.grid.density_n = 0;
+  }
+  launch_fill_f64(vs.acc.p, 0.0, uint64_t(vs.v.cam.nx) * vs.v.cam.ny, st);
+}
+
+void fill_desc(ilc_batch* b, size_t v) {
+  view_state& vs = *b->views[v];
+  const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
+  view_desc& d = b->desc_host.p[v];
+  d.g = grid_of(b, vs);
+  d.p = point_view{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
+  d.acc = vs.acc.p;
+  d.tiles_x = vs.tiles_x;
+  d.ntiles = vs.ntiles;
+}
+
+void ensure_lanes(ilc_batch* b) {
+  if (!b->ev_fork) ILC_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  while (b->lanes.size() < size_t(kLanes)) {
+    auto l = std::make_unique<build_lane>();
+    ILC_HIP(hipStreamCreateWithFlags(&l->st, hipStreamNonBlocking));
+    ILC_HIP(hipEventCreateWithFlags(&l->done, hipEventDisableTiming));
+    b->lanes.push_back(std::move(l));
+  }
+}
+
+// 2. device work of a planned render: persistent (the grid builds of all
+// views spread over kLanes streams, then one gather launch over every
+// (view, tile)) or plain (build, gather and retrieve one view after the
+// other on `s`).
 void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = false) {
   const size_t nv = b->views.size();
   b->tev_used = 0;
   const int op = b->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT;
-  b->p4tmp.ensure(b->ps.n);
   launch_fill_u32(b->counter.p, 0u, 2, s);
-  // full grid of each slab group (bounding box, cells and per-cell max h of
-  // every particle, as each member's reference grid has them)
-  for (size_t gi = 0; gi < b->group_leader.size(); ++gi) {
-    view_state& ls = *b->views[b->group_leader[gi]];
-    launch_view_project(b->ps, proj_of(b, ls), gaunt_of(b), b->p4tmp.p, s);
-    enqueue_grid_build(*b->group_grids[gi], b->scratch, b->p4tmp.p, b->ps.hsml, b->ps.n, ls.gp,
-                       ls.kz0, ls.max_h, b->kernel, b->counter.p + 1, s);
-  }
-  for (size_t v = 0; v < nv; ++v) {
-    view_state& vs = *b->views[v];
-    if (vs.windowed || vs.compact) {
-      // kept particles in input order, then the grid from their keys
-      const uint32_t nt = window_tiles(b->ps.n);
-      const uint64_t m = vs.nwin;
-      b->wcounts.ensure(size_t(nt) + 1);
-      b->woffs.ensure(size_t(nt) + 1);
-      b->htmp.ensure(std::max<uint64_t>(m, 1));
-      vs.grid.key.ensure(std::max<uint64_t>(m, 1));
-      vs.grid.perm.ensure(std::max<uint64_t>(m, 1));
-      b->scratch.temp.ensure(scan_temp_words(uint64_t(nt) + 1));
-      launch_window_count(b->ps, proj_of(b, vs), vs.win, b->wcounts.p, nullptr, s);
-      launch_exclusive_scan(b->wcounts.p, b->woffs.p, nt, b->scratch.temp.p, s);
-      launch_window_scatter(b->ps, proj_of(b, vs), gaunt_of(b), vs.win, b->woffs.p, vs.gp,
-                            b->p4tmp.p, b->htmp.p, vs.grid.key.p, vs.grid.perm.p,
-                            b->counter.p + 1, s);
-      enqueue_grid_from_keys(vs.grid, b->scratch, b->p4tmp.p, b->htmp.p, m, vs.gp, vs.kz0,
-                             vs.max_h, b->kernel, s);
-      vs.grid.density_n = vs.windowed ? b->ps.n : 0;
-    } else {
-      launch_view_project(b->ps, proj_of(b, vs), gaunt_of(b), b->p4tmp.p, s);
-      enqueue_grid_build(vs.grid, b->scratch, b->p4tmp.p, b->ps.hsml, b->ps.n, vs.gp, vs.kz0,
-                         vs.max_h, b->kernel, b->counter.p + 1, s);
-      vs.grid.density_n = 0;
-    }
-    const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
-    launch_fill_f64(vs.acc.p, 0.0, P, s);
-    view_desc& d = b->desc_host.p[v];
-    d.g = grid_of(b, vs);
-    d.p = point_view{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
-    d.acc = vs.acc.p;
-    d.tiles_x = vs.tiles_x;
-    d.ntiles = vs.ntiles;
-    if (!persistent) {
+  b->qcounters.ensure(size_t(kQueues) * kQueueStride);
+  launch_fill_u32(b->qcounters.p, 0u, uint64_t(kQueues) * kQueueStride, s);
+  const build_ctx main{&b->scratch, &b->p4tmp, &b->htmp, &b->wcounts, &b->woffs, s};
+  if (!persistent) {
+    for (size_t gi = 0; gi < b->group_leader.size(); ++gi) build_group(b, gi, main);
+    for (size_t v = 0; v < nv; ++v) {
+      view_state& vs = *b->views[v];
+      build_view(b, vs, main);
+      fill_desc(b, v);
+      const view_desc& d = b->desc_host.p[v];
       if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
       launch_gather(d.g, d.p, op, b->kernel, b->ts, vs.acc.p, nullptr, b->cubic.p, s);
       if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
-      launch_retrieve(vs.acc.p, P, vs.v.scale, vs.img.p, s);
+      launch_retrieve(vs.acc.p, d.p.n, vs.v.scale, vs.img.p, s);
     }
-  }
-  if (persistent) {
+  } else {
+    ensure_lanes(b);
+    ILC_HIP(hipEventRecord(b->ev_fork, s));
+    for (auto& l : b->lanes) ILC_HIP(hipStreamWaitEvent(l->st, b->ev_fork, 0));
+    size_t job = 0;
+    auto lane_ctx = [&](size_t j) {
+      build_lane& l = *b->lanes[j % b->lanes.size()];
+      return build_ctx{&l.scratch, &l.p4tmp, &l.htmp, &l.wcounts, &l.woffs, l.st};
+    };
+    for (size_t gi = 0; gi < b->group_leader.size(); ++gi) build_group(b, gi, lane_ctx(job++));
+    for (size_t v = 0; v < nv; ++v) {
+      build_view(b, *b->views[v], lane_ctx(job++));
+      fill_desc(b, v);
+    }
+    for (auto& l : b->lanes) {
+      ILC_HIP(hipEventRecord(l->done, l->st));
+      ILC_HIP(hipStreamWaitEvent(s, l->done, 0));
+    }
     ILC_HIP(hipMemcpyAsync(b->desc_dev.p, b->desc_host.p, sizeof(view_desc) * nv,
                            hipMemcpyHostToDevice, s));
     ILC_HIP(hipMemcpyAsync(b->items_dev.p, b->items_host.p, sizeof(uint2) * b->nitems,
                            hipMemcpyHostToDevice, s));
     if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
-    launch_gather_batch(b->desc_dev.p, b->items_dev.p, b->nitems, b->counter.p, op, b->kernel,
+    launch_gather_batch(b->desc_dev.p, b->items_dev.p, b->segs, b->qcounters.p, op, b->kernel,
                         b->ts, b->cubic.p, b->grid_blocks, s);
     if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
     for (size_t v = 0; v < nv; ++v) {

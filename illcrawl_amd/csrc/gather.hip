@@ -768,13 +768,17 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
   gather_tile<TW, PPT, OP, KERNEL>(g, p, tx, ty, acc_out, counts, support, sm);
 }
 
-// Persistent form over many views: every workgroup takes the next
-// (view, tile) item from a device counter until the list is exhausted.
+// Persistent gather over (view, tile) items.  Workgroups are dealt to the 8
+// XCDs round-robin by id, and each XCD has its own L2: queue q holds the q-th
+// eighth of every view's tiles (contiguous rows), so the tiles one XCD works
+// on at a time are neighbours that read the same particle spans.  A
+// workgroup drains its XCD's queue, then helps with the others; every queue
+// counter only grows, so every workgroup leaves once all queues are empty.
 template <int TW, int PPT, int OP, int KERNEL>
 __global__ __launch_bounds__(kBlock) void k_gather_batch(const view_desc* __restrict__ views,
                                                          const uint2* __restrict__ items,
-                                                         uint32_t nitems,
-                                                         uint32_t* __restrict__ counter,
+                                                         queue_segs segs,
+                                                         uint32_t* __restrict__ counters,
                                                          const float* __restrict__ cubic_table,
                                                          float support) {
   __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false> sm;
@@ -782,18 +786,23 @@ __global__ __launch_bounds__(kBlock) void k_gather_batch(const view_desc* __rest
   if constexpr (KERNEL == 2) {
     for (int i = threadIdx.x; i < kCubicTable; i += kBlock) sm.tab[i] = cubic_table[i];
   }
-  for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t w = uint32_t(uniform(int(next)));
-    if (w >= nitems) break;
-    const uint2 it = items[w];
-    const uint32_t vi = uint32_t(uniform(int(it.x))), ti = uint32_t(uniform(int(it.y)));
-    const view_desc& vd = views[vi];
-    const uint32_t tiles_x = vd.tiles_x;
-    gather_tile<TW, PPT, OP, KERNEL>(vd.g, vd.p, ti % tiles_x, ti / tiles_x, vd.acc, nullptr,
-                                     support, sm);
+  const uint32_t home = blockIdx.x % uint32_t(kQueues);
+  for (uint32_t k = 0; k < uint32_t(kQueues); ++k) {
+    const uint32_t q = (home + k) % uint32_t(kQueues);
+    const uint32_t qb = segs.seg[q], qe = segs.seg[q + 1];
+    for (;;) {
+      __syncthreads();
+      if (threadIdx.x == 0) next = atomicAdd(counters + q * kQueueStride, 1u);
+      __syncthreads();
+      const uint32_t got = uint32_t(uniform(int(next)));
+      if (got >= qe - qb) break;
+      const uint2 it = items[qb + got];
+      const uint32_t vi = uint32_t(uniform(int(it.x))), ti = uint32_t(uniform(int(it.y)));
+      const view_desc& vd = views[vi];
+      const uint32_t tiles_x = vd.tiles_x;
+      gather_tile<TW, PPT, OP, KERNEL>(vd.g, vd.p, ti % tiles_x, ti / tiles_x, vd.acc, nullptr,
+                                       support, sm);
+    }
   }
 }
 
@@ -843,10 +852,10 @@ struct plain_launcher {
 
 template <int TW, int PPT, int OP, int KERNEL>
 struct batch_launcher {
-  static int run(const view_desc* v, const uint2* it, uint32_t ni, uint32_t* c, const float* tab,
-                 float sup, int grid, hipStream_t s) {
+  static int run(const view_desc* v, const uint2* it, const queue_segs& segs, uint32_t* c,
+                 const float* tab, float sup, int grid, hipStream_t s) {
     hipLaunchKernelGGL((k_gather_batch<TW, PPT, OP, KERNEL>), dim3(grid), dim3(kBlock), 0, s, v,
-                       it, ni, c, tab, sup);
+                       it, segs, c, tab, sup);
     ILC_HIP(hipGetLastError());
     return 0;
   }
@@ -925,12 +934,14 @@ int gather_batch_grid(int op, int kernel, const tile_shape& ts) {
   return std::max(1, per_cu) * cus;
 }
 
-void launch_gather_batch(const view_desc* views, const uint2* items, uint32_t nitems,
-                         uint32_t* counter, int op, int kernel, const tile_shape& ts,
+void launch_gather_batch(const view_desc* views, const uint2* items, const queue_segs& segs,
+                         uint32_t* counters, int op, int kernel, const tile_shape& ts,
                          const float* cubic_table, int grid_blocks, hipStream_t s) {
   const float support = kernel == ILC_KERNEL_CUBIC ? 2.0f : 1.0f;
-  dispatch<batch_launcher>(ts, op, kernel, views, items, nitems, counter, cubic_table, support,
-                           grid_blocks, s);
+  // whole rounds of kQueues workgroups, so that every XCD gets the same number
+  const int grid = std::max(kQueues, grid_blocks / kQueues * kQueues);
+  dispatch<batch_launcher>(ts, op, kernel, views, items, segs, counters, cubic_table, support,
+                           grid, s);
 }
 
 void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,

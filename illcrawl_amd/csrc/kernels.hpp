@@ -146,8 +146,16 @@ void tile_grid(const tile_shape& ts, uint32_t lnx, uint32_t lny, uint32_t* tiles
                uint32_t* ntiles);
 // workgroups of the persistent batch gather that fit on the device at once
 int gather_batch_grid(int op, int kernel, const tile_shape& ts);
-void launch_gather_batch(const view_desc* views, const uint2* items, uint32_t nitems,
-                         uint32_t* counter, int op, int kernel, const tile_shape& ts,
+// Work items split into kQueues queues (queue q: items[seg[q] .. seg[q+1])):
+// a workgroup drains the queue of its XCD (blockIdx % kQueues) first, then
+// the others.  counters: kQueues counters kQueueStride words apart, zeroed.
+constexpr int kQueues = 8;
+constexpr int kQueueStride = 64;
+struct queue_segs {
+  uint32_t seg[kQueues + 1];
+};
+void launch_gather_batch(const view_desc* views, const uint2* items, const queue_segs& segs,
+                         uint32_t* counters, int op, int kernel, const tile_shape& ts,
                          const float* cubic_table, int grid_blocks, hipStream_t s);
 
 // ---- project.hip
