@@ -95,6 +95,7 @@ __device__ inline uint32_t prefix_count(uint64_t mask) {
 
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+
 // Exclusive scan over the 256 threads of the workgroup; *total = sum.
 // Contains two workgroup barriers.
 __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* ws) {
@@ -181,7 +182,18 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int col = t % TW, rg = t / TW;
+  // Point ownership: the wave's 64 * PPT points, in row-major order within
+  // its (64 * PPT / TW) rows, form 4 * PPT blocks of 16; lane l owns point
+  // (l % 16) of the blocks q = l / 16 + 4 k, k < PPT.
+  static_assert((64 * PPT) % TW == 0, "a wave covers whole rows");
+  constexpr int WROWS = (64 * PPT) / TW;
+  int prow[PPT], pcol[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const int pw = 16 * ((lane >> 4) + 4 * k) + (lane & 15);
+    prow[k] = wave * WROWS + pw / TW;
+    pcol[k] = pw % TW;
+  }
 
   float ex[PPT], ey[PPT];
   uint32_t pid[PPT];
@@ -198,8 +210,8 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
 
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
-    const uint32_t px = tx * TW + col;
-    const uint32_t py = ty * TH + rg * PPT + k;
+    const uint32_t px = tx * TW + uint32_t(pcol[k]);
+    const uint32_t py = ty * TH + uint32_t(prow[k]);
     pid[k] = py * p.lnx + px;
     pv[k] = px < p.lnx && py < p.lny && pid[k] < p.n;
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -309,7 +321,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
       const float gk = pv[k] ? adj->grad[pid[k]] * adj->scale : 0.0f;
-      sm.Pt[(rg * PPT + k) * TW + col] = make_float4(ex[k], ey[k], gk, 0.0f);
+      sm.Pt[prow[k] * TW + pcol[k]] = make_float4(ex[k], ey[k], gk, 0.0f);
     }
     __syncthreads();
     for (int sb = wave; sb < 16; sb += 4) {
