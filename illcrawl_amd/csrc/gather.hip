@@ -183,8 +183,8 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   // Point ownership: the wave's 64 * PPT points, in row-major order within
-  // its (64 * PPT / TW) rows, form 4 * PPT blocks of 16; lane l owns point
-  // (l % 16) of the blocks q = l / 16 + 4 k, k < PPT.
+  // its (64 * PPT / TW) rows of the tile; lane l owns point
+  // 16 (l / 16 + 4 k) + l % 16 of them, k < PPT.
   static_assert((64 * PPT) % TW == 0, "a wave covers whole rows");
   constexpr int WROWS = (64 * PPT) / TW;
   int prow[PPT], pcol[PPT];
@@ -489,7 +489,6 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
 #pragma unroll
         for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
         for (int j = 0; j < jn; ++j) {
-          const float4 q = sm.Lm[j0 + j];
           const int c = uniform(sm.Lmc[j0 + j]);
           if (c != cur) {
             cur = c;
@@ -499,6 +498,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
               keepf[k] = (zin[k] & (cx >= lox[k]) & (cx <= hix[k]) & (cy >= loy[k]) &
                           (cy <= hiy[k])) ? 1.0f : 0.0f;
           }
+          const float4 q = sm.Lm[j0 + j];
 #pragma unroll
           for (int k = 0; k < PPT; ++k) {
             const float dx = q.x - ex[k], dy = q.y - ey[k];
