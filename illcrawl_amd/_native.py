@@ -95,6 +95,12 @@ SIGNATURES = {
     "ilc_vector_scale": (_ST, [_P, _U64, ctypes.c_float, _P]),
     "ilc_crawl_block": (_ST, [ctypes.POINTER(VolumeC), _P, _U64, _P, ctypes.c_int32, _P, _P,
                               _P, ctypes.POINTER(_U64), _P]),
+    "ilc_quantity_xray_flux": (_ST, [_P, _P, _P, _U64, _P, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_int32, _P, _P, _P]),
+    "ilc_quantity_xray_spectral_flux": (_ST, [_P, _P, _P, _U64, _P, ctypes.c_int32,
+                                              ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                              ctypes.c_float, ctypes.c_float, _P, _P]),
     "ilc_quantity_scaled": (_ST, [_P, ctypes.c_double, _U64, _P, _P]),
     "ilc_quantity_xray_bolometric": (_ST, [_P, _P, _P, _U64, _P, ctypes.c_int32, ctypes.c_float,
                                            ctypes.c_float, _P, _P]),

@@ -258,6 +258,62 @@ def quantity_scaled(inp, scale, n=None, device=0, stream=None):
     return out
 
 
+def _gaunt2d(table):
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    if t.ndim != 2:
+        raise ValueError("the Gaunt table is 2-D: rows of constant ln(u), columns of ln(gamma^2)")
+    return t, t.shape[1], t.shape[0]
+
+
+def quantity_xray_flux(rho, u, xe, table, z, lum_dist, e_min, e_max, num_samples, device=0,
+                       stream=None, with_steps=False):
+    """xray_flux (quantities.cl:354-386): band-integrated bremsstrahlung per
+    particle from density, internal energy and electron abundance."""
+    dev = torch.device("cuda", device)
+    r = _dev_f32(rho, dev).reshape(-1)
+    uu = _dev_f32(u, dev).reshape(-1)
+    x = _dev_f32(xe, dev).reshape(-1)
+    t, nx, ny = _gaunt2d(table)
+    out = torch.empty_like(r)
+    steps = torch.empty(r.shape[0], dtype=torch.int32, device=dev) if with_steps else None
+    N.call("ilc_quantity_xray_flux", _ptr(r), _ptr(uu), _ptr(x), r.shape[0],
+           t.ctypes.data_as(ctypes.c_void_p), nx, ny, ctypes.c_float(z),
+           ctypes.c_float(lum_dist), ctypes.c_float(e_min), ctypes.c_float(e_max),
+           int(num_samples), _ptr(out), _ptr(steps), _stream(stream))
+    return (out, steps) if with_steps else out
+
+
+def quantity_xray_spectral_flux(rho, u, xe, table, z, lum_dist, e, de, device=0, stream=None):
+    """xray_spectral_flux (quantities.cl:400-430): one energy bin."""
+    dev = torch.device("cuda", device)
+    r = _dev_f32(rho, dev).reshape(-1)
+    uu = _dev_f32(u, dev).reshape(-1)
+    x = _dev_f32(xe, dev).reshape(-1)
+    t, nx, ny = _gaunt2d(table)
+    out = torch.empty_like(r)
+    N.call("ilc_quantity_xray_spectral_flux", _ptr(r), _ptr(uu), _ptr(x), r.shape[0],
+           t.ctypes.data_as(ctypes.c_void_p), nx, ny, ctypes.c_float(z),
+           ctypes.c_float(lum_dist), ctypes.c_float(e), ctypes.c_float(de), _ptr(out),
+           _stream(stream))
+    return out
+
+
+def load_gaunt_table(path):
+    """Reads a thermally averaged Gaunt table in the layout of van Hoof's
+    distribution (gaunt_factor.dat): '#' comment lines, then one row of
+    whitespace-separated values per ln(u) node.  Returns float32 (rows, cols)."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if not line or line.startswith("#"):
+                continue
+            rows.append([float(v) for v in line.split()])
+    if not rows or any(len(r) != len(rows[0]) for r in rows):
+        raise ValueError(f"{path}: not a rectangular table")
+    return np.asarray(rows, np.float32)
+
+
 def quantity_xray_bolometric(rho, temp, xe, table, x0, dx, device=0, stream=None):
     dev = torch.device("cuda", device)
     r = _dev_f32(rho, dev).reshape(-1)

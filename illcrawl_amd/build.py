@@ -22,7 +22,7 @@ DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "backend_demo.cpp")
 DEMO = os.path.join(LIBDIR, "backend_demo")
 
 SOURCES = ["host.cpp", "gridbuild.cpp", "capi.cpp", "batch.cpp", "crawl.hip", "grid.hip",
-           "gather.hip",
+           "gather.hip", "xray.hip",
            "project.hip", "batch.hip"]
 
 ARCH = "gfx950"

@@ -445,6 +445,56 @@ ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t
   });
 }
 
+namespace {
+// the Gaunt table on the device for the duration of one call
+struct device_table {
+  float* p = nullptr;
+  device_table(const float* host, int32_t nx, int32_t ny, hipStream_t s) {
+    ILC_REQUIRE(host, "null Gaunt table");
+    ILC_REQUIRE(nx >= 1 && ny >= 1 && int64_t(nx) * ny <= (int64_t(1) << 24),
+                "Gaunt table %d x %d out of range", nx, ny);
+    ILC_HIP(hipMalloc(&p, sizeof(float) * size_t(nx) * ny));
+    ILC_HIP(hipMemcpyAsync(p, host, sizeof(float) * size_t(nx) * ny, hipMemcpyHostToDevice, s));
+  }
+  ~device_table() {
+    if (p) (void)hipFree(p);
+  }
+};
+}  // namespace
+
+ilc_status ilc_quantity_xray_flux(const float* rho_dev, const float* u_dev, const float* xe_dev,
+                                  uint64_t n, const float* gaunt_host, int32_t gnx, int32_t gny,
+                                  float z, float lum_dist, float e_min, float e_max,
+                                  int32_t num_samples, float* w_dev, int32_t* steps_dev,
+                                  void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(num_samples >= 1, "num_samples must be >= 1");
+    if (!n) return;
+    ILC_REQUIRE(rho_dev && u_dev && xe_dev && w_dev, "null argument");
+    hipStream_t s = as_stream(stream);
+    device_table t(gaunt_host, gnx, gny, s);
+    launch_xray_flux(rho_dev, u_dev, xe_dev, n, t.p, gnx, gny, z, lum_dist, e_min, e_max,
+                     num_samples, w_dev, steps_dev, s);
+    ILC_HIP(hipStreamSynchronize(s));
+  });
+}
+
+ilc_status ilc_quantity_xray_spectral_flux(const float* rho_dev, const float* u_dev,
+                                           const float* xe_dev, uint64_t n,
+                                           const float* gaunt_host, int32_t gnx, int32_t gny,
+                                           float z, float lum_dist, float e, float de,
+                                           float* w_dev, void* stream) {
+  return guarded([&] {
+    if (!n) return;
+    ILC_REQUIRE(rho_dev && u_dev && xe_dev && w_dev, "null argument");
+    hipStream_t s = as_stream(stream);
+    device_table t(gaunt_host, gnx, gny, s);
+    launch_xray_spectral_flux(rho_dev, u_dev, xe_dev, n, t.p, gnx, gny, z, lum_dist, e, de,
+                              w_dev, s);
+    ILC_HIP(hipStreamSynchronize(s));
+  });
+}
+
 ilc_status ilc_quantity_scaled(const float* in_dev, double scale, uint64_t n, float* w_dev,
                                void* stream) {
   return guarded([&] {

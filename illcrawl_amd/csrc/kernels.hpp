@@ -170,6 +170,14 @@ void launch_project_points(const camera_frame& f, const float4* in, uint64_t n, 
 void launch_vector_scale(float* x, float alpha, uint64_t n, hipStream_t s);
 void launch_retrieve(const double* acc, uint64_t n, float scale, float* out, hipStream_t s);
 void launch_quantity_scaled(const float* in, double scale, uint64_t n, float* w, hipStream_t s);
+// The reference's spectral X-ray emissivity (2-D Gaunt table on the device,
+// t[i + nx j], i along ln(gamma^2) from -6, j along ln(u) from -16, step 0.2)
+void launch_xray_flux(const float* rho, const float* u, const float* xe, uint64_t n,
+                      const float* gaunt_dev, int nx, int ny, float z, float dl, float e_min,
+                      float e_max, int num_samples, float* out, int32_t* steps, hipStream_t s);
+void launch_xray_spectral_flux(const float* rho, const float* u, const float* xe, uint64_t n,
+                               const float* gaunt_dev, int nx, int ny, float z, float dl, float E,
+                               float dE, float* out, hipStream_t s);
 void launch_xray_bolometric(const float* rho, const float* temp, const float* xe, uint64_t n,
                             const float* table_dev, int32_t ntab, float x0, float dx, float* w,
                             hipStream_t s);

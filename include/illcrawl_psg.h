@@ -260,6 +260,35 @@ ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t
 
 /* ------------------------------------------------- quantity transformation */
 
+/* The reference's X-ray quantities (quantities.cl:39-239, 354-430): thermal
+ * bremsstrahlung of a hydrogen plasma in keV/s/kpc^3/m^2 from density rho
+ * (M_sun/kpc^3), specific internal energy u ((km/s)^2, temperature as
+ * get_temperature) and electron abundance xe, at redshift z and luminosity
+ * distance lum_dist (kpc).  gaunt_host: the van Hoof (2014) thermally averaged
+ * free-free Gaunt table as the reference loads it (gaunt_factor.cpp:30-41):
+ * gny rows of constant ln(u) from -16, gnx columns of constant ln(gamma^2)
+ * from -6, spacing 0.2, row-major; sampled like read_imagef with a linear,
+ * clamp-to-border (0) sampler.  exp, log, pow and rsqrt are correctly
+ * rounded FP32 (the reference leaves them to the OpenCL runtime).
+ * Synchronous (copies the table to the device and waits). */
+
+/* xray_flux: the band [e_min, e_max] keV (observer frame, times 1+z)
+ * integrated with the reference's adaptive Runge-Kutta-Fehlberg scheme
+ * (integration.cl:38-131; relative step tolerance 1/num_samples).
+ * steps_dev (may be NULL): RKF steps taken per particle (at most 65536). */
+ilc_status ilc_quantity_xray_flux(const float* rho_dev, const float* u_dev, const float* xe_dev,
+                                  uint64_t n, const float* gaunt_host, int32_t gnx, int32_t gny,
+                                  float z, float lum_dist, float e_min, float e_max,
+                                  int32_t num_samples, float* w_dev, int32_t* steps_dev,
+                                  void* stream);
+
+/* xray_spectral_flux: the emission in one energy bin of width dE at E keV. */
+ilc_status ilc_quantity_xray_spectral_flux(const float* rho_dev, const float* u_dev,
+                                           const float* xe_dev, uint64_t n,
+                                           const float* gaunt_host, int32_t gnx, int32_t gny,
+                                           float z, float lum_dist, float e, float de,
+                                           float* w_dev, void* stream);
+
 /* quantity_transformation with unprocessed_quantity / constant_quantity
  * (quantity.cpp:84-131, quantities.cl:701-726): w[i] = float(scale * in[i])
  * (in_dev == NULL: w[i] = float(scale)). */

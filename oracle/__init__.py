@@ -50,6 +50,11 @@ PINNING = {
     "orc_temperature": "closed form (gamma-1) u mu 121.1475502",
     "orc_xray_bolometric": "closed form on table-node-free inputs",
     "orc_backward_exact": "FP64 central differences of orc_gather_exact in w, h, x, y",
+    "orc_xray_flux": "constant Gaunt table: band integral of exp(-E/kT) in closed form "
+                     "(RKF within its tolerance); a table linear in its indices recovers "
+                     "the sampling coordinates ln(gamma^2), ln(u) exactly",
+    "orc_xray_spectral_flux": "closed form of the bremsstrahlung prefactor chain at one "
+                              "energy with a constant table",
     "orc_crawl_block": "hand-built rows: periodic images moved in by one period (each "
                        "axis, both signs), border tolerance, inclusive volume faces, "
                        "input order kept; numpy brute force on random rows",
@@ -136,6 +141,14 @@ def _declare(L):
                                   ctypes.c_double, _f32p, _u64, ctypes.c_void_p, ctypes.c_int32,
                                   _f32p, ctypes.c_void_p, _u64p]
     L.orc_crawl_block.restype = ctypes.c_uint64
+    L.orc_xray_flux.argtypes = [_f32p, _f32p, _f32p, _u64, ctypes.c_float, ctypes.c_float, _f32p,
+                                ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_int32, _f32p, _i32p]
+    L.orc_xray_flux.restype = None
+    L.orc_xray_spectral_flux.argtypes = [_f32p, _f32p, _f32p, _u64, ctypes.c_float,
+                                         ctypes.c_float, _f32p, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_float, ctypes.c_float, _f32p]
+    L.orc_xray_spectral_flux.restype = None
     L.orc_num_threads.argtypes = []
     L.orc_num_threads.restype = ctypes.c_int
     L.orc_set_num_threads.argtypes = [ctypes.c_int]
@@ -334,6 +347,30 @@ def crawl_block(center, extent, period, border, xyz, cols=()):
                               xyz, n, ctypes.cast(cin, ctypes.c_void_p), len(cols), out4,
                               ctypes.cast(cout, ctypes.c_void_p), idx)
     return out4[:m], [o[:m] for o in outs], idx[:m]
+
+
+def xray_flux(rho, u, xe, z, dl, table, e_min, e_max, num_samples):
+    """xray_flux: band-integrated bremsstrahlung flux per particle (RKF in
+    energy); table is the 2-D Gaunt table as (rows = ln u, cols = ln gamma^2).
+    Returns (flux, RKF steps)."""
+    rho, u, xe = (_c(a, np.float32).reshape(-1) for a in (rho, u, xe))
+    tab = _c(table, np.float32)
+    ny, nx = tab.shape
+    out = np.zeros(rho.shape[0], np.float32)
+    steps = np.zeros(rho.shape[0], np.int32)
+    lib().orc_xray_flux(rho, u, xe, rho.shape[0], float(z), float(dl), tab.reshape(-1), nx, ny,
+                        float(e_min), float(e_max), int(num_samples), out, steps)
+    return out, steps
+
+
+def xray_spectral_flux(rho, u, xe, z, dl, table, E, dE):
+    rho, u, xe = (_c(a, np.float32).reshape(-1) for a in (rho, u, xe))
+    tab = _c(table, np.float32)
+    ny, nx = tab.shape
+    out = np.zeros(rho.shape[0], np.float32)
+    lib().orc_xray_spectral_flux(rho, u, xe, rho.shape[0], float(z), float(dl), tab.reshape(-1),
+                                 nx, ny, float(E), float(dE), out)
+    return out
 
 
 def num_threads():

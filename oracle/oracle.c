@@ -810,3 +810,162 @@ uint64_t orc_crawl_block(const double center[3], const double extent[3],
   }
   return m;
 }
+
+/* ------------------------------------------------------------------ */
+/* Spectral bremsstrahlung X-ray emissivity (quantities.cl:69-239,     */
+/* 354-430) with the Runge-Kutta-Fehlberg energy integration of        */
+/* integration.cl:38-131.                                               */
+/*                                                                      */
+/* The reference leaves exp, log, pow and rsqrt to the OpenCL runtime  */
+/* (unspecified precision).  This restatement uses correctly rounded   */
+/* FP32 versions (the FP64 function rounded to FP32), the convention    */
+/* the HIP path shares, so that both take the same accept/reject        */
+/* decisions in the step-size control.  The step loop stops after       */
+/* ORC_RKF_MAX_STEPS steps (the reference has no bound).                 */
+/* ------------------------------------------------------------------ */
+#define ORC_K_B 8.61733e-5f
+#define ORC_RYD 13.605f
+#define ORC_RKF_MAX_STEPS 65536
+
+static float crf_expf(float x) { return (float)exp((double)x); }
+static float crf_logf(float x) { return (float)log((double)x); }
+static float crf_powf(float x, float y) { return (float)pow((double)x, (double)y); }
+static float crf_rsqrtf(float x) { return (float)(1.0 / sqrt((double)x)); }
+
+/* g_ff (quantities.cl:69-83): van Hoof table on (ln gamma^2, ln u), nodes
+ * from (-6, -16) in steps of 0.2, sampled as read_imagef linear */
+static float orc_gff(float E, float T, float Z, const float* tab, int32_t nx, int32_t ny)
+{
+  float kT = ORC_K_B * T;
+  float gam2 = Z * Z;
+  gam2 = gam2 * ORC_RYD;
+  gam2 = gam2 / kT;
+  float u = 1.e3f * E;
+  u = u / kT;
+  return orc_tabulated_2d(tab, nx, ny, -6.f, -16.f, 0.2f, 0.2f, crf_logf(gam2), crf_logf(u));
+}
+
+/* get_spectral_brems_emission (quantities.cl:97-136) */
+static float orc_brems(float E, float dE, float T, float xe, float rho, float r,
+                       const float* tab, int32_t nx, int32_t ny)
+{
+  float ion = 4.046263f * rho;
+  ion = ion * xe;
+  const float Z = 1.f;
+  float t_inv = 1000.f / (ORC_K_B * T);
+  float r2 = r * r;
+  float e = 0.002535f * dE;
+  e = e * Z;
+  e = e * Z;
+  e = e * ion;
+  e = e * ion;
+  e = e * orc_gff(E, T, Z, tab, nx, ny);
+  e = e * crf_rsqrtf(T);
+  e = e * crf_expf(-E * t_inv);
+  return e / r2;
+}
+
+static void orc_rkf_points(float pos, float step, float* p)
+{
+  p[0] = pos + (3.f / 8.f) * step;
+  p[1] = pos + (12.f / 13.f) * step;
+  p[2] = pos + step;
+  p[3] = pos + 0.5f * step;
+}
+
+/* integrate_spectral_flux (quantities.cl:156-239) for XRAY_FLUX, with
+ * rkf_advance (integration.cl:49-131) called as the reference calls it:
+ * absolute tolerance = rel_tol, relative tolerance = 1, epsilon = 0,
+ * step sizes in [1e-3, 100]. */
+static float orc_xray_integrate(float e_min, float e_max, float rel_tol, float T, float xe,
+                                float rho, float dl, const float* tab, int32_t nx, int32_t ny,
+                                int32_t* steps_out)
+{
+  float state = 0.0f;
+  float pos = e_min;
+  const float end = e_max;
+  float step = (end - pos) * rel_tol;
+  float rb = orc_brems(pos, 1.0f, T, xe, rho, dl, tab, nx, ny);
+  float pts[4];
+  orc_rkf_points(pos, step, pts);
+  const float abs_tol = rel_tol, rel = 1.0f, eps = 0.0f, min_step = 1.e-3f, max_step = 100.f;
+  int32_t it = 0;
+  while (pos < end && it < ORC_RKF_MAX_STEPS) {
+    ++it;
+    float v[4];
+    for (int k = 0; k < 4; ++k) v[k] = orc_brems(pts[k], 1.0f, T, xe, rho, dl, tab, nx, ny);
+    float d4 = (25.f / 216.f) * rb;
+    d4 = d4 + (1408.f / 2565.f) * v[0];
+    d4 = d4 + (2197.f / 4101.f) * v[1];
+    d4 = d4 - (1.f / 5.f) * v[2];
+    float d5 = (16.f / 135.f) * rb;
+    d5 = d5 + (6656.f / 12825.f) * v[0];
+    d5 = d5 + (28561.f / 56430.f) * v[1];
+    d5 = d5 - (9.f / 50.f) * v[2];
+    d5 = d5 + (2.f / 55.f) * v[3];
+    d4 = d4 * step;
+    d5 = d5 * step;
+    const float est4 = state + d4;
+    const float old = pos;
+    pos = pos + step;
+    float s = 2.0f;
+    if (d4 != d5) {
+      const float err = fabsf(d5 - d4);
+      float srel = state / pos;
+      srel = fabsf(srel * rel);
+      const float tol = fmaxf(abs_tol, srel);
+      float q = tol * step;
+      q = q / (2.f * err);
+      s = crf_powf(q, 0.25f);
+    }
+    float nstep = s * step;
+    if (nstep < min_step)
+      nstep = min_step;
+    else if (nstep > max_step)
+      nstep = max_step;
+    s = nstep / step;
+    if (s < 0.95f) {
+      pos = old;
+    } else {
+      state = est4;
+      rb = v[2];
+    }
+    step = nstep;
+    if (pos + step >= end) {
+      step = end - pos;
+      step = step + eps;
+    }
+    orc_rkf_points(pos, step, pts);
+  }
+  if (steps_out) *steps_out = it;
+  return state;
+}
+
+/* xray_flux (quantities.cl:354-386): out[i] in keV/s/kpc^3/m^2 for the
+ * band [E_min, E_max] keV at redshift z and luminosity distance dl kpc;
+ * T from the internal energy u and electron abundance xe. */
+void orc_xray_flux(const float* rho, const float* u, const float* xe, uint64_t n, float z,
+                   float dl, const float* tab, int32_t nx, int32_t ny, float e_min, float e_max,
+                   int32_t num_samples, float* out, int32_t* steps)
+{
+  const float zf = 1.f + z;
+  const float rel_tol = 1.f / (float)num_samples;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const float T = orc_temperature(u[i], xe[i]);
+    out[i] = orc_xray_integrate(zf * e_min, zf * e_max, rel_tol, T, xe[i], rho[i], dl, tab,
+                                nx, ny, steps ? steps + i : 0);
+  }
+}
+
+/* xray_spectral_flux (quantities.cl:400-430): one energy bin E +- dE/2 */
+void orc_xray_spectral_flux(const float* rho, const float* u, const float* xe, uint64_t n,
+                            float z, float dl, const float* tab, int32_t nx, int32_t ny, float E,
+                            float dE, float* out)
+{
+  const float zf = 1.f + z;
+  for (uint64_t i = 0; i < n; ++i) {
+    const float T = orc_temperature(u[i], xe[i]);
+    out[i] = orc_brems(zf * E, zf * dE, T, xe[i], rho[i], dl, tab, nx, ny);
+  }
+}

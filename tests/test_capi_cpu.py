@@ -107,3 +107,14 @@ def test_cpp_demo_built_and_reports_usage():
         pytest.skip("backend_demo not built")
     r = subprocess.run([demo], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_gaunt_table_loader(tmp_path):
+    from illcrawl_amd import backend
+    f = tmp_path / "g.dat"
+    f.write_text("#\n# comment\n#\n  1.0e+00  2.0e+00 3.5\n  4.0 5.0 6.0\n")
+    t = backend.load_gaunt_table(str(f))
+    assert t.shape == (2, 3) and t.dtype == np.float32 and t[1, 2] == 6.0
+    f.write_text("1 2 3\n4 5\n")
+    with pytest.raises(ValueError):
+        backend.load_gaunt_table(str(f))

@@ -436,3 +436,80 @@ def test_crawl_block_vs_numpy(orc):
     assert np.array_equal(idx, np.flatnonzero(keep))
     assert np.array_equal(p4[:, :3], xc[keep].astype(np.float32))
     assert np.array_equal(ck, col[keep])
+
+
+K_B, RYD = 8.61733e-5, 13.605
+
+
+def _temperature64(u, xe):
+    mu = 4.0 / (1.0 + 3 * 0.76 + 4 * 0.76 * xe)
+    return (5.0 / 3.0 - 1.0) * u * mu * 121.1475502
+
+
+def test_xray_spectral_flux_closed_form(orc):
+    """One energy bin with a constant Gaunt table g0: 0.002535 dE ion^2 g0
+    T^-1/2 exp(-E/kT) / r^2, ion = 4.046263 rho xe (quantities.cl:97-136)."""
+    g0 = 1.37
+    tab = np.full((146, 81), g0, np.float32)
+    rho = np.array([1e4, 3e5, 2e7], np.float32)
+    u = np.array([2e4, 8e4, 6e5], np.float32)   # T inside the table's gamma^2 range
+    xe = np.array([1.05, 1.16, 1.2], np.float32)
+    z, dl, E, dE = 0.05, 1.2e5, 1.3, 0.25
+    got = orc.xray_spectral_flux(rho, u, xe, z, dl, tab, E, dE)
+    T = _temperature64(u.astype(np.float64), xe.astype(np.float64))
+    ion = 4.046263 * rho.astype(np.float64) * xe
+    Ez, dEz = (1 + z) * E, (1 + z) * dE
+    want = 0.002535 * dEz * ion ** 2 * g0 / np.sqrt(T) * np.exp(-Ez * 1000.0 / (K_B * T)) / dl ** 2
+    assert np.allclose(got, want, rtol=2e-5)
+
+
+def test_xray_gaunt_coordinates(orc):
+    """A table linear in its indices, t[j][i] = i + 1000 j, is reproduced
+    exactly by the linear sampler inside the table: the flux then reveals the
+    sampling position (ln(gamma^2) + 6) / 0.2 + 1000 (ln(u) + 16) / 0.2 - 500.5."""
+    j, i = np.meshgrid(np.arange(146), np.arange(81), indexing="ij")
+    tab = (i + 1000.0 * j).astype(np.float32)
+    rho, u, xe = (np.array([v], np.float32) for v in (2e6, 3e5, 1.1))
+    z, dl, E, dE = 0.0, 1.0, 2.0, 1.0
+    got = float(orc.xray_spectral_flux(rho, u, xe, z, dl, tab, E, dE)[0])
+    T = float(_temperature64(3e5, np.float64(np.float32(1.1))))
+    kT = K_B * T
+    gx = (np.log(RYD / kT) + 6.0) / 0.2 - 0.5
+    gy = (np.log(1000.0 * E / kT) + 16.0) / 0.2 - 0.5
+    assert 1 < gx < 79 and 1 < gy < 144
+    pref = 0.002535 * (4.046263 * 2e6 * 1.1) ** 2 / np.sqrt(T) * np.exp(-E * 1000.0 / kT)
+    assert abs(got / pref - (gx + 1000.0 * gy)) < 2e-3 * (gx + 1000.0 * gy) * 1e-2
+
+
+def test_xray_flux_band_integral(orc):
+    """Constant table: the RKF band integral equals
+    C kT (exp(-E0/kT) - exp(-E1/kT)) over the redshifted band.  The
+    reference's step control compares step errors with an absolute
+    tolerance of 1/num_samples and a relative one scaled by the running
+    integral (integration.cl:86-96 as called at quantities.cl:226-237): it
+    is accurate for integrands of order one or more that vary slowly over
+    the band (kT above about 1.3 keV here), and the densities, distance and
+    temperatures are chosen for that (steep integrands take steps of
+    several keV and miss by up to ~10%, as the reference does).  Its
+    4th-order weights (2197/4101 where Fehlberg has 2197/4104,
+    integration.cl:69) add a bias of about 4e-4 per step."""
+    g0 = 1.21
+    tab = np.full((146, 81), g0, np.float32)
+    rng = np.random.default_rng(4)
+    n = 64
+    rho = (10.0 ** rng.uniform(5.5, 6.5, n)).astype(np.float32)
+    u = (10.0 ** rng.uniform(5.5, 5.95, n)).astype(np.float32)
+    xe = rng.uniform(1.0, 1.2, n).astype(np.float32)
+    z, dl, e0, e1 = 0.02, 450.0, 0.5, 7.0
+    T = _temperature64(u.astype(np.float64), xe.astype(np.float64))
+    t_inv = 1000.0 / (K_B * T)
+    ion = 4.046263 * rho.astype(np.float64) * xe
+    C = 0.002535 * ion ** 2 * g0 / np.sqrt(T) / dl ** 2
+    a, b = (1 + z) * e0, (1 + z) * e1
+    want = C * (np.exp(-a * t_inv) - np.exp(-b * t_inv)) / t_inv
+    errs, steps_mean = [], []
+    for ns in (100, 1000):
+        got, steps = orc.xray_flux(rho, u, xe, z, dl, tab, e0, e1, ns)
+        errs.append(float(np.max(np.abs(got - want) / want)))
+        steps_mean.append(float(steps.mean()))
+    assert errs[0] < 5e-3 and errs[1] < 5e-3
