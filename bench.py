@@ -13,8 +13,9 @@ Workloads (synthetic, seeded; see illcrawl_amd/synth.py):
   c5            32 X-ray slabs (500 kpc/h, centre mask) of a 64M-cell snapshot at
                 4096^2 each, batched;
   c4            forward and adjoint of a 1024^2 column-density image of 4M
-                cells (exact mode, quartic): adjoint cost relative to forward
-                and gradient error against the FP64 oracle on a sample.
+                cells (exact mode, --kernel quartic|cubic): adjoint cost
+                relative to forward and gradient error against the FP64 oracle
+                on a sample.
 
 A step is one pass of the hot path over the workload, with the snapshot
 already resident in HBM: per view the quantity functor, the camera projection
@@ -195,7 +196,7 @@ def adjoint_main(args):
     pts = B.camera_pixel_points(view)
     rng = np.random.default_rng(7)
     m_t = torch.from_numpy((snap.mass * rng.lognormal(0.0, 0.05, snap.n)).astype(np.float32))
-    b = B.ProjectiveSmoothingGrid(mode="exact", kernel="quartic")
+    b = B.ProjectiveSmoothingGrid(mode="exact", kernel=args.kernel)
     b.init_backend(snap.n)
 
     def forward(w=None):
@@ -226,7 +227,7 @@ def adjoint_main(args):
     bwd_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     log(f"[bench] c4 forward {fwd_ms:.2f} ms, adjoint {bwd_ms:.2f} ms, evals {total:.6g}")
     line = {
-        "metric": "adjoint vs forward cost, projected SPH image (exact quartic)",
+        "metric": f"adjoint vs forward cost, projected SPH image (exact {args.kernel})",
         "value": total / (fwd_ms * 1e-3), "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": fwd_ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -234,7 +235,7 @@ def adjoint_main(args):
         "data": "synthetic (seeded: uniform + NFW halo, 32-NN density, lognormal masses)",
         "config": dict(workload="adjoint of a 1024^2 column-density image of 4M cells",
                        n_cells=snap.n, box_kpc_h=snap.box, resolution=[1024, 1024],
-                       width_kpc_h=2000.0, mode="exact", kernel="quartic",
+                       width_kpc_h=2000.0, mode="exact", kernel=args.kernel,
                        evals_per_step=total, parallelism="single GPU"),
         "adjoint": dict(ms=bwd_ms, forward_ms=fwd_ms, ratio=bwd_ms / fwd_ms,
                         evals_per_s=total / (bwd_ms * 1e-3)),
@@ -245,7 +246,8 @@ def adjoint_main(args):
         sel = np.sort(rng.choice(snap.n, size=2000, replace=False))
         pp = proj.cpu().numpy()[sel]
         ref = backward_exact(pp, snap.hsml[sel], pts.cpu().numpy(),
-                             G.cpu().numpy().astype(np.float64), 1)
+                             G.cpu().numpy().astype(np.float64),
+                             {"quartic": 1, "quartic_ref": 0, "cubic": 2}[args.kernel])
         errs = {}
         for name, g, r in zip(("m", "h", "x", "y"), grads, ref):
             got = g.cpu().numpy()[sel].astype(np.float64)
@@ -262,6 +264,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=os.environ.get("ILC_BENCH_WORKLOAD", "c3"))
     ap.add_argument("--mode", default="reference", choices=["reference", "exact"])
+    ap.add_argument("--kernel", default="quartic", choices=["quartic", "quartic_ref", "cubic"],
+                    help="projected kernel of the adjoint workload (c4)")
     ap.add_argument("--tile", default=None, help="WxH gather tile; default: autotuned")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")

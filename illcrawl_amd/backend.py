@@ -115,9 +115,9 @@ class ProjectiveSmoothingGrid:
         return out
 
     def backward(self, dl_dvalue, scale=1.0, stream=None):
-        """Adjoint of run + retrieve_results(scale) (exact mode, quartic
-        kernels): returns (dL/dw, dL/dh, dL/dx, dL/dy) per particle in input
-        order, x and y in the projected frame."""
+        """Adjoint of run + retrieve_results(scale) (exact mode): returns
+        (dL/dw, dL/dh, dL/dx, dL/dy) per particle in input order, x and y in
+        the projected frame."""
         g = _dev_f32(dl_dvalue, self.device).reshape(-1)
         if g.shape[0] != self._npoints:
             raise ValueError(f"expected {self._npoints} point gradients, got {g.shape[0]}")

@@ -15,6 +15,7 @@
 namespace ilc {
 const char* last_error_cstr();
 std::vector<float> cubic_projected_table(int n);
+std::vector<float> cubic_projected_dtable(int n);
 }  // namespace ilc
 
 using namespace ilc;
@@ -38,8 +39,8 @@ struct ilc_psg {
   uint32_t lnx = 0, lny = 0;
   dbuf<double> acc;
 
-  dbuf<float> cubic;
-  bool cubic_ready = false;
+  dbuf<float> cubic, dcubic;
+  bool cubic_ready = false, dcubic_ready = false;
   dbuf<unsigned long long> cnt, cnt_scratch;
   dbuf<float> grad_sorted;  // adjoint: 4 x n gradients in grid order
 
@@ -235,8 +236,6 @@ ilc_status ilc_psg_backward(ilc_psg* h, const float* dl_dvalue_dev, float scale,
     if (!h->pts) fail(ILC_ENOTREADY, "backward before setup_evaluation_points");
     if (h->mode != ILC_MODE_EXACT)
       fail(ILC_EINVAL, "backward needs exact mode (the reference-mode search box is not smooth)");
-    if (h->kernel != ILC_KERNEL_QUARTIC_REF && h->kernel != ILC_KERNEL_QUARTIC)
-      fail(ILC_EINVAL, "backward is implemented for the quartic projected kernels");
     ILC_REQUIRE(dl_dvalue_dev && dl_dw_dev && dl_dh_dev && dl_dx_dev && dl_dy_dev,
                 "null gradient pointer");
     device_guard dg(h->device);
@@ -247,8 +246,15 @@ ilc_status ilc_psg_backward(ilc_psg* h, const float* dl_dvalue_dev, float scale,
     float* gs = h->grad_sorted.p;
     launch_fill_u32(reinterpret_cast<uint32_t*>(gs), 0u, 4 * n, s);
     point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
-    launch_adjoint(h->grid.view(), p, dl_dvalue_dev, scale, kernel_norm(h->kernel), gs, gs + n,
-                   gs + 2 * n, gs + 3 * n, s);
+    ensure_cubic(h);
+    if (h->kernel == ILC_KERNEL_CUBIC && !h->dcubic_ready) {
+      const std::vector<float> t = cubic_projected_dtable(4096);
+      h->dcubic.ensure(t.size());
+      ILC_HIP(hipMemcpy(h->dcubic.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+      h->dcubic_ready = true;
+    }
+    launch_adjoint(h->grid.view(), p, h->kernel, dl_dvalue_dev, scale, kernel_norm(h->kernel),
+                   h->cubic.p, h->dcubic.p, gs, gs + n, gs + 2 * n, gs + 3 * n, s);
     launch_unpermute4(h->grid.perm.p, static_cast<uint32_t>(n), gs, gs + n, gs + 2 * n, gs + 3 * n,
                       dl_dw_dev, dl_dh_dev, dl_dx_dev, dl_dy_dev, s);
   });

@@ -136,7 +136,9 @@ __device__ inline float cubic_shape(float u, const float* tab) {
 struct adjoint_io {
   const float* grad;  // dL/d(point value), one per point
   float scale;        // points' value = scale * accumulator
-  float norm;         // kernel normalisation K (W = K s^2.5 / h^2)
+  float norm;         // kernel normalisation K (W = K shape / h^2)
+  const float* ftab;  // cubic: F(u) and dF/du on u = r^2/h^2 in [0, 4] (kCubicTable nodes)
+  const float* dtab;
   float* gw;          // per sorted particle: dL/dw, dL/dh, dL/dx, dL/dy (atomic adds)
   float* gh;
   float* gx;
@@ -153,6 +155,7 @@ struct gather_smem {
   float redf[4][4];
   int redi[4][10];
   float tab[KERNEL == 2 ? kCubicTable : 1];
+  float dtab[KERNEL == 2 && ADJ ? kCubicTable : 1];
   uint32_t cell_b[kBlock];
   uint32_t cell_off[kBlock];
   int cell_info[kBlock];  // cells: packed cell (cx | cy << 16) << 1 | full; rows: base key
@@ -532,7 +535,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
           const float4 bx = sm.Sb[sb];
           const float bdx = fmaxf(fmaxf(bx.x - q.x, q.x - bx.z), 0.0f);
           const float bdy = fmaxf(fmaxf(bx.y - q.y, q.y - bx.w), 0.0f);
-          if (!(act && fmaf(bdy, bdy, bdx * bdx) * q.z < 1.0f)) continue;
+          if (!(act && fmaf(bdy, bdy, bdx * bdx) * q.z < cull2)) continue;
           const int base = (sb >> 2) * 8 * TW + (sb & 3) * 8;
           for (int rr = 0; rr < 8; ++rr) {
 #pragma unroll
@@ -540,23 +543,50 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
               const float4 pt = sm.Pt[base + rr * TW + cc];
               const float dx = q.x - pt.x, dy = q.y - pt.y;
               const float r2 = fmaf(dy, dy, dx * dx);
-              const float sv = fmaxf(fmaf(-r2, q.z, 1.0f), 0.0f);
-              const float gq = pt.z * (sv * __builtin_amdgcn_sqrtf(sv));
-              A0 = fmaf(gq, sv, A0);
-              A1 += gq;
-              A2 = fmaf(gq, r2, A2);
-              A3 = fmaf(gq, dx, A3);
-              A4 = fmaf(gq, dy, A4);
+              if constexpr (KERNEL == 2) {
+                // F and dF/du at u = r^2/h^2, linear in the tables (0 beyond u = 4)
+                const float uu = r2 * q.z;
+                const float xx =
+                    fminf(uu * (float(kCubicTable - 1) / 4.0f), float(kCubicTable - 1));
+                const int ti = min(int(xx), kCubicTable - 2);
+                const float tf = xx - float(ti);
+                const float F = fmaf(tf, sm.tab[ti + 1] - sm.tab[ti], sm.tab[ti]);
+                const float D = fmaf(tf, sm.dtab[ti + 1] - sm.dtab[ti], sm.dtab[ti]);
+                const float gd = pt.z * D;
+                A0 = fmaf(pt.z, F, A0);
+                A1 = fmaf(gd, uu, A1);
+                A3 = fmaf(gd, dx, A3);
+                A4 = fmaf(gd, dy, A4);
+              } else {
+                const float sv = fmaxf(fmaf(-r2, q.z, 1.0f), 0.0f);
+                const float gq = pt.z * (sv * __builtin_amdgcn_sqrtf(sv));
+                A0 = fmaf(gq, sv, A0);
+                A1 += gq;
+                A2 = fmaf(gq, r2, A2);
+                A3 = fmaf(gq, dx, A3);
+                A4 = fmaf(gq, dy, A4);
+              }
             }
           }
         }
         if (act) {
           const uint32_t si = sm.Lidx[j];
-          const float c5 = -5.0f * q.w * q.z;  // -5 w K / h^4
-          atomicAdd(adj->gw + si, adj->norm * q.z * A0);
-          atomicAdd(adj->gh + si, q.w * __builtin_amdgcn_sqrtf(q.z) * fmaf(7.0f * q.z, A2, -2.0f * A1));
-          atomicAdd(adj->gx + si, c5 * A3);
-          atomicAdd(adj->gy + si, c5 * A4);
+          if constexpr (KERNEL == 2) {
+            // value = w K F(u) / h^2: d/dh = -2 w K / h^3 (F + u F'),
+            // d/dx = 2 w K / h^4 F' (x - x_point)
+            const float c2 = 2.0f * q.w * q.z;
+            atomicAdd(adj->gw + si, adj->norm * q.z * A0);
+            atomicAdd(adj->gh + si, -2.0f * q.w * __builtin_amdgcn_sqrtf(q.z) * (A0 + A1));
+            atomicAdd(adj->gx + si, c2 * A3);
+            atomicAdd(adj->gy + si, c2 * A4);
+          } else {
+            const float c5 = -5.0f * q.w * q.z;  // -5 w K / h^4
+            atomicAdd(adj->gw + si, adj->norm * q.z * A0);
+            atomicAdd(adj->gh + si,
+                      q.w * __builtin_amdgcn_sqrtf(q.z) * fmaf(7.0f * q.z, A2, -2.0f * A1));
+            atomicAdd(adj->gx + si, c5 * A3);
+            atomicAdd(adj->gy + si, c5 * A4);
+          }
         }
       }
     }
@@ -818,16 +848,23 @@ __global__ __launch_bounds__(kBlock) void k_gather_batch(const view_desc* __rest
   }
 }
 
-// Adjoint of the exact-mode quartic gather over one point image.
-template <int TW, int PPT>
+// Adjoint of the exact-mode gather over one point image (quartic or the
+// tabulated cubic spline).
+template <int TW, int PPT, int KERNEL>
 __global__ __launch_bounds__(kBlock) void k_adjoint(grid_view g, point_view p, uint32_t tiles_x,
                                                     uint32_t ntiles, adjoint_io io) {
-  __shared__ gather_smem<false, 0, true> sm;
+  __shared__ gather_smem<false, KERNEL, true> sm;
+  if constexpr (KERNEL == 2) {
+    for (int i = threadIdx.x; i < kCubicTable; i += kBlock) {
+      sm.tab[i] = io.ftab[i];
+      sm.dtab[i] = io.dtab[i];
+    }
+  }
   const uint32_t bid = blockIdx.x;
   const uint32_t per = ntiles / 8u;
   const uint32_t tile = bid < per * 8u ? (bid % 8u) * per + bid / 8u : bid;
-  gather_tile<TW, PPT, OP_ADJOINT, 0>(g, p, tile % tiles_x, tile / tiles_x, nullptr, nullptr,
-                                      1.0f, sm, &io);
+  gather_tile<TW, PPT, OP_ADJOINT, KERNEL>(g, p, tile % tiles_x, tile / tiles_x, nullptr, nullptr,
+                                           KERNEL == 2 ? 2.0f : 1.0f, sm, &io);
 }
 
 // gradients in sorted order -> input order
@@ -910,15 +947,22 @@ int dispatch(const tile_shape& ts, int op, int kernel, A&&... a) {
 
 }  // namespace
 
-void launch_adjoint(const grid_view& g, const point_view& p, const float* grad, float scale,
-                    float norm, float* gw, float* gh, float* gx, float* gy, hipStream_t s) {
+void launch_adjoint(const grid_view& g, const point_view& p, int kernel, const float* grad,
+                    float scale, float norm, const float* ftab, const float* dtab, float* gw,
+                    float* gh, float* gx, float* gy, hipStream_t s) {
   constexpr int TW = 32, PPT = 4, TH = (kBlock / TW) * PPT;
   const uint32_t tiles_x = (p.lnx + TW - 1) / TW;
   const uint32_t ntiles = tiles_x * ((p.lny + TH - 1) / TH);
   if (!ntiles) return;
-  adjoint_io io{grad, scale, norm, gw, gh, gx, gy};
-  hipLaunchKernelGGL((k_adjoint<TW, PPT>), dim3(ntiles), dim3(kBlock), 0, s, g, p, tiles_x,
-                     ntiles, io);
+  adjoint_io io{grad, scale, norm, ftab, dtab, gw, gh, gx, gy};
+  if (kernel == ILC_KERNEL_CUBIC) {
+    if (!ftab || !dtab) fail(ILC_EINVAL, "cubic adjoint without its tables");
+    hipLaunchKernelGGL((k_adjoint<TW, PPT, 2>), dim3(ntiles), dim3(kBlock), 0, s, g, p, tiles_x,
+                       ntiles, io);
+  } else {
+    hipLaunchKernelGGL((k_adjoint<TW, PPT, 0>), dim3(ntiles), dim3(kBlock), 0, s, g, p, tiles_x,
+                       ntiles, io);
+  }
   ILC_HIP(hipGetLastError());
 }
 

@@ -188,7 +188,53 @@ double column_piece(double b2, double a, double b) {
   return s * hw;
 }
 
+// d/du of m4(sqrt(u + z^2)) = m4'(rho) / (2 rho), rho = sqrt(u + z^2)
+double m4_du(double rho) {
+  if (rho >= 2.0) return 0.0;
+  if (rho <= 1.0) return (-1.5 + 1.125 * rho) / M_PI;
+  const double t = 2.0 - rho;
+  return -0.375 * t * t / (M_PI * rho);
+}
+
+double column_piece_du(double b2, double a, double b) {
+  static const double x[5] = {0.1488743389816312, 0.4333953941292472, 0.6794095682990244,
+                              0.8650633666889845, 0.9739065285171717};
+  static const double w[5] = {0.2955242247147529, 0.2692667193099963, 0.2190863625159820,
+                              0.1494513491505806, 0.0666713443086881};
+  const double c = 0.5 * (a + b), hw = 0.5 * (b - a);
+  double s = 0.0;
+  for (int k = 0; k < 5; ++k) {
+    const double z0 = c - hw * x[k], z1 = c + hw * x[k];
+    s += w[k] * (m4_du(std::sqrt(b2 + z0 * z0)) + m4_du(std::sqrt(b2 + z1 * z1)));
+  }
+  return s * hw;
+}
+
 }  // namespace
+
+// dF/du of the projected M4 column (for the adjoint), tabulated like F:
+// d/du int m4(sqrt(u + z^2)) dz = int m4'(rho) / (2 rho) dz (the column's end
+// moves with u but the integrand vanishes there).
+std::vector<float> cubic_projected_dtable(int n) {
+  std::vector<float> t(n);
+  for (int i = 0; i < n; ++i) {
+    const double u = 4.0 * double(i) / double(n - 1);
+    if (u >= 4.0) {
+      t[i] = 0.0f;
+      continue;
+    }
+    const double z2 = std::sqrt(4.0 - u);
+    const double z1 = u < 1.0 ? std::sqrt(1.0 - u) : 0.0;
+    double s = 0.0;
+    const int m = 8;
+    for (int k = 0; k < m; ++k) {
+      if (z1 > 0.0) s += column_piece_du(u, z1 * k / m, z1 * (k + 1) / m);
+      s += column_piece_du(u, z1 + (z2 - z1) * k / m, z1 + (z2 - z1) * (k + 1) / m);
+    }
+    t[i] = static_cast<float>(2.0 * s);
+  }
+  return t;
+}
 
 // Projected M4 column density F(u) = int m4(sqrt(u + z^2)) dz for h = 1,
 // tabulated on n nodes u_i = 4 i / (n - 1).  The integrand is a polynomial

@@ -123,11 +123,13 @@ void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,
                    const tile_shape& ts, double* acc, unsigned long long* counts,
                    const float* cubic_table, hipStream_t s);
 
-// Adjoint of the exact-mode quartic gather: adds into per-sorted-particle
-// gradient arrays (dL/dw, dL/dh, dL/dx, dL/dy) for dL/d(point value) = grad,
-// point value = scale * accumulator, W = norm * s^2.5 / h^2.
-void launch_adjoint(const grid_view& g, const point_view& p, const float* grad, float scale,
-                    float norm, float* gw, float* gh, float* gx, float* gy, hipStream_t s);
+// Adjoint of the exact-mode gather: adds into per-sorted-particle gradient
+// arrays (dL/dw, dL/dh, dL/dx, dL/dy) for dL/d(point value) = grad, point
+// value = scale * accumulator, W = norm * shape / h^2 (quartic: s^2.5; cubic:
+// the tabulated F with its tabulated derivative dtab).
+void launch_adjoint(const grid_view& g, const point_view& p, int kernel, const float* grad,
+                    float scale, float norm, const float* ftab, const float* dtab, float* gw,
+                    float* gh, float* gx, float* gy, hipStream_t s);
 void launch_unpermute4(const uint32_t* perm, uint32_t n, const float* a, const float* b,
                        const float* c, const float* d, float* oa, float* ob, float* oc, float* od,
                        hipStream_t s);

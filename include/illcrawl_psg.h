@@ -159,9 +159,10 @@ ilc_status ilc_psg_retrieve_results(ilc_psg* h, float scale, float* out_dev, voi
  * per particle, in input order (n floats each, device):
  *   dl_dw = dL/dw, dl_dh = dL/dh, dl_dx = dL/dx, dl_dy = dL/dy
  * (x, y in the projected frame).  Weights are recomputed, not stored.
- * Exact mode and the quartic kernels only (ILC_EINVAL otherwise): the
- * reference-mode search box makes the image a non-smooth function of the
- * particles.  Gradients are FP32 sums whose order varies from call to call
+ * Exact mode only (ILC_EINVAL otherwise): the reference-mode search box
+ * makes the image a non-smooth function of the particles.  For the cubic
+ * kernel, value[i] = scale * sum_j w_j F(r^2/h^2) / h^2 with F and dF/du read
+ * from 4096-node tables, as the forward reads F.  Gradients are FP32 sums whose order varies from call to call
  * (atomic adds), so repeated calls agree to rounding, not bit for bit. */
 ilc_status ilc_psg_backward(ilc_psg* h, const float* dl_dvalue_dev, float scale, float* dl_dw_dev,
                             float* dl_dh_dev, float* dl_dx_dev, float* dl_dy_dev, void* stream);

@@ -49,7 +49,8 @@ PINNING = {
     "orc_tabulated_2d": "hand-computed bilinear half-texel answers",
     "orc_temperature": "closed form (gamma-1) u mu 121.1475502",
     "orc_xray_bolometric": "closed form on table-node-free inputs",
-    "orc_backward_exact": "FP64 central differences of orc_gather_exact in w, h, x, y",
+    "orc_backward_exact": "FP64 central differences of orc_gather_exact in w, h, x, y "
+                          "(quartic_ref, quartic, cubic)",
     "orc_xray_flux": "constant Gaunt table: band integral of exp(-E/kT) in closed form "
                      "(RKF within its tolerance); a table linear in its indices recovers "
                      "the sampling coordinates ln(gamma^2), ln(u) exactly",

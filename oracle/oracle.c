@@ -512,6 +512,42 @@ double orc_cubic_projected_unit(double b)
   return 2.0 * s;
 }
 
+/* d/du of the projected cubic at u = b^2 (h = 1): the integral over z of
+ * d/du M4(sqrt(u + z^2)) = M4'(rho) / (2 rho); the column end moves with u
+ * but the integrand vanishes there. */
+static double d_cubic3d_du(double rho)
+{
+  if (rho >= 2.0) return 0.0;
+  if (rho <= 1.0) return (-1.5 + 1.125 * rho) / M_PI;
+  double t = 2.0 - rho;
+  return -0.375 * t * t / (M_PI * rho);
+}
+
+static double gl_integrate_cubic_du(double b, double z0, double z1)
+{
+  double c = 0.5 * (z0 + z1), hw = 0.5 * (z1 - z0), s = 0.0;
+  for (int k = 0; k < 8; ++k) {
+    double za = c - hw * gl_x[k], zb = c + hw * gl_x[k];
+    s += gl_w[k] * (d_cubic3d_du(sqrt(b * b + za * za)) + d_cubic3d_du(sqrt(b * b + zb * zb)));
+  }
+  return s * hw;
+}
+
+double orc_cubic_projected_du(double b)
+{
+  if (b >= 2.0) return 0.0;
+  double z2 = sqrt(4.0 - b * b);
+  double z1 = b < 1.0 ? sqrt(1.0 - b * b) : 0.0;
+  double s = 0.0;
+  const int m = 16;
+  if (z1 > 0.0)
+    for (int k = 0; k < m; ++k)
+      s += gl_integrate_cubic_du(b, z1 * k / m, z1 * (k + 1) / m);
+  for (int k = 0; k < m; ++k)
+    s += gl_integrate_cubic_du(b, z1 + (z2 - z1) * k / m, z1 + (z2 - z1) * (k + 1) / m);
+  return 2.0 * s;
+}
+
 static double d_kernel2d(int kind, double r2, double h)
 {
   double h2 = h * h;
@@ -745,6 +781,26 @@ void orc_backward_exact(const float* p4, const float* h, uint64_t n, const float
     const double px = p4[4 * j], py = p4[4 * j + 1], hh = h[j];
     const double h2 = hh * hh;
     double sw = 0, sh = 0, sx = 0, sy = 0;
+    if (kind == ORC_K_CUBIC) {
+      /* value = w F(u) / h^2, u = r^2 / h^2, support 2h */
+      for (uint64_t t = 0; t < npts; ++t) {
+        const double dx = px - (double)pts4[4 * t], dy = py - (double)pts4[4 * t + 1];
+        const double r2 = dx * dx + dy * dy;
+        if (!(r2 < 4.0 * h2)) continue;
+        const double b = sqrt(r2) / hh, u = r2 / h2;
+        const double F = orc_cubic_projected_unit(b), D = orc_cubic_projected_du(b);
+        const double g = G[t];
+        sw += g * F / h2;
+        sh += g * (-2.0) * (F + u * D) / (h2 * hh);
+        sx += g * 2.0 * D * dx / (h2 * h2);
+        sy += g * 2.0 * D * dy / (h2 * h2);
+      }
+      gw[j] = sw;
+      gh[j] = w * sh;
+      gx[j] = w * sx;
+      gy[j] = w * sy;
+      continue;
+    }
     for (uint64_t t = 0; t < npts; ++t) {
       const double dx = px - (double)pts4[4 * t], dy = py - (double)pts4[4 * t + 1];
       const double r2 = dx * dx + dy * dy;

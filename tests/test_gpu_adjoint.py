@@ -59,7 +59,8 @@ def projected_scene(n, seed, nx, ny, width=1000.0):
     return p4, h.astype(np.float32), pts
 
 
-@pytest.mark.parametrize("kernel,kind,seed", [("quartic", 1, 0), ("quartic_ref", 0, 1)])
+@pytest.mark.parametrize("kernel,kind,seed", [("quartic", 1, 0), ("quartic_ref", 0, 1),
+                                              ("cubic", 2, 2)])
 def test_backward_vs_fp64_oracle(B, orc, kernel, kind, seed):
     nx, ny = 96, 80
     say(f"case: backward vs oracle kernel={kernel} seed={seed} n=3000 points={nx}x{ny}")
@@ -75,10 +76,13 @@ def test_backward_vs_fp64_oracle(B, orc, kernel, kind, seed):
     torch.cuda.synchronize()
     ref = orc.backward_exact(p4, h, pts, G.astype(np.float64) * np.float64(np.float32(scale)),
                              kind)
+    # the cubic's shape and derivative are tables (4096 nodes, linear): its
+    # gradients carry their interpolation error, like its forward (2e-5)
+    tol = 3e-4 if kernel == "cubic" else GRAD_TOL
     for name, got, r in zip(("w", "h", "x", "y"), (gw, gh, gx, gy), ref):
         e = rel_to_max(got, r)
         say(f"  dL/d{name}: rel-to-max err {e:.2e}")
-        assert e < GRAD_TOL, name
+        assert e < tol, name
     # w = 0 particles still get dL/dw, and nothing else
     z = p4[:, 3] == 0
     assert np.all(gh[z] == 0) and np.all(gx[z] == 0) and np.any(gw[z] != 0)
@@ -110,7 +114,7 @@ def test_backward_errors(B):
     from illcrawl_amd import IllcrawlError
     p4, h, pts = projected_scene(100, 2, 32, 32)
     G = torch.zeros(32 * 32, device="cuda")
-    for mode, kernel in (("reference", "quartic_ref"), ("exact", "cubic")):
+    for mode, kernel in (("reference", "quartic_ref"),):
         b = B.ProjectiveSmoothingGrid(mode=mode, kernel=kernel)
         b.init_backend(p4.shape[0])
         b.setup_projected_particles(p4, h)
@@ -181,7 +185,12 @@ def test_autograd_vs_fp64_central_differences(B, orc):
                         scale)
     ref_one = orc.gather_exact(p4[j0:j0 + 1], hs[j0:j0 + 1], pts, kind=1) * scale
     assert np.allclose(one, ref_one, rtol=1e-12, atol=1e-300)
-    R = I64 - target.cpu().numpy().ravel().astype(np.float64)
+    # the residual the backward receives (the GPU image minus the target);
+    # the central differences then test dI/dtheta alone
+    R = img.detach().cpu().numpy().ravel().astype(np.float64) - \
+        target.cpu().numpy().ravel().astype(np.float64)
+    assert np.allclose(img.detach().cpu().numpy().ravel(), I64, rtol=1e-5,
+                       atol=1e-6 * float(np.abs(I64).max()))
 
     # 256 particles whose support reaches the image
     reach = np.array([np.any(_single_image(p4[j, :2].astype(np.float64), 1.0, float(hs[j]), px,

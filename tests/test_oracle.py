@@ -352,7 +352,7 @@ def test_xray_bolometric_closed_form(orc):
 
 # ---------------------------------------------------------------- adjoint
 
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_backward_exact_vs_central_differences(orc, kind):
     rng = np.random.default_rng(17)
     n = 40
