@@ -257,6 +257,64 @@ def adjoint_main(args):
     print(json.dumps(line), flush=True)
 
 
+def batch_adjoint_main(args):
+    """c5 --adjoint: forward and adjoint of L = 1/2 sum over the 32 slabs of
+    |I_v(w) - I_v(w')|^2, w' = w * lognormal(0.05) (densities scaled by the
+    square root: the emission goes as rho^2), exact mode; one batched render
+    and one batched backward (gradients w.r.t. w, h and positions)."""
+    import numpy as np
+    import torch
+    from illcrawl_amd import backend as B, synth
+    t0 = time.time()
+    snap, views, desc = workload(args.workload)
+    log(f"[bench] {args.workload} adjoint: {snap.n} cells, {len(views)} views "
+        f"({time.time() - t0:.1f}s)")
+    tab, x0, dx = synth.gaunt_table_1d()
+    rng = np.random.default_rng(7)
+    rho_t = (snap.rho * np.sqrt(rng.lognormal(0.0, 0.05, snap.n))).astype(np.float32)
+    r = B.BatchRenderer(mode="exact")
+    r.set_gaunt_table(tab, x0, dx)
+    if args.tile:
+        r.set_tile(*(int(x) for x in args.tile.split("x")))
+    r.set_particles(snap.pos, snap.mass, snap.hsml, rho_t, snap.temp, snap.xe)
+    r.set_views(views)
+    r.render()
+    targets = [r.image(k).clone() for k in range(len(views))]
+    r.set_particles(snap.pos, snap.mass, snap.hsml, snap.rho, snap.temp, snap.xe)
+    r.render()
+    grads = [(r.image(k) - targets[k]).contiguous() for k in range(len(views))]
+    total = int(sum(r.count_evals()))
+    for _ in range(args.warmup):
+        r.render()
+        r.backward(grads)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.render()
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gw, gh, gp = r.backward(grads)
+    torch.cuda.synchronize()
+    bwd_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"[bench] {args.workload} forward {fwd_ms:.2f} ms, adjoint {bwd_ms:.2f} ms")
+    line = {
+        "metric": "adjoint vs forward cost, batched slab projection (exact quartic_ref)",
+        "value": total / (fwd_ms * 1e-3), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": fwd_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 per-pixel accumulation)",
+        "data": "synthetic (seeded: uniform + NFW halos, 32-NN density, lognormal masses)",
+        "config": dict(desc, mode="exact", evals_per_step=total, parallelism="single GPU"),
+        "adjoint": dict(ms=bwd_ms, forward_ms=fwd_ms, ratio=bwd_ms / fwd_ms,
+                        nonzero_dw=int((gw != 0).sum().item()),
+                        finite=bool(torch.isfinite(gw).all() and torch.isfinite(gh).all()
+                                    and torch.isfinite(gp).all())),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +322,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=os.environ.get("ILC_BENCH_WORKLOAD", "c3"))
     ap.add_argument("--mode", default="reference", choices=["reference", "exact"])
+    ap.add_argument("--adjoint", action="store_true",
+                    help="batched forward + adjoint timing of the workload (exact mode)")
     ap.add_argument("--kernel", default="quartic", choices=["quartic", "quartic_ref", "cubic"],
                     help="projected kernel of the adjoint workload (c4)")
     ap.add_argument("--tile", default=None, help="WxH gather tile; default: autotuned")
@@ -288,6 +348,8 @@ def main():
         raise SystemExit("bench.py needs a GPU")
     if args.workload == "c4":
         return adjoint_main(args)
+    if args.adjoint:
+        return batch_adjoint_main(args)
 
     t0 = time.time()
     snap, views, desc = workload(args.workload)

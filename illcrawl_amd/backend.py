@@ -409,6 +409,7 @@ class BatchRenderer:
         arrs = [pos] + [(_dev_f32(a, d).reshape(-1) if a is not None else None)
                         for a in (mass, hsml, rho, temp, xe)]
         self._keep = arrs
+        self._n = pos.shape[0]
         N.call("ilc_batch_set_particles", self._h, *[_ptr(a) for a in arrs], pos.shape[0])
 
     def set_gaunt_table(self, table, x0, dx):
@@ -448,6 +449,25 @@ class BatchRenderer:
         arr = (ctypes.c_uint64 * len(self.views))()
         N.call("ilc_batch_count_evals", self._h, arr, _stream(stream))
         return [int(x) for x in arr]
+
+    def backward(self, dl_dimages, stream=None):
+        """Adjoint of the last render (exact mode): dl_dimages[v] = dL/d(image
+        of view v).  Returns (dL/dw, dL/dh, dL/dpos (n, 3)) per snapshot
+        particle, summed over views."""
+        if len(dl_dimages) != len(self.views):
+            raise ValueError("one gradient image per view")
+        dev = self.device
+        gs = [_dev_f32(g, dev).reshape(-1) for g in dl_dimages]
+        for g, v in zip(gs, self.views):
+            if g.shape[0] != int(v["nx"]) * int(v.get("ny", v["nx"])):
+                raise ValueError("a gradient image has the wrong size")
+        ptrs = (ctypes.c_void_p * len(gs))(*[g.data_ptr() for g in gs])
+        gw = torch.empty(self._n, dtype=torch.float32, device=dev)
+        gh = torch.empty_like(gw)
+        gp = torch.empty((self._n, 3), dtype=torch.float32, device=dev)
+        N.call("ilc_batch_backward", self._h, ctypes.cast(ptrs, ctypes.c_void_p), _ptr(gw),
+               _ptr(gh), _ptr(gp), _stream(stream))
+        return gw, gh, gp
 
     def view_info(self, view):
         """Grid of one view after a render (num_particles: particles it holds)."""

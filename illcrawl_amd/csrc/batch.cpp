@@ -33,6 +33,7 @@
 
 namespace ilc {
 std::vector<float> cubic_projected_table(int n);
+std::vector<float> cubic_projected_dtable(int n);
 }
 
 using namespace ilc;
@@ -70,6 +71,7 @@ struct view_state {
   int group = -1;         // slab group (views with identical projected x', y')
   window_f win{};
   uint32_t nwin = 0;      // particles kept by the filter (from the plan)
+  dbuf<uint32_t> orig;    // filtered grids: snapshot index of each grid input row
 };
 
 template <class T>
@@ -104,7 +106,8 @@ struct ilc_batch {
   dbuf<float> gaunt;
   int gaunt_n = 0;
   float gx0 = 0.f, gdx = 1.f;
-  dbuf<float> cubic;
+  dbuf<float> cubic, dcubic;
+  bool dcubic_ready = false;
   bool cubic_ready = false;
 
   std::vector<std::unique_ptr<view_state>> views;
@@ -434,8 +437,9 @@ void build_view(ilc_batch* b, view_state& vs, const build_ctx& c) {
     c.sc->temp.ensure(scan_temp_words(uint64_t(nt) + 1));
     launch_window_count(b->ps, proj_of(b, vs), vs.win, c.wcounts->p, nullptr, st);
     launch_exclusive_scan(c.wcounts->p, c.woffs->p, nt, c.sc->temp.p, st);
+    vs.orig.ensure(std::max<uint64_t>(m, 1));
     launch_window_scatter(b->ps, proj_of(b, vs), gaunt_of(b), vs.win, c.woffs->p, vs.gp,
-                          c.p4tmp->p, c.htmp->p, vs.grid.key.p, vs.grid.perm.p,
+                          c.p4tmp->p, c.htmp->p, vs.grid.key.p, vs.grid.perm.p, vs.orig.p,
                           b->counter.p + 1, st);
     enqueue_grid_from_keys(vs.grid, *c.sc, c.p4tmp->p, c.htmp->p, m, vs.gp, vs.kz0, vs.max_h,
                            b->kernel, st);
@@ -756,6 +760,49 @@ ilc_status ilc_batch_count_evals(ilc_batch* b, uint64_t* per_view_host, void* st
       launch_gather(grid_of(b, vs), p, OP_COUNT, b->kernel, b->ts, nullptr, cnt.p, nullptr, s);
       per_view_host[v] = launch_sum_u64(cnt.p, P, tmp.p, s);
     }
+  });
+}
+
+ilc_status ilc_batch_backward(ilc_batch* b, const float* const* dl_dimage_dev, float* dl_dw_dev,
+                              float* dl_dh_dev, float* dl_dpos3_dev, void* stream) {
+  return guarded([&] {
+    require_ready(b);
+    ILC_REQUIRE(dl_dimage_dev && dl_dw_dev && dl_dh_dev && dl_dpos3_dev, "null argument");
+    if (b->mode != ILC_MODE_EXACT)
+      fail(ILC_EINVAL, "backward needs exact mode (the reference-mode search box is not smooth)");
+    if (!b->planned) fail(ILC_ENOTREADY, "render before backward");
+    device_guard dg(b->device);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const uint64_t n = b->ps.n;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(dl_dw_dev), 0u, n, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(dl_dh_dev), 0u, n, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(dl_dpos3_dev), 0u, 3 * n, s);
+    ensure_cubic(b);
+    if (b->kernel == ILC_KERNEL_CUBIC && !b->dcubic_ready) {
+      const std::vector<float> t = cubic_projected_dtable(4096);
+      b->dcubic.ensure(t.size());
+      ILC_HIP(hipMemcpy(b->dcubic.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+      b->dcubic_ready = true;
+    }
+    dbuf<float> g4;
+    for (size_t v = 0; v < b->views.size(); ++v) {
+      view_state& vs = *b->views[v];
+      ILC_REQUIRE(dl_dimage_dev[v], "null gradient image for view %zu", v);
+      if (!vs.grid.ready) fail(ILC_ENOTREADY, "view %zu has no grid (render first)", v);
+      const uint64_t m = vs.grid.n;
+      if (!m) continue;
+      g4.ensure(4 * m);
+      launch_fill_u32(reinterpret_cast<uint32_t*>(g4.p), 0u, 4 * m, s);
+      const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
+      const point_view p{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
+      launch_adjoint(grid_of(b, vs), p, b->kernel, dl_dimage_dev[v], vs.v.scale,
+                     kernel_norm(b->kernel), b->cubic.p, b->dcubic.p, g4.p, g4.p + m, g4.p + 2 * m,
+                     g4.p + 3 * m, s);
+      launch_scatter_grads(vs.grid.perm.p, (vs.windowed || vs.compact) ? vs.orig.p : nullptr,
+                           static_cast<uint32_t>(m), g4.p, g4.p + m, g4.p + 2 * m, g4.p + 3 * m,
+                           b->ps.pos3, proj_of(b, vs), dl_dw_dev, dl_dh_dev, dl_dpos3_dev, s);
+    }
+    ILC_HIP(hipStreamSynchronize(s));
   });
 }
 

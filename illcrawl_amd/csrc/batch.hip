@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_window_scatter(
     particle_soa ps, view_proj vp, gaunt_table gt, window_f win,
     const uint32_t* __restrict__ offsets, grid_params g, float4* __restrict__ p4c,
     float* __restrict__ hc, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-    uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ orig, uint32_t* __restrict__ err) {
 #pragma clang fp contract(off)
   __shared__ float tab[kMaxGaunt];
   __shared__ uint32_t wc[2][4];
@@ -280,12 +280,51 @@ __global__ __launch_bounds__(kBlock) void k_window_scatter(
         atomicAdd(err, 1u);
       keys[j] = key;
       vals[j] = j;
+      orig[j] = uint32_t(i);
     }
     out += tot;
   }
 }
 
+// Adjoint of one view of a batch: gradients of the view's grid (sorted
+// order, projected frame) added to the snapshot's per-particle gradients:
+// dL/dw (0 for particles the view's depth cut removes, whose w does not
+// enter the image), dL/dh, and dL/dpos = J^T (dL/dx', dL/dy') with J the
+// first two rows of the view's projection.
+__global__ __launch_bounds__(kBlock) void k_scatter_grads(
+    const uint32_t* __restrict__ perm, const uint32_t* __restrict__ orig, uint32_t n,
+    const float* __restrict__ gw, const float* __restrict__ gh, const float* __restrict__ gx,
+    const float* __restrict__ gy, const float* __restrict__ pos3, view_proj vp,
+    float* __restrict__ out_w, float* __restrict__ out_h, float* __restrict__ out_pos3) {
+#pragma clang fp contract(off)
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t c = perm[s];
+  const uint32_t i = orig ? orig[c] : c;
+  const float x = pos3[3 * uint64_t(i)], y = pos3[3 * uint64_t(i) + 1],
+              z = pos3[3 * uint64_t(i) + 2];
+  const float pz = dot3(vp.m + 6, x - vp.c[0], y - vp.c[1], z - vp.c[2]);
+  const bool kept = !(pz < 0.0f || pz > vp.depth);
+  if (kept) atomicAdd(out_w + i, gw[s]);
+  atomicAdd(out_h + i, gh[s]);
+  const float ax = gx[s], ay = gy[s];
+  for (int k = 0; k < 3; ++k) {
+    const float v = fmaf(ax, vp.m[k], ay * vp.m[3 + k]);
+    if (v != 0.0f) atomicAdd(out_pos3 + 3 * uint64_t(i) + k, v);
+  }
+}
+
 }  // namespace
+
+void launch_scatter_grads(const uint32_t* perm, const uint32_t* orig, uint32_t n,
+                          const float* gw, const float* gh, const float* gx, const float* gy,
+                          const float* pos3, const view_proj& vp, float* out_w, float* out_h,
+                          float* out_pos3, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_scatter_grads, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, perm,
+                     orig, n, gw, gh, gx, gy, pos3, vp, out_w, out_h, out_pos3);
+  ILC_HIP(hipGetLastError());
+}
 
 uint32_t window_tiles(uint64_t n) { return static_cast<uint32_t>((n + kWinTile - 1) / kWinTile); }
 
@@ -300,14 +339,14 @@ void launch_window_count(const particle_soa& ps, const view_proj& vp, const wind
 
 void launch_window_scatter(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
                            const window_f& win, const uint32_t* offsets, const grid_params& g,
-                           float4* p4c, float* hc, uint32_t* keys, uint32_t* vals, uint32_t* err,
-                           hipStream_t s) {
+                           float4* p4c, float* hc, uint32_t* keys, uint32_t* vals, uint32_t* orig,
+                           uint32_t* err, hipStream_t s) {
   if (vp.quantity == 1 && (gt.n < 1 || gt.n > kMaxGaunt || !gt.y))
     fail(ILC_EINVAL, "X-ray view without a Gaunt table (1..%d entries)", kMaxGaunt);
   const uint32_t nb = window_tiles(ps.n);
   if (!nb) return;
   hipLaunchKernelGGL(k_window_scatter, dim3(nb), dim3(kBlock), 0, s, ps, vp, gt, win, offsets, g,
-                     p4c, hc, keys, vals, err);
+                     p4c, hc, keys, vals, orig, err);
   ILC_HIP(hipGetLastError());
 }
 

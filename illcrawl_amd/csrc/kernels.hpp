@@ -79,8 +79,14 @@ void launch_window_count(const particle_soa& ps, const view_proj& vp, const wind
                          uint32_t* counts, uint32_t* total, hipStream_t s);
 void launch_window_scatter(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
                            const window_f& win, const uint32_t* offsets, const grid_params& g,
-                           float4* p4c, float* hc, uint32_t* keys, uint32_t* vals, uint32_t* err,
-                           hipStream_t s);
+                           float4* p4c, float* hc, uint32_t* keys, uint32_t* vals, uint32_t* orig,
+                           uint32_t* err, hipStream_t s);
+// one view's adjoint (sorted order, projected frame) into snapshot gradients;
+// orig (may be NULL): the snapshot index of each grid input row
+void launch_scatter_grads(const uint32_t* perm, const uint32_t* orig, uint32_t n,
+                          const float* gw, const float* gh, const float* gx, const float* gy,
+                          const float* pos3, const view_proj& vp, float* out_w, float* out_h,
+                          float* out_pos3, hipStream_t s);
 int view_bbox_parts(uint64_t n);
 void launch_view_bbox(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
                       float* partial, uint32_t* bad, bbox_result* out, hipStream_t s);

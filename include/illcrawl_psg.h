@@ -366,6 +366,19 @@ ilc_status ilc_batch_image_f64(const ilc_batch* b, int32_t view, double* out_dev
 ilc_status ilc_batch_view_info(const ilc_batch* b, int32_t view, ilc_grid_info* out);
 /* Exact particle-kernel evaluation count of each view (after a render). */
 ilc_status ilc_batch_count_evals(ilc_batch* b, uint64_t* per_view_host, void* stream);
+/* Adjoint of the last render (exact mode; the reference has no counterpart).
+ * dl_dimage_dev[v]: dL/d(image of view v) (nx*ny floats, device, as
+ * ilc_batch_image returns it, i.e. with the view's dA scale).  Writes per
+ * snapshot particle (n entries, device, overwritten):
+ *   dl_dw:    sum over views of dL/dw_v, w_v the particle's weight in view v
+ *             (its mass, or its X-ray emissivity), 0 from views whose depth
+ *             cut removes the particle;
+ *   dl_dh:    dL/dh summed over views;
+ *   dl_dpos3: dL/d(x, y, z) in world coordinates (3 n floats, xyz interleaved).
+ * Gradients are FP32 atomic sums (order varies from call to call).
+ * Synchronises the stream. */
+ilc_status ilc_batch_backward(ilc_batch* b, const float* const* dl_dimage_dev, float* dl_dw_dev,
+                              float* dl_dh_dev, float* dl_dpos3_dev, void* stream);
 /* Device time (ms, HIP events on the render stream) of the gather launch(es)
  * of the last non-graph render: the one persistent launch, or the sum of the
  * per-view launches of a plain render.  Measurement only. */
