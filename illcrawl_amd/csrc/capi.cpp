@@ -305,9 +305,17 @@ ilc_status ilc_psg_export_grid(const ilc_psg* h, int32_t* cells2_dev, float* max
     hipStream_t s = as_stream(stream);
     launch_export_cells(h->grid.start.p, h->grid.maxh_bits.p,
                         static_cast<uint32_t>(h->grid.gp.ncells()), cells2_dev, maxh_dev, s);
-    if (perm_dev)
-      ILC_HIP(hipMemcpyAsync(perm_dev, h->grid.perm.p, sizeof(uint32_t) * h->grid.n,
-                             hipMemcpyDeviceToDevice, s));
+    if (perm_dev && h->grid.n) {
+      // the gather keeps a cell's particles by descending h; the reference's
+      // order within a cell is the input order
+      const uint64_t n = h->grid.n;
+      dbuf<uint32_t> by_index, temp;
+      by_index.ensure(n);
+      temp.ensure(radix_temp_words(n));
+      launch_reference_perm(h->grid.key.p, h->grid.perm.p, static_cast<uint32_t>(n),
+                            key_bits_for(h->grid.gp.ncells() - 1), perm_dev, by_index.p, temp.p, s);
+      ILC_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+    }
   });
 }
 

@@ -14,7 +14,9 @@
 //      points' index boxes;
 //   3. visits the candidate cells 256 at a time, skips every cell whose max h
 //      cannot reach the tile, and streams the particles of the others with a
-//      block-wide offset table (coalesced 16-byte loads);
+//      block-wide offset table (coalesced 16-byte loads).  A cell holds its
+//      particles by descending h, so a binary search on h finds the prefix
+//      that can reach the tile and the rest is never read;
 //   4. culls each particle against the tile (support disk vs tile box) and
 //      compacts the survivors into LDS lists with wave ballots:
 //        big    - every point may use it and its support is at least twice
@@ -640,7 +642,22 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
         const float mgy = 1e-4f * g.cs[1] + 4e-7f * (fabsf(g.minc[1]) + fabsf(lo_y) + g.cs[1]);
         const float dx = fmaxf(fmaxf((lo_x - mgx) - tbx1, tbx0 - (lo_x + g.cs[0] + mgx)), 0.0f);
         const float dy = fmaxf(fmaxf((lo_y - mgy) - tby1, tby0 - (lo_y + g.cs[1] + mgy)), 0.0f);
-        if (dx * dx + dy * dy < mh * mh * sup_h2) cn = ce - cb;
+        const float d2c = dx * dx + dy * dy;
+        if (d2c < mh * mh * sup_h2) {
+          // the cell's particles come by descending h (to the top 16 bits of
+          // its pattern): only a prefix can reach the tile.  With h rounded
+          // up to its 16-bit bucket the test is monotone along the cell.
+          uint32_t lo = 0, hi = ce - cb;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            const float hu = __uint_as_float(__float_as_uint(g.hs[cb + mid]) | 0xffffu);
+            if (d2c < hu * hu * sup_h2)
+              lo = mid + 1;
+            else
+              hi = mid;
+          }
+          cn = lo;
+        }
         if constexpr (REF)
           cfull = (allz_u && ccy >= r_ily && ccy <= r_ihy && ccx >= r_ilx && ccx <= r_ihx) ? 1 : 0;
       }

@@ -346,6 +346,49 @@ __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__
   }
 }
 
+// Within-cell order for the gather: descending h, to the top 16 bits of its
+// float pattern (h > 0, so the pattern orders like the value).  A stable sort
+// of positions by this key, then the stable sort by cell key, leaves every
+// cell's particles biggest first (ties in input order); a tile then only
+// needs the prefix of a cell whose h can reach it.
+__global__ __launch_bounds__(kBlock) void k_h_order_keys(const float* __restrict__ h,
+                                                         const uint32_t* __restrict__ perm,
+                                                         uint32_t n, uint32_t* __restrict__ hk,
+                                                         uint32_t* __restrict__ hv) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  hk[j] = 0x7fffu - (__float_as_uint(h[perm[j]]) >> 16);
+  hv[j] = j;
+}
+
+// out_key[i] = key[order[i]], out_perm[i] = perm[order[i]]
+__global__ __launch_bounds__(kBlock) void k_permute_pairs(const uint32_t* __restrict__ order,
+                                                          const uint32_t* __restrict__ key,
+                                                          const uint32_t* __restrict__ perm,
+                                                          uint32_t n,
+                                                          uint32_t* __restrict__ out_key,
+                                                          uint32_t* __restrict__ out_perm) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = order[i];
+  out_key[i] = key[j];
+  out_perm[i] = perm[j];
+}
+
+// Cell key of every input index (by_index[perm[j]] = key[j]) and the
+// identity, the input of a stable sort by cell key that gives the
+// reference's order (stable by input index within a cell).
+__global__ __launch_bounds__(kBlock) void k_keys_by_index(const uint32_t* __restrict__ key,
+                                                          const uint32_t* __restrict__ perm,
+                                                          uint32_t n,
+                                                          uint32_t* __restrict__ by_index,
+                                                          uint32_t* __restrict__ iota) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  by_index[perm[j]] = key[j];
+  iota[j] = j;
+}
+
 // Reference layout of the grid for parity checks.
 __global__ __launch_bounds__(kBlock) void k_export_cells(const uint32_t* __restrict__ start,
                                                           const uint32_t* __restrict__ maxh_bits,
@@ -448,6 +491,31 @@ void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
   hipLaunchKernelGGL(k_build_soa, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, perm,
                      skeys, n, norm, part, hs, count, maxh_bits);
   ILC_HIP(hipGetLastError());
+}
+
+void launch_h_order(const float* h, uint32_t* key, uint32_t* perm, uint32_t n, uint32_t* horder,
+                    uint32_t* temp, hipStream_t s) {
+  if (n <= 1) return;
+  uint32_t* hk = horder;
+  uint32_t* hv = horder + n;
+  hipLaunchKernelGGL(k_h_order_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, h, perm, n,
+                     hk, hv);
+  ILC_HIP(hipGetLastError());
+  launch_radix_sort(hk, hv, n, 16, temp, s);
+  hipLaunchKernelGGL(k_permute_pairs, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, hv, key,
+                     perm, n, temp, temp + n);
+  ILC_HIP(hipGetLastError());
+  ILC_HIP(hipMemcpyAsync(key, temp, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  ILC_HIP(hipMemcpyAsync(perm, temp + n, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+}
+
+void launch_reference_perm(const uint32_t* key, const uint32_t* perm, uint32_t n, int key_bits,
+                           uint32_t* out_perm, uint32_t* by_index, uint32_t* temp, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_keys_by_index, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, key, perm,
+                     n, by_index, out_perm);
+  ILC_HIP(hipGetLastError());
+  launch_radix_sort(by_index, out_perm, n, key_bits, temp, s);
 }
 
 void launch_export_cells(const uint32_t* start, const uint32_t* maxh_bits, uint32_t ncells,

@@ -8,13 +8,11 @@
 
 namespace ilc {
 
-namespace {
-int bits_for(uint64_t maxval) {
+int key_bits_for(uint64_t maxval) {
   int b = 0;
   while (b < 64 && (maxval >> b)) ++b;
   return b;
 }
-}  // namespace
 
 float kernel_norm(int kernel) {
   switch (kernel) {
@@ -87,7 +85,9 @@ void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, 
   gs.hs.ensure(n);
   sc.counts.ensure(ncells + 1);
   sc.temp.ensure(std::max(radix_temp_words(n), scan_temp_words(ncells + 1)));
-  launch_radix_sort(gs.key.p, gs.perm.p, uint32_t(n), bits_for(ncells - 1), sc.temp.p, s);
+  sc.horder.ensure(2 * std::max<uint64_t>(n, 1));
+  launch_h_order(h, gs.key.p, gs.perm.p, uint32_t(n), sc.horder.p, sc.temp.p, s);
+  launch_radix_sort(gs.key.p, gs.perm.p, uint32_t(n), key_bits_for(ncells - 1), sc.temp.p, s);
   launch_fill_u32(sc.counts.p, 0u, ncells + 1, s);
   launch_fill_u32(gs.maxh_bits.p, 0u, ncells, s);
   launch_build_soa(p4, h, gs.perm.p, gs.key.p, uint32_t(n), kernel_norm(kernel), gs.part.p,

@@ -8,7 +8,10 @@
 namespace ilc {
 
 // One built grid: the reference's smoothing_particle_grid
-// (smoothing_particle_grid.hpp:29-46) as sorted SoA + cell offsets.
+// (smoothing_particle_grid.hpp:29-46) as sorted SoA + cell offsets.  Cells,
+// their ranges and max h are the reference's; within a cell the particles are
+// ordered by descending h (the reference keeps input order, which
+// launch_reference_perm recovers).
 struct grid_store {
   dbuf<float4> part;          // (x, y, coef, 1/h^2), cell-sorted
   dbuf<uint32_t> key;         // sorted cell keys
@@ -30,7 +33,7 @@ struct grid_store {
 
 // Scratch shared by successive builds on one stream.
 struct grid_scratch {
-  dbuf<uint32_t> temp, counts, flags;
+  dbuf<uint32_t> temp, counts, flags, horder;
   dbuf<float> bbox_partial;
   dbuf<bbox_result> bbox_dev;
 };
@@ -54,5 +57,8 @@ void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, 
                             int kernel, hipStream_t s);
 
 float kernel_norm(int kernel);
+
+// Bits of a radix key that holds values up to maxval.
+int key_bits_for(uint64_t maxval);
 
 }  // namespace ilc

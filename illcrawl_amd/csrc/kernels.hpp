@@ -24,6 +24,14 @@ void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits,
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
                       const uint32_t* skeys, uint32_t n, float norm, float4* part, float* hs,
                       uint32_t* count, uint32_t* maxh_bits, hipStream_t s);
+// Reorder (key, perm) pairs, n of them, by descending h[perm] (top 16 bits,
+// stable); horder holds 2n words, temp radix_temp_words(n).
+void launch_h_order(const float* h, uint32_t* key, uint32_t* perm, uint32_t n, uint32_t* horder,
+                    uint32_t* temp, hipStream_t s);
+// The reference's within-cell order (stable by input index) of a grid whose
+// sorted keys and perm are given; by_index holds n words.
+void launch_reference_perm(const uint32_t* key, const uint32_t* perm, uint32_t n, int key_bits,
+                           uint32_t* out_perm, uint32_t* by_index, uint32_t* temp, hipStream_t s);
 void launch_export_cells(const uint32_t* start, const uint32_t* maxh_bits, uint32_t ncells,
                          int32_t* cells2, float* maxh, hipStream_t s);
 void launch_bbox_final(const float* partial, int nparts, const uint32_t* bad, bbox_result* out,

@@ -176,8 +176,10 @@ ilc_status ilc_psg_grid_info(const ilc_psg* h, ilc_grid_info* out);
  * (2*ncells int32: begin, end; (0,0) when empty; particle_grid.cpp:104-113,
  * particle_grid.cl:114-190), maxh_dev (ncells floats,
  * grid3d_determine_max_per_cell), perm_dev (n uint32: sorted position ->
- * input index, the sorted_to_unsorted map of particle_grid.hpp:77-93).
- * Any pointer may be NULL. */
+ * input index, the sorted_to_unsorted map of particle_grid.hpp:77-93, in
+ * the reference's within-cell order).  Any pointer may be NULL.  Returning
+ * perm synchronises the stream: the gather orders a cell's particles by
+ * descending h, and the reference's order is rebuilt with a sort. */
 ilc_status ilc_psg_export_grid(const ilc_psg* h, int32_t* cells2_dev, float* maxh_dev,
                                uint32_t* perm_dev, void* stream);
 
