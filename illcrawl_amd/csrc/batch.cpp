@@ -128,6 +128,13 @@ struct ilc_batch {
   pinned<bbox_result> bbox_host;
   dbuf<view_desc> desc_dev;
   pinned<view_desc> desc_host;
+  // the multi-view passes of the plan: per-view projection and window, the
+  // filtered views, and their kept counts per window tile (view-major)
+  dbuf<view_build_desc> bdesc_dev;
+  pinned<view_build_desc> bdesc_host;
+  dbuf<int32_t> fviews_dev;
+  pinned<int32_t> fviews_host;
+  dbuf<uint32_t> wcounts_all;
   dbuf<uint2> items_dev;
   pinned<uint2> items_host;
   uint32_t nitems = 0;
@@ -235,14 +242,21 @@ void ensure_cubic(ilc_batch* b) {
 // 1. plan: bounding boxes of every view, one sync, grid parameters.
 void plan(ilc_batch* b, hipStream_t s) {
   const size_t nv = b->views.size();
-  const int nparts = view_bbox_parts(b->ps.n);
+  const int nparts = multi_bbox_parts(b->ps.n);
   b->bbox_parts.ensure(nv * size_t(nparts) * 8);
   b->bbox_dev.ensure(nv);
   b->bad.ensure(nv);
   b->bbox_host.ensure(nv);
-  for (size_t v = 0; v < nv; ++v)
-    launch_view_bbox(b->ps, proj_of(b, *b->views[v]), gaunt_of(b),
-                     b->bbox_parts.p + v * size_t(nparts) * 8, b->bad.p + v, b->bbox_dev.p + v, s);
+  b->bdesc_host.ensure(nv);
+  b->bdesc_dev.ensure(nv);
+  for (size_t v = 0; v < nv; ++v) {
+    b->bdesc_host.p[v].vp = proj_of(b, *b->views[v]);
+    b->bdesc_host.p[v].win = window_f{0.f, 0.f, 0.f, 0.f, 0, 0};
+  }
+  ILC_HIP(hipMemcpyAsync(b->bdesc_dev.p, b->bdesc_host.p, sizeof(view_build_desc) * nv,
+                         hipMemcpyHostToDevice, s));
+  launch_multi_bbox(b->ps, b->bdesc_dev.p, static_cast<int>(nv), gaunt_of(b), b->bbox_parts.p,
+                    b->bad.p, b->bbox_dev.p, s);
   ILC_HIP(hipMemcpyAsync(b->bbox_host.p, b->bbox_dev.p, sizeof(bbox_result) * nv,
                          hipMemcpyDeviceToHost, s));
   ILC_HIP(hipStreamSynchronize(s));
@@ -329,15 +343,27 @@ void plan(ilc_batch* b, hipStream_t s) {
     }
   }
   if (any_win) {
+    // kept counts per window tile of every filtered view, in one pass; the
+    // render scans and scatters from them
     const uint32_t nt = window_tiles(b->ps.n);
-    b->wcounts.ensure(size_t(nt) + 1);
+    b->wcounts_all.ensure(nv * size_t(nt));
     b->wtotal.ensure(nv);
     b->wtotal_host.ensure(nv);
-    launch_fill_u32(b->wtotal.p, 0u, nv, s);
-    for (size_t v = 0; v < nv; ++v)
+    b->fviews_host.ensure(nv);
+    b->fviews_dev.ensure(nv);
+    int nf = 0;
+    for (size_t v = 0; v < nv; ++v) {
+      b->bdesc_host.p[v].win = b->views[v]->win;
       if (b->views[v]->windowed || b->views[v]->compact)
-        launch_window_count(b->ps, proj_of(b, *b->views[v]), b->views[v]->win, b->wcounts.p,
-                            b->wtotal.p + v, s);
+        b->fviews_host.p[nf++] = static_cast<int32_t>(v);
+    }
+    ILC_HIP(hipMemcpyAsync(b->bdesc_dev.p, b->bdesc_host.p, sizeof(view_build_desc) * nv,
+                           hipMemcpyHostToDevice, s));
+    ILC_HIP(hipMemcpyAsync(b->fviews_dev.p, b->fviews_host.p, sizeof(int32_t) * size_t(nf),
+                           hipMemcpyHostToDevice, s));
+    launch_fill_u32(b->wtotal.p, 0u, nv, s);
+    launch_multi_window_count(b->ps.pos3, b->ps.n, b->bdesc_dev.p, b->fviews_dev.p, nf,
+                              b->wcounts_all.p, b->wtotal.p, s);
     ILC_HIP(hipMemcpyAsync(b->wtotal_host.p, b->wtotal.p, sizeof(uint32_t) * nv,
                            hipMemcpyDeviceToHost, s));
     ILC_HIP(hipStreamSynchronize(s));
@@ -412,6 +438,12 @@ struct build_ctx {
   hipStream_t st;
 };
 
+size_t view_index(const ilc_batch* b, const view_state& vs) {
+  for (size_t v = 0; v < b->views.size(); ++v)
+    if (b->views[v].get() == &vs) return v;
+  fail(ILC_EINVAL, "view not in batch");
+}
+
 // the full grid of slab group gi
 void build_group(ilc_batch* b, size_t gi, const build_ctx& c) {
   view_state& ls = *b->views[b->group_leader[gi]];
@@ -429,14 +461,14 @@ void build_view(ilc_batch* b, view_state& vs, const build_ctx& c) {
     // kept particles in input order, then the grid from their keys
     const uint32_t nt = window_tiles(b->ps.n);
     const uint64_t m = vs.nwin;
-    c.wcounts->ensure(size_t(nt) + 1);
     c.woffs->ensure(size_t(nt) + 1);
     c.htmp->ensure(std::max<uint64_t>(m, 1));
     vs.grid.key.ensure(std::max<uint64_t>(m, 1));
     vs.grid.perm.ensure(std::max<uint64_t>(m, 1));
     c.sc->temp.ensure(scan_temp_words(uint64_t(nt) + 1));
-    launch_window_count(b->ps, proj_of(b, vs), vs.win, c.wcounts->p, nullptr, st);
-    launch_exclusive_scan(c.wcounts->p, c.woffs->p, nt, c.sc->temp.p, st);
+    // the plan counted the kept particles of every window tile of this view
+    launch_exclusive_scan(b->wcounts_all.p + view_index(b, vs) * size_t(nt), c.woffs->p, nt,
+                          c.sc->temp.p, st);
     vs.orig.ensure(std::max<uint64_t>(m, 1));
     launch_window_scatter(b->ps, proj_of(b, vs), gaunt_of(b), vs.win, c.woffs->p, vs.gp,
                           c.p4tmp->p, c.htmp->p, vs.grid.key.p, vs.grid.perm.p, vs.orig.p,

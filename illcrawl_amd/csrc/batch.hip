@@ -153,6 +153,155 @@ __global__ __launch_bounds__(kBlock) void k_view_pass(particle_soa ps, view_proj
   }
 }
 
+// ---- every view in one pass
+//
+// A workgroup holds kMultiItems particles per thread in registers and loops
+// over the views; per view it reduces what the single-view kernels reduce
+// (bounding box, max h and bad count; or kept counts per window tile), so each
+// snapshot read serves every view.
+
+constexpr int kMultiItems = 8;
+constexpr int kMultiChunk = kBlock * kMultiItems;
+
+__global__ __launch_bounds__(kBlock) void k_multi_bbox(particle_soa ps,
+                                                        const view_build_desc* __restrict__ vd,
+                                                        int nv, gaunt_table gt,
+                                                        float* __restrict__ part,
+                                                        uint32_t* __restrict__ bad) {
+#pragma clang fp contract(off)
+  __shared__ float tab[kMaxGaunt];
+  __shared__ float red[2][4][5];
+  __shared__ uint32_t redb[2][4];
+  bool any_xray = false, any_mass = false;
+  for (int v = 0; v < nv; ++v) {
+    any_xray |= vd[v].vp.quantity == 1;
+    any_mass |= vd[v].vp.quantity != 1;
+  }
+  if (any_xray)
+    for (int i = threadIdx.x; i < gt.n; i += kBlock) tab[i] = gt.y[i];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nparts = gridDim.x;
+  float px[kMultiItems], py[kMultiItems], pz[kMultiItems], hh[kMultiItems];
+  float wm[kMultiItems], wx[kMultiItems];
+  bool in[kMultiItems];
+#pragma unroll
+  for (int r = 0; r < kMultiItems; ++r) {
+    const uint64_t i = uint64_t(blockIdx.x) * kMultiChunk + uint64_t(r) * kBlock + threadIdx.x;
+    in[r] = i < ps.n;
+    px[r] = py[r] = pz[r] = hh[r] = wm[r] = wx[r] = 0.0f;
+    if (in[r]) {
+      px[r] = ps.pos3[3 * i];
+      py[r] = ps.pos3[3 * i + 1];
+      pz[r] = ps.pos3[3 * i + 2];
+      hh[r] = ps.hsml[i];
+      // the two quantities of particle_of (views of this pass have scale 1,
+      // where float(1.0 * double(m)) is m)
+      if (any_mass) wm[r] = ps.mass[i];
+      if (any_xray) {
+        const float T = ps.temp[i];
+        float ion = 4.046263f * ps.rho[i];
+        ion = ion * ps.xe[i];
+        const float g = sample_1d(tab, gt.n, gt.x0, gt.dx, logf(T));
+        float e = ion * ion;
+        e = e * sqrtf(T);
+        wx[r] = e * g;
+      }
+    }
+  }
+  for (int v = 0; v < nv; ++v) {
+    const view_proj vp = vd[v].vp;
+    float mn0 = INFINITY, mn1 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY, mh = 0.0f;
+    uint32_t nbad = 0;
+#pragma unroll
+    for (int r = 0; r < kMultiItems; ++r) {
+      if (!in[r]) continue;
+      const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
+      const float ax = dot3(vp.m + 0, vx, vy, vz);
+      const float ay = dot3(vp.m + 3, vx, vy, vz);
+      const float az = dot3(vp.m + 6, vx, vy, vz);
+      float w = vp.quantity == 1 ? wx[r] : wm[r];
+      if (az < 0.0f || az > vp.depth) w = 0.0f;
+      const bool ok = isfinite(ax) && isfinite(ay) && isfinite(w) && hh[r] >= 1.1e-19f &&
+                      hh[r] <= 1.8e19f;
+      if (!ok) {
+        ++nbad;
+        continue;
+      }
+      mn0 = fminf(mn0, ax);
+      mx0 = fmaxf(mx0, ax);
+      mn1 = fminf(mn1, ay);
+      mx1 = fmaxf(mx1, ay);
+      mh = fmaxf(mh, hh[r]);
+    }
+    const float rv[5] = {wmin(mn0), wmin(mn1), wmax(mx0), wmax(mx1), wmax(mh)};
+    uint32_t bsum = nbad;
+    for (int o = 32; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 64);
+    const int slot = v & 1;
+    if (lane == 0) {
+      for (int k = 0; k < 5; ++k) red[slot][wave][k] = rv[k];
+      redb[slot][wave] = bsum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float q[5];
+      for (int k = 0; k < 5; ++k) q[k] = red[slot][0][k];
+      uint32_t bb = redb[slot][0];
+      for (int w = 1; w < 4; ++w) {
+        q[0] = fminf(q[0], red[slot][w][0]);
+        q[1] = fminf(q[1], red[slot][w][1]);
+        q[2] = fmaxf(q[2], red[slot][w][2]);
+        q[3] = fmaxf(q[3], red[slot][w][3]);
+        q[4] = fmaxf(q[4], red[slot][w][4]);
+        bb += redb[slot][w];
+      }
+      float* o = part + (uint64_t(v) * nparts + blockIdx.x) * 8;
+      o[0] = q[0];
+      o[1] = q[1];
+      o[2] = 0.0f;
+      o[3] = q[2];
+      o[4] = q[3];
+      o[5] = 0.0f;
+      o[6] = q[4];
+      if (bb) atomicAdd(bad + v, bb);
+    }
+  }
+}
+
+// one workgroup per view: the reduction of k_bbox_final
+__global__ __launch_bounds__(kBlock) void k_multi_bbox_final(const float* __restrict__ part,
+                                                              int nparts,
+                                                              const uint32_t* __restrict__ bad,
+                                                              bbox_result* __restrict__ out) {
+  __shared__ float red[4][7];
+  const int v = blockIdx.x;
+  const float* pv = part + uint64_t(v) * nparts * 8;
+  float r[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0f};
+  for (int i = threadIdx.x; i < nparts; i += kBlock) {
+    for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], pv[i * 8 + k]);
+    for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], pv[i * 8 + k]);
+  }
+  for (int k = 0; k < 3; ++k) r[k] = wmin(r[k]);
+  for (int k = 3; k < 7; ++k) r[k] = wmax(r[k]);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+    for (int k = 0; k < 7; ++k) red[wave][k] = r[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], red[w][k]);
+      for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], red[w][k]);
+    }
+    bbox_result& o = out[v];
+    for (int k = 0; k < 3; ++k) {
+      o.mn[k] = r[k];
+      o.mx[k] = r[3 + k];
+    }
+    o.max_h = r[6];
+    o.bad = bad[v];
+  }
+}
+
 // ---- filtered grid input
 //
 // Window: only the cells that some point of the view can reach matter to its
@@ -219,6 +368,54 @@ __global__ __launch_bounds__(kBlock) void k_window_count(const float* __restrict
     const uint32_t t = wc[0] + wc[1] + wc[2] + wc[3];
     counts[blockIdx.x] = t;
     if (total && t) atomicAdd(total, t);
+  }
+}
+
+// counts of k_window_count for every listed view, from one read of the tile
+__global__ __launch_bounds__(kBlock) void k_multi_window_count(
+    const float* __restrict__ pos3, uint64_t n, const view_build_desc* __restrict__ vd,
+    const int32_t* __restrict__ views, int nviews, uint32_t ntiles, uint32_t* __restrict__ counts,
+    uint32_t* __restrict__ totals) {
+#pragma clang fp contract(off)
+  __shared__ uint32_t wc[2][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
+  float px[kWinItems], py[kWinItems], pz[kWinItems];
+  bool in[kWinItems];
+#pragma unroll
+  for (int r = 0; r < kWinItems; ++r) {
+    const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
+    in[r] = i < n;
+    px[r] = in[r] ? pos3[3 * i] : 0.0f;
+    py[r] = in[r] ? pos3[3 * i + 1] : 0.0f;
+    pz[r] = in[r] ? pos3[3 * i + 2] : 0.0f;
+  }
+  for (int k = 0; k < nviews; ++k) {
+    const int v = views[k];
+    const view_proj vp = vd[v].vp;
+    const window_f win = vd[v].win;
+    uint32_t c = 0;
+#pragma unroll
+    for (int r = 0; r < kWinItems; ++r) {
+      const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
+      const float ax = dot3(vp.m + 0, vx, vy, vz);
+      const float ay = dot3(vp.m + 3, vx, vy, vz);
+      bool keep = in[r];
+      if (win.use_window) keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
+      if (win.use_depth) {
+        const float az = dot3(vp.m + 6, vx, vy, vz);
+        keep = keep && !(az < 0.0f || az > vp.depth);
+      }
+      c += __popcll(__ballot(keep));
+    }
+    const int slot = k & 1;
+    if (lane == 0) wc[slot][wave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t t = wc[slot][0] + wc[slot][1] + wc[slot][2] + wc[slot][3];
+      counts[uint64_t(v) * ntiles + blockIdx.x] = t;
+      if (t) atomicAdd(totals + v, t);
+    }
   }
 }
 
@@ -327,6 +524,35 @@ void launch_scatter_grads(const uint32_t* perm, const uint32_t* orig, uint32_t n
 }
 
 uint32_t window_tiles(uint64_t n) { return static_cast<uint32_t>((n + kWinTile - 1) / kWinTile); }
+
+int multi_bbox_parts(uint64_t n) {
+  return static_cast<int>(std::max<uint64_t>(1, (n + kMultiChunk - 1) / kMultiChunk));
+}
+
+void launch_multi_bbox(const particle_soa& ps, const view_build_desc* vd_dev, int nv,
+                       const gaunt_table& gt, float* partial, uint32_t* bad, bbox_result* out,
+                       hipStream_t s) {
+  if (nv <= 0) return;
+  if (gt.n > kMaxGaunt || (gt.n > 0 && !gt.y))
+    fail(ILC_EINVAL, "Gaunt table of %d entries (at most %d)", gt.n, kMaxGaunt);
+  const int nparts = multi_bbox_parts(ps.n);
+  launch_fill_u32(bad, 0u, uint64_t(nv), s);
+  hipLaunchKernelGGL(k_multi_bbox, dim3(nparts), dim3(kBlock), 0, s, ps, vd_dev, nv, gt, partial,
+                     bad);
+  ILC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_multi_bbox_final, dim3(nv), dim3(kBlock), 0, s, partial, nparts, bad, out);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_multi_window_count(const float* pos3, uint64_t n, const view_build_desc* vd_dev,
+                               const int32_t* views_dev, int nviews, uint32_t* counts,
+                               uint32_t* totals, hipStream_t s) {
+  const uint32_t nb = window_tiles(n);
+  if (!nb || nviews <= 0) return;
+  hipLaunchKernelGGL(k_multi_window_count, dim3(nb), dim3(kBlock), 0, s, pos3, n, vd_dev,
+                     views_dev, nviews, nb, counts, totals);
+  ILC_HIP(hipGetLastError());
+}
 
 void launch_window_count(const particle_soa& ps, const view_proj& vp, const window_f& win,
                          uint32_t* counts, uint32_t* total, hipStream_t s) {

@@ -125,6 +125,9 @@ __device__ inline float pow25(float cs) {
   return (cs * cs) * __builtin_amdgcn_sqrtf(cs);
 }
 
+// The same for an argument known to be positive.
+__device__ inline float pow25_pos(float cs) { return (cs * cs) * __builtin_amdgcn_sqrtf(cs); }
+
 // Tabulated projected cubic spline on u = r^2/h^2 in [0, 4), linear.
 __device__ inline float cubic_shape(float u, const float* tab) {
   const float x = fminf(u * (float(kCubicTable - 1) / 4.0f), float(kCubicTable - 1));
@@ -153,7 +156,7 @@ struct gather_smem {
   float4 Ls[kCap];                    // small: position form
   float4 Lm[REF ? kCap : 1];          // masked: position form
   int Lmc[REF ? kCap : 1];            // masked: packed cell (cx | cy << 16)
-  uint32_t wcnt[2][12];
+  uint32_t wcnt[2][16];
   float redf[4][4];
   int redi[4][10];
   float tab[KERNEL == 2 ? kCubicTable : 1];
@@ -387,38 +390,49 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   const bool allz_u = uniform(r_allz) != 0;
   const float cull2 = COUNT ? support * support * 1.0001f : support * support;
 
-  int nb = 0, ns = 0, nm = 0;  // list sizes (identical in every thread)
+  // list sizes (identical in every thread).  The big list holds the
+  // particles whose support covers the whole tile (argument positive at every
+  // point, no clamp) from the front of Lb, the others from the back.
+  int nb = 0, nbp = 0, ns = 0, nm = 0;
   int slot = 0;
 
   // ---- evaluation of the lists (loop bounds are wave-uniform)
-  auto eval_big = [&](int cnt) {
+  auto eval_big = [&](int cnt, int cntp) {
     if constexpr (QUARTIC) {
-      cnt = uniform(cnt);
-      for (int j0 = 0; j0 < cnt; j0 += 32) {
-        const int jn = min(32, cnt - j0);
-        float a[PPT];
+      auto run = [&](int base, int n, auto clamp) {
+        n = uniform(n);
+        for (int j0 = 0; j0 < n; j0 += 32) {
+          const int jn = min(32, n - j0);
+          float a[PPT];
 #pragma unroll
-        for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
-        int j = 0;
-        for (; j + 4 <= jn; j += 4) {
-          float4 q[4];
+          for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
+          int j = 0;
+          for (; j + 4 <= jn; j += 4) {
+            float4 q[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) q[u] = sm.Lb[j0 + j + u];
+            for (int u = 0; u < 4; ++u) q[u] = sm.Lb[base + j0 + j + u];
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int k = 0; k < PPT; ++k)
-              a[k] += pow25(fmaf(q[u].y, lx[k], fmaf(q[u].z, ly[k], fmaf(q[u].w, l2[k], q[u].x))));
+              for (int k = 0; k < PPT; ++k) {
+                const float v = fmaf(q[u].y, lx[k], fmaf(q[u].z, ly[k], fmaf(q[u].w, l2[k], q[u].x)));
+                a[k] += decltype(clamp)::value ? pow25(v) : pow25_pos(v);
+              }
+          }
+          for (; j < jn; ++j) {
+            const float4 q = sm.Lb[base + j0 + j];
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+              const float v = fmaf(q.y, lx[k], fmaf(q.z, ly[k], fmaf(q.w, l2[k], q.x)));
+              a[k] += decltype(clamp)::value ? pow25(v) : pow25_pos(v);
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
         }
-        for (; j < jn; ++j) {
-          const float4 q = sm.Lb[j0 + j];
-#pragma unroll
-          for (int k = 0; k < PPT; ++k)
-            a[k] += pow25(fmaf(q.y, lx[k], fmaf(q.z, ly[k], fmaf(q.w, l2[k], q.x))));
-        }
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
-      }
+      };
+      run(0, cnt, std::false_type{});
+      run(kCap - cntp, cntp, std::true_type{});
     }
   };
   auto eval_small = [&](int cnt) {
@@ -677,7 +691,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
     for (uint32_t v0 = 0; v0 < tot; v0 += kBlock) {
       const uint32_t v = v0 + uint32_t(t);
       float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-      bool keep = false, full = false, big = false;
+      bool keep = false, full = false, big = false, cover = false;
       int cinf = 0;
       uint32_t adj_idx = 0;
       if (v < tot) {
@@ -722,36 +736,47 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
               const float xr = q.x - tcx, yr = q.y - tcy;
               const float d2r = fmaf(yr, yr, xr * xr);
               e = make_float4(c * fmaf(-q.w, d2r, 1.0f), 2.0f * ci2 * xr, 2.0f * ci2 * yr, -ci2);
+              // s >= 0.01 at every point of the tile box: far beyond the
+              // rounding of the expanded argument (~1e-6 of c)
+              const float fx = fmaxf(q.x - tbx0, tbx1 - q.x), fy = fmaxf(q.y - tby0, tby1 - q.y);
+              cover = fmaf(fy, fy, fx * fx) * q.w < 0.99f;
             } else {
               e = make_float4(q.x, q.y, ci2, c);
             }
           }
         }
       }
-      const uint64_t bb = __ballot(big);
+      const uint64_t bb = __ballot(big && cover);
+      const uint64_t bp = __ballot(big && !cover);
       const uint64_t bs = __ballot(full && !big);
       const uint64_t bm = __ballot(keep && !full);
       if (lane == 0) {
         sm.wcnt[slot][wave] = __popcll(bb);
         sm.wcnt[slot][4 + wave] = __popcll(bs);
         sm.wcnt[slot][8 + wave] = __popcll(bm);
+        sm.wcnt[slot][12 + wave] = __popcll(bp);
       }
       __syncthreads();
-      int baseb = nb, bases = ns, basem = nm, totb = 0, tots = 0, totm = 0;
+      int baseb = nb, basebp = nbp, bases = ns, basem = nm, totb = 0, totbp = 0, tots = 0,
+          totm = 0;
       for (int w = 0; w < 4; ++w) {
         const int cb_ = sm.wcnt[slot][w], cs_ = sm.wcnt[slot][4 + w],
-                  cm_ = sm.wcnt[slot][8 + w];
+                  cm_ = sm.wcnt[slot][8 + w], cp_ = sm.wcnt[slot][12 + w];
         if (w < wave) {
           baseb += cb_;
+          basebp += cp_;
           bases += cs_;
           basem += cm_;
         }
         totb += cb_;
+        totbp += cp_;
         tots += cs_;
         totm += cm_;
       }
       if constexpr (QUARTIC) {
-        if (big) sm.Lb[baseb + prefix_count(bb)] = e;
+        // nb + nbp <= 2 * kFlush - 1 <= kCap: the two ends never meet
+        if (big && cover) sm.Lb[baseb + prefix_count(bb)] = e;
+        if (big && !cover) sm.Lb[kCap - 1 - (basebp + int(prefix_count(bp)))] = e;
       }
       if (full && !big) {
         const int pos = bases + prefix_count(bs);
@@ -766,30 +791,31 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
         }
       }
       nb += totb;
+      nbp += totbp;
       ns += tots;
       nm += totm;
       slot ^= 1;
-      if (nb >= kFlush || ns >= kFlush || nm >= kFlush) {
+      if (nb + nbp >= kFlush || ns >= kFlush || nm >= kFlush) {
         __syncthreads();
         if constexpr (ADJ) {
           eval_adjoint(ns);
         } else {
-          eval_big(nb);
+          eval_big(nb, nbp);
           eval_small(ns);
           eval_masked(nm);
         }
         __syncthreads();
-        nb = ns = nm = 0;
+        nb = nbp = ns = nm = 0;
       }
     }
     __syncthreads();  // the cell table is rewritten by the next chunk
   }
-  if (nb > 0 || ns > 0 || nm > 0) {
+  if (nb > 0 || nbp > 0 || ns > 0 || nm > 0) {
     __syncthreads();
     if constexpr (ADJ) {
       eval_adjoint(ns);
     } else {
-      eval_big(nb);
+      eval_big(nb, nbp);
       eval_small(ns);
       eval_masked(nm);
     }

@@ -82,6 +82,23 @@ struct window_f {
   float x0, x1, y0, y1;
   int use_window, use_depth;
 };
+// One view of a multi-view pass over the snapshot.
+struct view_build_desc {
+  view_proj vp;
+  window_f win;
+};
+// Bounding boxes of every view in one read of the snapshot: out[v] as
+// launch_view_bbox would give it (every view's vp.scale must be 1).  partial
+// holds multi_bbox_parts(n) * 8 * nv floats, bad nv words.
+int multi_bbox_parts(uint64_t n);
+void launch_multi_bbox(const particle_soa& ps, const view_build_desc* vd_dev, int nv,
+                       const gaunt_table& gt, float* partial, uint32_t* bad, bbox_result* out,
+                       hipStream_t s);
+// Kept particles per window tile (window_tiles(n) of them) of every listed view
+// in one read: counts[v * ntiles + t], totals[v] += the same (not cleared).
+void launch_multi_window_count(const float* pos3, uint64_t n, const view_build_desc* vd_dev,
+                               const int32_t* views_dev, int nviews, uint32_t* counts,
+                               uint32_t* totals, hipStream_t s);
 uint32_t window_tiles(uint64_t n);
 void launch_window_count(const particle_soa& ps, const view_proj& vp, const window_f& win,
                          uint32_t* counts, uint32_t* total, hipStream_t s);
