@@ -72,6 +72,15 @@ struct view_state {
   window_f win{};
   uint32_t nwin = 0;      // particles kept by the filter (from the plan)
   dbuf<uint32_t> orig;    // filtered grids: snapshot index of each grid input row
+  // the view's grid as the last render built it: its own grid, or a slice of
+  // the batch's shared grid arrays
+  grid_view gv{};
+  const uint32_t* gperm = nullptr;  // sorted position -> grid input row
+  const uint32_t* gorig = nullptr;  // grid input row -> snapshot index (NULL: identity)
+  uint64_t gbase = 0;               // first sorted position of the view
+  uint64_t gn = 0;                  // particles in the grid
+  bool gready = false;
+  uint64_t sbase = 0, cbase = 0;    // shared build: first kept row, first cell
 };
 
 template <class T>
@@ -135,6 +144,17 @@ struct ilc_batch {
   dbuf<int32_t> fviews_dev;
   pinned<int32_t> fviews_host;
   dbuf<uint32_t> wcounts_all;
+  // shared build of every view's grid (persistent renders): one scatter of
+  // every view's kept particles, one sort by (view, cell, descending h), one
+  // SoA pass; a view's grid is a slice
+  bool shared = false;
+  uint64_t sh_ntot = 0, sh_ncells = 0;
+  int sh_cellbits = 0;
+  dbuf<uint32_t> sh_offs, sh_ckey, sh_orig, sh_vid, sh_key, sh_perm, sh_sk, sh_counts, sh_start,
+      sh_maxh, sh_temp, sh_cbase;
+  pinned<uint32_t> sh_cbase_host;
+  dbuf<float> sh_wx, sh_hc, sh_hs;
+  dbuf<float4> sh_p4c, sh_part;
   dbuf<uint2> items_dev;
   pinned<uint2> items_host;
   uint32_t nitems = 0;
@@ -213,12 +233,29 @@ view_proj proj_of(const ilc_batch* b, const view_state& vs) {
   return vp;
 }
 
-// A view's grid as the gather sees it: a depth-compacted view takes the
-// per-cell max h of its group's full grid.
+// A view's grid as the gather sees it (set by the build of the last render).
 grid_view grid_of(const ilc_batch* b, const view_state& vs) {
-  grid_view g = vs.grid.view();
-  if (vs.compact) g.maxh_bits = b->group_grids[size_t(vs.group)]->maxh_bits.p;
-  return g;
+  (void)b;
+  return vs.gv;
+}
+
+float cell_radius_of(const grid_params& g) {
+  float dd = g.cs[0] * g.cs[0];
+  dd = dd + g.cs[1] * g.cs[1];
+  dd = dd + g.cs[2] * g.cs[2];
+  return 0.5f * std::sqrt(dd);
+}
+
+// the view's own grid (per-view build); a depth-compacted view takes the
+// per-cell max h of its group's full grid
+void own_grid(const ilc_batch* b, view_state& vs) {
+  vs.gv = vs.grid.view();
+  if (vs.compact) vs.gv.maxh_bits = b->group_grids[size_t(vs.group)]->maxh_bits.p;
+  vs.gperm = vs.grid.perm.p;
+  vs.gorig = (vs.windowed || vs.compact) ? vs.orig.p : nullptr;
+  vs.gbase = 0;
+  vs.gn = vs.grid.n;
+  vs.gready = true;
 }
 
 gaunt_table gaunt_of(const ilc_batch* b) {
@@ -309,7 +346,6 @@ void plan(ilc_batch* b, hipStream_t s) {
   // <= max h; support up to 2 h) plus whole cells.  Reference mode needs every
   // point's own cell inside the grid (an out-of-grid point's unclamped key
   // can alias any cell), else the view keeps all particles.
-  bool any_win = false;
   for (size_t v = 0; v < nv; ++v) {
     view_state& vs = *b->views[v];
     const double ps = vs.v.cam.width / double(vs.v.cam.nx);
@@ -336,15 +372,15 @@ void plan(ilc_batch* b, hipStream_t s) {
                         std::nextafter(float(hx + reach), INFINITY),
                         std::nextafter(float(-hy - reach), -INFINITY),
                         std::nextafter(float(hy + reach), INFINITY), 1, 0};
-      any_win = true;
     } else if (vs.compact) {
       vs.win = window_f{0.f, 0.f, 0.f, 0.f, 0, 1};
-      any_win = true;
+    } else {
+      vs.win = window_f{0.f, 0.f, 0.f, 0.f, 0, 0};  // keeps every particle
     }
   }
-  if (any_win) {
-    // kept counts per window tile of every filtered view, in one pass; the
-    // render scans and scatters from them
+  {
+    // kept counts per window tile of every view (unfiltered views keep
+    // all), in one pass; the render scans and scatters from them
     const uint32_t nt = window_tiles(b->ps.n);
     b->wcounts_all.ensure(nv * size_t(nt));
     b->wtotal.ensure(nv);
@@ -354,8 +390,8 @@ void plan(ilc_batch* b, hipStream_t s) {
     int nf = 0;
     for (size_t v = 0; v < nv; ++v) {
       b->bdesc_host.p[v].win = b->views[v]->win;
-      if (b->views[v]->windowed || b->views[v]->compact)
-        b->fviews_host.p[nf++] = static_cast<int32_t>(v);
+      b->bdesc_host.p[v].g = b->views[v]->gp;
+      b->fviews_host.p[nf++] = static_cast<int32_t>(v);
     }
     ILC_HIP(hipMemcpyAsync(b->bdesc_dev.p, b->bdesc_host.p, sizeof(view_build_desc) * nv,
                            hipMemcpyHostToDevice, s));
@@ -368,13 +404,28 @@ void plan(ilc_batch* b, hipStream_t s) {
                            hipMemcpyDeviceToHost, s));
     ILC_HIP(hipStreamSynchronize(s));
   }
+  uint64_t ntot = 0, ncells = 0;
+  int cellbits = 1;
+  b->sh_cbase_host.ensure(nv);
   for (size_t v = 0; v < nv; ++v) {
     view_state& vs = *b->views[v];
     const bool filtered = vs.windowed || vs.compact;
     vs.nwin = filtered ? b->wtotal_host.p[v] : 0u;
     sig.push_back(filtered ? uint64_t(vs.nwin) : ~uint64_t(0));
     sig.push_back(uint64_t(int64_t(vs.group)));
+    vs.sbase = ntot;
+    vs.cbase = ncells;
+    b->sh_cbase_host.p[v] = static_cast<uint32_t>(ncells);
+    ntot += b->wtotal_host.p[v];
+    ncells += vs.gp.ncells();
+    cellbits = std::max(cellbits, key_bits_for(vs.gp.ncells() - 1));
   }
+  // shared build when every index fits in 31 bits
+  b->shared = ntot > 0 && ntot < (uint64_t(1) << 31) && ncells < (uint64_t(1) << 31);
+  b->sh_ntot = ntot;
+  b->sh_ncells = ncells;
+  b->sh_cellbits = cellbits;
+  sig.push_back(b->shared ? ntot : 0);
   sig.push_back(uint64_t(b->ts.tw) << 32 | uint64_t(b->ts.ppt));
   // work items: largest views first, tiles in image order
   std::vector<uint32_t> order(nv);
@@ -482,7 +533,92 @@ void build_view(ilc_batch* b, view_state& vs, const build_ctx& c) {
                        b->kernel, b->counter.p + 1, st);
     vs.grid.density_n = 0;
   }
+  own_grid(b, vs);
   launch_fill_f64(vs.acc.p, 0.0, uint64_t(vs.v.cam.nx) * vs.v.cam.ny, st);
+}
+
+// every view's grid in one pipeline on stream s (persistent renders)
+void build_shared(ilc_batch* b, hipStream_t s) {
+  const size_t nv = b->views.size();
+  const uint32_t nt = window_tiles(b->ps.n);
+  const uint64_t ntot = b->sh_ntot, nc = b->sh_ncells;
+  const uint32_t n32 = static_cast<uint32_t>(ntot);
+  b->sh_offs.ensure(nv * size_t(nt) + 1);
+  b->sh_temp.ensure(std::max({radix_temp_words(ntot), scan_temp_words(nv * uint64_t(nt) + 1),
+                              scan_temp_words(nc + 1)}));
+  for (auto* d : {&b->sh_ckey, &b->sh_orig, &b->sh_vid, &b->sh_key, &b->sh_perm, &b->sh_sk})
+    d->ensure(ntot);
+  b->sh_hc.ensure(ntot);
+  b->sh_hs.ensure(ntot);
+  b->sh_p4c.ensure(ntot);
+  b->sh_part.ensure(ntot);
+  b->sh_counts.ensure(nc + 1);
+  b->sh_start.ensure(nc + 1);
+  b->sh_maxh.ensure(nc);
+  b->sh_cbase.ensure(nv);
+  ILC_HIP(hipMemcpyAsync(b->sh_cbase.p, b->sh_cbase_host.p, sizeof(uint32_t) * nv,
+                         hipMemcpyHostToDevice, s));
+  // kept particles of every view at global positions
+  launch_exclusive_scan(b->wcounts_all.p, b->sh_offs.p, nv * uint64_t(nt), b->sh_temp.p, s);
+  bool any_xray = false;
+  for (auto& v : b->views) any_xray |= v->v.quantity == ILC_QUANTITY_XRAY;
+  if (any_xray) {
+    b->sh_wx.ensure(b->ps.n);
+    launch_particle_xray(b->ps, gaunt_of(b), b->sh_wx.p, s);
+  }
+  launch_multi_window_scatter(b->ps, any_xray ? b->sh_wx.p : nullptr, b->bdesc_dev.p,
+                              b->fviews_dev.p, static_cast<int>(nv), b->sh_offs.p, b->sh_p4c.p,
+                              b->sh_hc.p, b->sh_ckey.p, b->sh_orig.p, b->sh_vid.p,
+                              b->counter.p + 1, s);
+  // stable sorts by descending h, cell, view: a view's rows end up in one
+  // range, by cell, biggest h first, ties in input order
+  launch_hq_keys(b->sh_hc.p, n32, b->sh_sk.p, b->sh_perm.p, s);
+  launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, 16, b->sh_temp.p, s);
+  launch_gather_u32(b->sh_ckey.p, b->sh_perm.p, n32, b->sh_sk.p, s);
+  launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, b->sh_cellbits, b->sh_temp.p, s);
+  if (nv > 1) {
+    launch_gather_u32(b->sh_vid.p, b->sh_perm.p, n32, b->sh_sk.p, s);
+    launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, key_bits_for(nv - 1), b->sh_temp.p, s);
+  }
+  // view-local cell keys for the gather, global ones (view's first cell +
+  // local) for the cell ranges
+  launch_final_keys(b->sh_perm.p, b->sh_ckey.p, b->sh_vid.p, b->sh_cbase.p, n32, b->sh_key.p,
+                    b->sh_sk.p, s);
+  launch_fill_u32(b->sh_counts.p, 0u, nc + 1, s);
+  launch_fill_u32(b->sh_maxh.p, 0u, nc, s);
+  launch_build_soa(b->sh_p4c.p, b->sh_hc.p, b->sh_perm.p, b->sh_sk.p, n32, kernel_norm(b->kernel),
+                   b->sh_part.p, b->sh_hs.p, b->sh_counts.p, b->sh_maxh.p, s);
+  launch_exclusive_scan(b->sh_counts.p, b->sh_start.p, nc + 1, b->sh_temp.p, s);
+  for (size_t v = 0; v < nv; ++v) {
+    view_state& vs = *b->views[v];
+    grid_view g{};
+    g.part = b->sh_part.p;
+    g.key = b->sh_key.p;
+    g.hs = b->sh_hs.p;
+    g.start = b->sh_start.p + vs.cbase;
+    g.maxh_bits = vs.compact ? b->group_grids[size_t(vs.group)]->maxh_bits.p
+                             : b->sh_maxh.p + vs.cbase;
+    for (int a = 0; a < 3; ++a) {
+      g.nc[a] = vs.gp.nc[a];
+      g.minc[a] = vs.gp.minc[a];
+      g.cs[a] = vs.gp.cs[a];
+    }
+    g.max_h = vs.max_h;
+    g.cell_radius = cell_radius_of(vs.gp);
+    g.kz0 = vs.kz0;
+    g.ncells = static_cast<uint32_t>(vs.gp.ncells());
+    const uint64_t kept = vs.compact ? uint64_t(vs.nwin) : (vs.windowed ? vs.nwin : b->ps.n);
+    // enumerate like the grid the view stands for (grid_store::view)
+    const uint64_t dn = vs.compact ? kept : b->ps.n;
+    g.sparse = dn * 5 < uint64_t(g.ncells) * 2 ? 1 : 0;
+    vs.gv = g;
+    vs.gperm = b->sh_perm.p;
+    vs.gorig = b->sh_orig.p;
+    vs.gbase = vs.sbase;
+    vs.gn = kept;
+    vs.gready = true;
+    launch_fill_f64(vs.acc.p, 0.0, uint64_t(vs.v.cam.nx) * vs.v.cam.ny, s);
+  }
 }
 
 void fill_desc(ilc_batch* b, size_t v) {
@@ -540,9 +676,14 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
       return build_ctx{&l.scratch, &l.p4tmp, &l.htmp, &l.wcounts, &l.woffs, l.st};
     };
     for (size_t gi = 0; gi < b->group_leader.size(); ++gi) build_group(b, gi, lane_ctx(job++));
-    for (size_t v = 0; v < nv; ++v) {
-      build_view(b, *b->views[v], lane_ctx(job++));
-      fill_desc(b, v);
+    if (b->shared) {
+      build_shared(b, s);
+      for (size_t v = 0; v < nv; ++v) fill_desc(b, v);
+    } else {
+      for (size_t v = 0; v < nv; ++v) {
+        build_view(b, *b->views[v], lane_ctx(job++));
+        fill_desc(b, v);
+      }
     }
     for (auto& l : b->lanes) {
       ILC_HIP(hipEventRecord(l->done, l->st));
@@ -741,14 +882,14 @@ ilc_status ilc_batch_view_info(const ilc_batch* b, int32_t view, ilc_grid_info* 
     ILC_REQUIRE(b && out, "null argument");
     ILC_REQUIRE(view >= 0 && size_t(view) < b->views.size(), "view %d out of range", view);
     const view_state& vs = *b->views[view];
-    if (!vs.grid.ready) fail(ILC_ENOTREADY, "view %d has no grid (render first)", view);
+    if (!vs.gready) fail(ILC_ENOTREADY, "view %d has no grid (render first)", view);
     for (int a = 0; a < 3; ++a) {
-      out->num_cells[a] = vs.grid.gp.nc[a];
-      out->min_corner[a] = vs.grid.gp.minc[a];
-      out->cell_size[a] = vs.grid.gp.cs[a];
+      out->num_cells[a] = vs.gv.nc[a];
+      out->min_corner[a] = vs.gv.minc[a];
+      out->cell_size[a] = vs.gv.cs[a];
     }
-    out->max_h = vs.grid.max_h;
-    out->num_particles = vs.grid.n;
+    out->max_h = vs.gv.max_h;
+    out->num_particles = vs.gn;
   });
 }
 
@@ -816,23 +957,35 @@ ilc_status ilc_batch_backward(ilc_batch* b, const float* const* dl_dimage_dev, f
       ILC_HIP(hipMemcpy(b->dcubic.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
       b->dcubic_ready = true;
     }
+    // per-particle gradients in the grids' sorted order: one array of each
+    // component spanning every view's range (the adjoint indexes a view's
+    // particles by their sorted position)
+    uint64_t span = 0;
+    for (size_t v = 0; v < b->views.size(); ++v) {
+      const view_state& vs = *b->views[v];
+      ILC_REQUIRE(dl_dimage_dev[v], "null gradient image for view %zu", v);
+      if (!vs.gready) fail(ILC_ENOTREADY, "view %zu has no grid (render first)", v);
+      span = std::max(span, vs.gbase + vs.gn);
+    }
     dbuf<float> g4;
+    g4.ensure(4 * std::max<uint64_t>(span, 1));
+    float* gw = g4.p;
+    float* gh = g4.p + span;
+    float* gx = g4.p + 2 * span;
+    float* gy = g4.p + 3 * span;
     for (size_t v = 0; v < b->views.size(); ++v) {
       view_state& vs = *b->views[v];
-      ILC_REQUIRE(dl_dimage_dev[v], "null gradient image for view %zu", v);
-      if (!vs.grid.ready) fail(ILC_ENOTREADY, "view %zu has no grid (render first)", v);
-      const uint64_t m = vs.grid.n;
+      const uint64_t m = vs.gn, o = vs.gbase;
       if (!m) continue;
-      g4.ensure(4 * m);
-      launch_fill_u32(reinterpret_cast<uint32_t*>(g4.p), 0u, 4 * m, s);
+      for (float* c : {gw, gh, gx, gy})
+        launch_fill_u32(reinterpret_cast<uint32_t*>(c + o), 0u, m, s);
       const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
       const point_view p{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
       launch_adjoint(grid_of(b, vs), p, b->kernel, dl_dimage_dev[v], vs.v.scale,
-                     kernel_norm(b->kernel), b->cubic.p, b->dcubic.p, g4.p, g4.p + m, g4.p + 2 * m,
-                     g4.p + 3 * m, s);
-      launch_scatter_grads(vs.grid.perm.p, (vs.windowed || vs.compact) ? vs.orig.p : nullptr,
-                           static_cast<uint32_t>(m), g4.p, g4.p + m, g4.p + 2 * m, g4.p + 3 * m,
-                           b->ps.pos3, proj_of(b, vs), dl_dw_dev, dl_dh_dev, dl_dpos3_dev, s);
+                     kernel_norm(b->kernel), b->cubic.p, b->dcubic.p, gw, gh, gx, gy, s);
+      launch_scatter_grads(vs.gperm + o, vs.gorig, static_cast<uint32_t>(m), gw + o, gh + o,
+                           gx + o, gy + o, b->ps.pos3, proj_of(b, vs), dl_dw_dev, dl_dh_dev,
+                           dl_dpos3_dev, s);
     }
     ILC_HIP(hipStreamSynchronize(s));
   });

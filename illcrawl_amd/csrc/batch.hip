@@ -419,6 +419,146 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_particle_xray(particle_soa ps, gaunt_table gt,
+                                                           float* __restrict__ wx) {
+#pragma clang fp contract(off)
+  __shared__ float tab[kMaxGaunt];
+  for (int i = threadIdx.x; i < gt.n; i += kBlock) tab[i] = gt.y[i];
+  __syncthreads();
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < ps.n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    const float T = ps.temp[i];
+    float ion = 4.046263f * ps.rho[i];
+    ion = ion * ps.xe[i];
+    const float g = sample_1d(tab, gt.n, gt.x0, gt.dx, logf(T));
+    float e = ion * ion;
+    e = e * sqrtf(T);
+    wx[i] = e * g;
+  }
+}
+
+// k_window_scatter for every listed view from one read of the tile.  The
+// ballots of a view's kWinItems rows are counted per wave first, so one
+// barrier per view gives every row's offset; the order is the one of
+// k_window_scatter (rows, then threads).
+__global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
+    particle_soa ps, const float* __restrict__ wx, const view_build_desc* __restrict__ vd,
+    const int32_t* __restrict__ views, int nviews, uint32_t ntiles,
+    const uint32_t* __restrict__ offs, float4* __restrict__ p4c, float* __restrict__ hc,
+    uint32_t* __restrict__ ckey, uint32_t* __restrict__ orig, uint32_t* __restrict__ vid,
+    uint32_t* __restrict__ err) {
+#pragma clang fp contract(off)
+  __shared__ uint32_t wc[2][4][kWinItems];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
+  float px[kWinItems], py[kWinItems], pz[kWinItems];
+  bool in[kWinItems];
+#pragma unroll
+  for (int r = 0; r < kWinItems; ++r) {
+    const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
+    in[r] = i < ps.n;
+    px[r] = in[r] ? ps.pos3[3 * i] : 0.0f;
+    py[r] = in[r] ? ps.pos3[3 * i + 1] : 0.0f;
+    pz[r] = in[r] ? ps.pos3[3 * i + 2] : 0.0f;
+  }
+  for (int k = 0; k < nviews; ++k) {
+    const int v = views[k];
+    const view_proj vp = vd[v].vp;
+    const window_f win = vd[v].win;
+    const grid_params g = vd[v].g;
+    const int slot = k & 1;
+    uint64_t bal[kWinItems];
+#pragma unroll
+    for (int r = 0; r < kWinItems; ++r) {
+      const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
+      const float ax = dot3(vp.m + 0, vx, vy, vz);
+      const float ay = dot3(vp.m + 3, vx, vy, vz);
+      bool keep = in[r];
+      if (win.use_window) keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
+      if (win.use_depth) {
+        const float az = dot3(vp.m + 6, vx, vy, vz);
+        keep = keep && !(az < 0.0f || az > vp.depth);
+      }
+      bal[r] = __ballot(keep);
+      if (lane == 0) wc[slot][wave][r] = __popcll(bal[r]);
+    }
+    __syncthreads();
+    uint32_t out = offs[uint64_t(v) * ntiles + blockIdx.x];
+#pragma unroll
+    for (int r = 0; r < kWinItems; ++r) {
+      uint32_t before = 0, tot = 0;
+      for (int w = 0; w < 4; ++w) {
+        const uint32_t cw = wc[slot][w][r];
+        if (w < wave) before += cw;
+        tot += cw;
+      }
+      if ((bal[r] >> lane) & 1ull) {
+        const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
+        const uint32_t j = out + before +
+                           __builtin_amdgcn_mbcnt_hi(uint32_t(bal[r] >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(bal[r]), 0u));
+        const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
+        const float ax = dot3(vp.m + 0, vx, vy, vz);
+        const float ay = dot3(vp.m + 3, vx, vy, vz);
+        const float az = dot3(vp.m + 6, vx, vy, vz);
+        float w = vp.quantity == 1 ? wx[i] : static_cast<float>(vp.scale * double(ps.mass[i]));
+        if (az < 0.0f || az > vp.depth) w = 0.0f;
+        p4c[j] = make_float4(ax, ay, 0.0f, w);
+        hc[j] = ps.hsml[i];
+        // cell key of (x', y', 0): grid3d_get_cell_indices, particle_grid.cl:45-53
+        const float c3[3] = {ax, ay, 0.0f};
+        int32_t idx[3];
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+          const float d = c3[a] - g.minc[a];
+          idx[a] = floor_to_int(d / g.cs[a]);
+          ok = ok && idx[a] >= 0 && idx[a] < g.nc[a];
+        }
+        uint32_t key = 0;
+        if (ok)
+          key = uint32_t(idx[0]) + uint32_t(idx[1]) * uint32_t(g.nc[0]) +
+                uint32_t(idx[2]) * uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
+        else
+          atomicAdd(err, 1u);
+        ckey[j] = key;
+        orig[j] = uint32_t(i);
+        vid[j] = uint32_t(v);
+      }
+      out += tot;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_hq_keys(const float* __restrict__ h, uint32_t n,
+                                                    uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  keys[j] = 0x7fffu - (__float_as_uint(h[j]) >> 16);
+  vals[j] = j;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_u32(const uint32_t* __restrict__ src,
+                                                       const uint32_t* __restrict__ idx,
+                                                       uint32_t n, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_final_keys(const uint32_t* __restrict__ order,
+                                                       const uint32_t* __restrict__ ckey,
+                                                       const uint32_t* __restrict__ vid,
+                                                       const uint32_t* __restrict__ cbase,
+                                                       uint32_t n, uint32_t* __restrict__ lkey,
+                                                       uint32_t* __restrict__ gkey) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = order[i];
+  const uint32_t k = ckey[j];
+  lkey[i] = k;
+  gkey[i] = cbase[vid[j]] + k;
+}
+
 // Kept particles in input order: offsets = exclusive scan of the counts;
 // writes the projected float4 (x', y', 0, w with the depth mask), h, the cell
 // key (as k_keys computes it) and the compacted index.
@@ -527,6 +667,53 @@ uint32_t window_tiles(uint64_t n) { return static_cast<uint32_t>((n + kWinTile -
 
 int multi_bbox_parts(uint64_t n) {
   return static_cast<int>(std::max<uint64_t>(1, (n + kMultiChunk - 1) / kMultiChunk));
+}
+
+void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* wx,
+                          hipStream_t s) {
+  if (!ps.n) return;
+  if (gt.n < 1 || gt.n > kMaxGaunt || !gt.y)
+    fail(ILC_EINVAL, "X-ray view without a Gaunt table (1..%d entries)", kMaxGaunt);
+  uint64_t nb = (ps.n + kBlock - 1) / kBlock;
+  if (nb > 16384) nb = 16384;
+  hipLaunchKernelGGL(k_particle_xray, dim3(uint32_t(nb)), dim3(kBlock), 0, s, ps, gt, wx);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
+                                 const view_build_desc* vd_dev, const int32_t* views_dev,
+                                 int nviews, const uint32_t* offs, float4* p4c, float* hc,
+                                 uint32_t* ckey, uint32_t* orig, uint32_t* vid, uint32_t* err,
+                                 hipStream_t s) {
+  const uint32_t nb = window_tiles(ps.n);
+  if (!nb || nviews <= 0) return;
+  hipLaunchKernelGGL(k_multi_window_scatter, dim3(nb), dim3(kBlock), 0, s, ps, wx, vd_dev,
+                     views_dev, nviews, nb, offs, p4c, hc, ckey, orig, vid, err);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_hq_keys(const float* h, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_hq_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, h, n, keys,
+                     vals);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* out,
+                       hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_gather_u32, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, src, idx,
+                     n, out);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_final_keys(const uint32_t* order, const uint32_t* ckey, const uint32_t* vid,
+                       const uint32_t* cbase, uint32_t n, uint32_t* lkey, uint32_t* gkey,
+                       hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_final_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, order,
+                     ckey, vid, cbase, n, lkey, gkey);
+  ILC_HIP(hipGetLastError());
 }
 
 void launch_multi_bbox(const particle_soa& ps, const view_build_desc* vd_dev, int nv,

@@ -86,6 +86,7 @@ struct window_f {
 struct view_build_desc {
   view_proj vp;
   window_f win;
+  grid_params g;  // the view's grid (cell keys of the scatter)
 };
 // Bounding boxes of every view in one read of the snapshot: out[v] as
 // launch_view_bbox would give it (every view's vp.scale must be 1).  partial
@@ -100,6 +101,27 @@ void launch_multi_window_count(const float* pos3, uint64_t n, const view_build_d
                                const int32_t* views_dev, int nviews, uint32_t* counts,
                                uint32_t* totals, hipStream_t s);
 uint32_t window_tiles(uint64_t n);
+// X-ray weight of every particle (the quantity of particle_of, view-free).
+void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* wx,
+                          hipStream_t s);
+// The kept particles of every listed view, in input order, at the global
+// positions offs[v * ntiles + t] (exclusive scan of the multi-window counts):
+// projected float4 (x', y', 0, w with the depth mask), h, the cell key in the
+// view's grid, the snapshot index and the view.  wx: launch_particle_xray.
+void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
+                                 const view_build_desc* vd_dev, const int32_t* views_dev,
+                                 int nviews, const uint32_t* offs, float4* p4c, float* hc,
+                                 uint32_t* ckey, uint32_t* orig, uint32_t* vid, uint32_t* err,
+                                 hipStream_t s);
+// keys[j] = descending-h key of h[j] (as launch_h_order), vals[j] = j
+void launch_hq_keys(const float* h, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
+// out[i] = src[idx[i]]
+void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* out,
+                       hipStream_t s);
+// lkey[i] = ckey[order[i]], gkey[i] = cbase[vid[order[i]]] + lkey[i]
+void launch_final_keys(const uint32_t* order, const uint32_t* ckey, const uint32_t* vid,
+                       const uint32_t* cbase, uint32_t n, uint32_t* lkey, uint32_t* gkey,
+                       hipStream_t s);
 void launch_window_count(const particle_soa& ps, const view_proj& vp, const window_f& win,
                          uint32_t* counts, uint32_t* total, hipStream_t s);
 void launch_window_scatter(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
