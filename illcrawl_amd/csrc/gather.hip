@@ -53,8 +53,10 @@ namespace ilc {
 
 namespace {
 
-constexpr int kCap = 512;    // list capacity (entries)
-constexpr int kFlush = 256;  // evaluate when a list reaches this size
+constexpr int kCap = 320;    // list capacity (entries)
+constexpr int kFlush = 64;   // evaluate when a list reaches this size
+// a batch of kBlock candidates lands on a list holding at most kFlush - 1
+static_assert(kFlush - 1 + kBlock <= kCap, "list capacity");
 constexpr int kCubicTable = 4096;
 
 __device__ inline int32_t floor_to_int(float f) {
@@ -774,7 +776,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
         totm += cm_;
       }
       if constexpr (QUARTIC) {
-        // nb + nbp <= 2 * kFlush - 1 <= kCap: the two ends never meet
+        // nb + nbp <= kFlush - 1 + kBlock <= kCap: the two ends never meet
         if (big && cover) sm.Lb[baseb + prefix_count(bb)] = e;
         if (big && !cover) sm.Lb[kCap - 1 - (basebp + int(prefix_count(bp)))] = e;
       }
@@ -860,7 +862,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, ui
 // workgroup drains its XCD's queue, then helps with the others; every queue
 // counter only grows, so every workgroup leaves once all queues are empty.
 template <int TW, int PPT, int OP, int KERNEL>
-__global__ __launch_bounds__(kBlock) void k_gather_batch(const view_desc* __restrict__ views,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1 ? 7 : (PPT <= 2 ? 5 : 4), 8))) void k_gather_batch(const view_desc* __restrict__ views,
                                                          const uint2* __restrict__ items,
                                                          queue_segs segs,
                                                          uint32_t* __restrict__ counters,
