@@ -157,7 +157,7 @@ struct gather_smem {
   float4 Lb[KERNEL == 0 && !ADJ ? kCap : 1];  // big: expansion coefficients
   float4 Ls[kCap];                    // small: position form
   float4 Lm[REF ? kCap : 1];          // masked: position form
-  int Lmc[REF ? kCap : 1];            // masked: packed cell (cx | cy << 16)
+  alignas(16) int Lmc[REF ? kCap : 1];  // masked: packed cell (cx | cy << 16)
   uint32_t wcnt[2][16];
   float redf[4][4];
   int redi[4][10];
@@ -495,38 +495,53 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       }
     }
   };
-  // masked entries come in runs of one cell: the per-point membership test
-  // is recomputed only when the (wave-uniform) cell changes
+  // masked entries: each point tests the entry's cell against its own search
+  // box, as unsigned(cx - x0) <= width (and the same for y) on the box
+  // clamped to the cells a list entry can have (0..65535 per axis; an empty
+  // box or a point outside the grid layer fails every cell).  Four entries and
+  // their cells are read at once.
   auto eval_masked = [&](int cnt) {
     if constexpr (REF) {
       cnt = uniform(cnt);
-      int cur = -1;
-      float keepf[PPT];
+      int bx0[PPT], by0[PPT];
+      uint32_t bxw[PPT], byw[PPT];
 #pragma unroll
-      for (int k = 0; k < PPT; ++k) keepf[k] = 0.0f;
+      for (int k = 0; k < PPT; ++k) {
+        const int x0 = max(lox[k], 0), x1 = min(hix[k], 65535);
+        const int y0 = max(loy[k], 0), y1 = min(hiy[k], 65535);
+        const bool ok = zin[k] && x0 <= x1 && y0 <= y1;
+        bx0[k] = ok ? x0 : INT32_MAX;
+        by0[k] = y0;
+        bxw[k] = ok ? uint32_t(x1 - x0) : 0u;
+        byw[k] = ok ? uint32_t(y1 - y0) : 0u;
+      }
+      auto add = [&](float* a, const float4& q, int c) {
+        const int cx = c & 0xffff, cy = c >> 16;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+          const bool in = uint32_t(cx - bx0[k]) <= bxw[k] && uint32_t(cy - by0[k]) <= byw[k];
+          const float dx = q.x - ex[k], dy = q.y - ey[k];
+          const float r2 = fmaf(dy, dy, dx * dx);
+          a[k] = fmaf(in ? 1.0f : 0.0f, pow25(fmaf(-r2, q.z, q.w)), a[k]);
+        }
+      };
       for (int j0 = 0; j0 < cnt; j0 += 32) {
         const int jn = min(32, cnt - j0);
         float a[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
-        for (int j = 0; j < jn; ++j) {
-          const int c = uniform(sm.Lmc[j0 + j]);
-          if (c != cur) {
-            cur = c;
-            const int cx = c & 0xffff, cy = c >> 16;
+        int j = 0;
+        for (; j + 4 <= jn; j += 4) {
+          const int4 cc = *reinterpret_cast<const int4*>(&sm.Lmc[j0 + j]);
+          float4 q[4];
 #pragma unroll
-            for (int k = 0; k < PPT; ++k)
-              keepf[k] = (zin[k] & (cx >= lox[k]) & (cx <= hix[k]) & (cy >= loy[k]) &
-                          (cy <= hiy[k])) ? 1.0f : 0.0f;
-          }
-          const float4 q = sm.Lm[j0 + j];
-#pragma unroll
-          for (int k = 0; k < PPT; ++k) {
-            const float dx = q.x - ex[k], dy = q.y - ey[k];
-            const float r2 = fmaf(dy, dy, dx * dx);
-            a[k] = fmaf(keepf[k], pow25(fmaf(-r2, q.z, q.w)), a[k]);
-          }
+          for (int u = 0; u < 4; ++u) q[u] = sm.Lm[j0 + j + u];
+          add(a, q[0], uniform(cc.x));
+          add(a, q[1], uniform(cc.y));
+          add(a, q[2], uniform(cc.z));
+          add(a, q[3], uniform(cc.w));
         }
+        for (; j < jn; ++j) add(a, sm.Lm[j0 + j], uniform(sm.Lmc[j0 + j]));
 #pragma unroll
         for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
       }
