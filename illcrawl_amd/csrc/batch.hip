@@ -437,10 +437,11 @@ __global__ __launch_bounds__(kBlock) void k_particle_xray(particle_soa ps, gaunt
   }
 }
 
-// k_window_scatter for every listed view from one read of the tile.  The
-// ballots of a view's kWinItems rows are counted per wave first, so one
-// barrier per view gives every row's offset; the order is the one of
-// k_window_scatter (rows, then threads).
+// k_window_scatter for every listed view from one read of the tile.  Per
+// view, the kept counts of the kWinItems rows (rows of kBlock particles) are
+// taken per wave first, so one barrier per view gives every row's offset;
+// the rows are then written one at a time, skipping those with nothing kept,
+// in the order of k_window_scatter (rows, then threads).
 __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     particle_soa ps, const float* __restrict__ wx, const view_build_desc* __restrict__ vd,
     const int32_t* __restrict__ views, int nviews, uint32_t ntiles,
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     uint32_t* __restrict__ ckey, uint32_t* __restrict__ orig, uint32_t* __restrict__ vid,
     uint32_t* __restrict__ err) {
 #pragma clang fp contract(off)
-  __shared__ uint32_t wc[2][4][kWinItems];
+  __shared__ __align__(16) uint32_t wc[2][kWinItems][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
   float px[kWinItems], py[kWinItems], pz[kWinItems];
@@ -461,46 +462,51 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     py[r] = in[r] ? ps.pos3[3 * i + 1] : 0.0f;
     pz[r] = in[r] ? ps.pos3[3 * i + 2] : 0.0f;
   }
+  auto kept = [](const view_proj& vp, const window_f& win, float x, float y, float z, bool ok,
+                 float* ax, float* ay, float* az) {
+    const float vx = x - vp.c[0], vy = y - vp.c[1], vz = z - vp.c[2];
+    *ax = dot3(vp.m + 0, vx, vy, vz);
+    *ay = dot3(vp.m + 3, vx, vy, vz);
+    *az = dot3(vp.m + 6, vx, vy, vz);
+    bool keep = ok;
+    if (win.use_window)
+      keep = keep && *ax >= win.x0 && *ax <= win.x1 && *ay >= win.y0 && *ay <= win.y1;
+    if (win.use_depth) keep = keep && !(*az < 0.0f || *az > vp.depth);
+    return keep;
+  };
   for (int k = 0; k < nviews; ++k) {
     const int v = views[k];
     const view_proj vp = vd[v].vp;
     const window_f win = vd[v].win;
-    const grid_params g = vd[v].g;
     const int slot = k & 1;
-    uint64_t bal[kWinItems];
 #pragma unroll
     for (int r = 0; r < kWinItems; ++r) {
-      const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
-      const float ax = dot3(vp.m + 0, vx, vy, vz);
-      const float ay = dot3(vp.m + 3, vx, vy, vz);
-      bool keep = in[r];
-      if (win.use_window) keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
-      if (win.use_depth) {
-        const float az = dot3(vp.m + 6, vx, vy, vz);
-        keep = keep && !(az < 0.0f || az > vp.depth);
-      }
-      bal[r] = __ballot(keep);
-      if (lane == 0) wc[slot][wave][r] = __popcll(bal[r]);
+      float ax, ay, az;
+      const bool keep = kept(vp, win, px[r], py[r], pz[r], in[r], &ax, &ay, &az);
+      const uint32_t c = __popcll(__ballot(keep));
+      if (lane == 0) wc[slot][r][wave] = c;
     }
     __syncthreads();
     uint32_t out = offs[uint64_t(v) * ntiles + blockIdx.x];
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < kWinItems; ++r) {
-      uint32_t before = 0, tot = 0;
-      for (int w = 0; w < 4; ++w) {
-        const uint32_t cw = wc[slot][w][r];
-        if (w < wave) before += cw;
-        tot += cw;
-      }
-      if ((bal[r] >> lane) & 1ull) {
-        const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
+      const uint4 cw = *reinterpret_cast<const uint4*>(wc[slot][r]);
+      const uint32_t tot = cw.x + cw.y + cw.z + cw.w;
+      if (tot == 0) continue;
+      const uint32_t before = (wave > 0 ? cw.x : 0u) + (wave > 1 ? cw.y : 0u) +
+                              (wave > 2 ? cw.z : 0u);
+      const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
+      const bool ok = i < ps.n;
+      const float x = ok ? ps.pos3[3 * i] : 0.0f, y = ok ? ps.pos3[3 * i + 1] : 0.0f,
+                  z = ok ? ps.pos3[3 * i + 2] : 0.0f;
+      float ax, ay, az;
+      const bool keep = kept(vp, win, x, y, z, ok, &ax, &ay, &az);
+      const uint64_t bal = __ballot(keep);
+      if (keep) {
+        const grid_params& g = vd[v].g;
         const uint32_t j = out + before +
-                           __builtin_amdgcn_mbcnt_hi(uint32_t(bal[r] >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(bal[r]), 0u));
-        const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
-        const float ax = dot3(vp.m + 0, vx, vy, vz);
-        const float ay = dot3(vp.m + 3, vx, vy, vz);
-        const float az = dot3(vp.m + 6, vx, vy, vz);
+                           __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
         float w = vp.quantity == 1 ? wx[i] : static_cast<float>(vp.scale * double(ps.mass[i]));
         if (az < 0.0f || az > vp.depth) w = 0.0f;
         p4c[j] = make_float4(ax, ay, 0.0f, w);
@@ -508,14 +514,14 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
         // cell key of (x', y', 0): grid3d_get_cell_indices, particle_grid.cl:45-53
         const float c3[3] = {ax, ay, 0.0f};
         int32_t idx[3];
-        bool ok = true;
+        bool in_grid = true;
         for (int a = 0; a < 3; ++a) {
           const float d = c3[a] - g.minc[a];
           idx[a] = floor_to_int(d / g.cs[a]);
-          ok = ok && idx[a] >= 0 && idx[a] < g.nc[a];
+          in_grid = in_grid && idx[a] >= 0 && idx[a] < g.nc[a];
         }
         uint32_t key = 0;
-        if (ok)
+        if (in_grid)
           key = uint32_t(idx[0]) + uint32_t(idx[1]) * uint32_t(g.nc[0]) +
                 uint32_t(idx[2]) * uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
         else

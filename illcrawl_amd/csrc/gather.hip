@@ -851,7 +851,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
 }
 
 template <int TW, int PPT, int OP, int KERNEL>
-__global__ __launch_bounds__(kBlock) void k_gather(grid_view g, point_view p, uint32_t tiles_x,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1 ? 7 : (PPT <= 2 ? 5 : 4), 8))) void k_gather(grid_view g, point_view p, uint32_t tiles_x,
                                                    uint32_t ntiles, double* __restrict__ acc_out,
                                                    unsigned long long* __restrict__ counts,
                                                    const float* __restrict__ cubic_table,
