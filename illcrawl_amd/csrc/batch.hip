@@ -327,7 +327,7 @@ __device__ inline int32_t floor_to_int(float f) {
   return static_cast<int32_t>(fl);
 }
 
-constexpr int kWinItems = 16;                   // particles per thread
+constexpr int kWinItems = 8;                    // particles per thread
 constexpr int kWinTile = kBlock * kWinItems;    // particles per workgroup
 
 __device__ inline bool in_window(const float* pos3, const view_proj& vp, const window_f& win,
@@ -395,18 +395,24 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
     const view_proj vp = vd[v].vp;
     const window_f win = vd[v].win;
     uint32_t c = 0;
+    if (!win.use_window && !win.use_depth) {
 #pragma unroll
-    for (int r = 0; r < kWinItems; ++r) {
-      const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
-      const float ax = dot3(vp.m + 0, vx, vy, vz);
-      const float ay = dot3(vp.m + 3, vx, vy, vz);
-      bool keep = in[r];
-      if (win.use_window) keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
-      if (win.use_depth) {
-        const float az = dot3(vp.m + 6, vx, vy, vz);
-        keep = keep && !(az < 0.0f || az > vp.depth);
+      for (int r = 0; r < kWinItems; ++r) c += __popcll(__ballot(in[r]));
+    } else {
+#pragma unroll
+      for (int r = 0; r < kWinItems; ++r) {
+        const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
+        bool keep = in[r];
+        if (win.use_window) {
+          const float ax = dot3(vp.m + 0, vx, vy, vz), ay = dot3(vp.m + 3, vx, vy, vz);
+          keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
+        }
+        if (win.use_depth) {
+          const float az = dot3(vp.m + 6, vx, vy, vz);
+          keep = keep && !(az < 0.0f || az > vp.depth);
+        }
+        c += __popcll(__ballot(keep));
       }
-      c += __popcll(__ballot(keep));
     }
     const int slot = k & 1;
     if (lane == 0) wc[slot][wave] = c;
@@ -450,28 +456,32 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     uint32_t* __restrict__ err) {
 #pragma clang fp contract(off)
   __shared__ __align__(16) uint32_t wc[2][kWinItems][4];
+  // the tile's positions, read once (SoA)
+  __shared__ float sx[kWinTile], sy[kWinTile], sz[kWinTile];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
-  float px[kWinItems], py[kWinItems], pz[kWinItems];
-  bool in[kWinItems];
 #pragma unroll
   for (int r = 0; r < kWinItems; ++r) {
     const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
-    in[r] = i < ps.n;
-    px[r] = in[r] ? ps.pos3[3 * i] : 0.0f;
-    py[r] = in[r] ? ps.pos3[3 * i + 1] : 0.0f;
-    pz[r] = in[r] ? ps.pos3[3 * i + 2] : 0.0f;
+    const bool ok = i < ps.n;
+    sx[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i] : 0.0f;
+    sy[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i + 1] : 0.0f;
+    sz[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i + 2] : 0.0f;
   }
-  auto kept = [](const view_proj& vp, const window_f& win, float x, float y, float z, bool ok,
-                 float* ax, float* ay, float* az) {
+  __syncthreads();
+  // the filter computes only the coordinates it tests (the flags are
+  // uniform); the kept path recomputes all three
+  auto kept = [](const view_proj& vp, const window_f& win, float x, float y, float z, bool ok) {
     const float vx = x - vp.c[0], vy = y - vp.c[1], vz = z - vp.c[2];
-    *ax = dot3(vp.m + 0, vx, vy, vz);
-    *ay = dot3(vp.m + 3, vx, vy, vz);
-    *az = dot3(vp.m + 6, vx, vy, vz);
     bool keep = ok;
-    if (win.use_window)
-      keep = keep && *ax >= win.x0 && *ax <= win.x1 && *ay >= win.y0 && *ay <= win.y1;
-    if (win.use_depth) keep = keep && !(*az < 0.0f || *az > vp.depth);
+    if (win.use_window) {
+      const float ax = dot3(vp.m + 0, vx, vy, vz), ay = dot3(vp.m + 3, vx, vy, vz);
+      keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
+    }
+    if (win.use_depth) {
+      const float az = dot3(vp.m + 6, vx, vy, vz);
+      keep = keep && !(az < 0.0f || az > vp.depth);
+    }
     return keep;
   };
   for (int k = 0; k < nviews; ++k) {
@@ -479,10 +489,13 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     const view_proj vp = vd[v].vp;
     const window_f win = vd[v].win;
     const int slot = k & 1;
+    uint32_t bits = 0;  // this thread's kept rows
 #pragma unroll
     for (int r = 0; r < kWinItems; ++r) {
-      float ax, ay, az;
-      const bool keep = kept(vp, win, px[r], py[r], pz[r], in[r], &ax, &ay, &az);
+      const int t = r * kBlock + threadIdx.x;
+      const bool ok = base + uint64_t(t) < ps.n;
+      const bool keep = kept(vp, win, sx[t], sy[t], sz[t], ok);
+      bits |= keep ? 1u << r : 0u;
       const uint32_t c = __popcll(__ballot(keep));
       if (lane == 0) wc[slot][r][wave] = c;
     }
@@ -495,14 +508,15 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
       if (tot == 0) continue;
       const uint32_t before = (wave > 0 ? cw.x : 0u) + (wave > 1 ? cw.y : 0u) +
                               (wave > 2 ? cw.z : 0u);
-      const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
-      const bool ok = i < ps.n;
-      const float x = ok ? ps.pos3[3 * i] : 0.0f, y = ok ? ps.pos3[3 * i + 1] : 0.0f,
-                  z = ok ? ps.pos3[3 * i + 2] : 0.0f;
-      float ax, ay, az;
-      const bool keep = kept(vp, win, x, y, z, ok, &ax, &ay, &az);
+      const int t = r * kBlock + threadIdx.x;
+      const uint64_t i = base + uint64_t(t);
+      const bool keep = (bits >> r) & 1u;
       const uint64_t bal = __ballot(keep);
       if (keep) {
+        const float vx = sx[t] - vp.c[0], vy = sy[t] - vp.c[1], vz = sz[t] - vp.c[2];
+        const float ax = dot3(vp.m + 0, vx, vy, vz);
+        const float ay = dot3(vp.m + 3, vx, vy, vz);
+        const float az = dot3(vp.m + 6, vx, vy, vz);
         const grid_params& g = vd[v].g;
         const uint32_t j = out + before +
                            __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
