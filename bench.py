@@ -43,7 +43,7 @@ FLOP_PER_EVAL = 12
 # HBM bytes per k_gather_batch launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs of this workload scaled by a 1 GiB copy (profiles/k_gather_batch_xray_16M_64views.md);
 # keyed by (workload, mode, tile)
-MEASURED_TRAFFIC = {("c3", "reference", (16, 16)): dict(read=12.95e9, write=0.724e9)}
+MEASURED_TRAFFIC = {("c3", "reference", (16, 16)): dict(read=21.33e9, write=1.784e9)}
 
 
 def log(msg):
