@@ -93,6 +93,7 @@ SIGNATURES = {
     "ilc_camera_generate_pixel_coordinates": (_ST, [ctypes.POINTER(CameraC), _P, _P]),
     "ilc_project_evaluation_points": (_ST, [ctypes.POINTER(CameraC), _P, _U64, _P, _P]),
     "ilc_vector_scale": (_ST, [_P, _U64, ctypes.c_float, _P]),
+    "ilc_particles_set_w": (_ST, [_P, _P, _U64, _P]),
     "ilc_crawl_block": (_ST, [ctypes.POINTER(VolumeC), _P, _U64, _P, ctypes.c_int32, _P, _P,
                               _P, ctypes.POINTER(_U64), _P]),
     "ilc_quantity_xray_flux": (_ST, [_P, _P, _P, _U64, _P, ctypes.c_int32, ctypes.c_int32,

@@ -411,6 +411,14 @@ ilc_status ilc_vector_scale(float* x_dev, uint64_t n, float alpha, void* stream)
   });
 }
 
+ilc_status ilc_particles_set_w(float* particles4_dev, const float* w_dev, uint64_t n,
+                               void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(n == 0 || (particles4_dev && w_dev), "null argument");
+    launch_set_w(reinterpret_cast<float4*>(particles4_dev), w_dev, n, as_stream(stream));
+  });
+}
+
 ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t n,
                            const float* const* columns_dev, int32_t ncolumns,
                            float* particles4_dev, float* const* columns_out_dev,

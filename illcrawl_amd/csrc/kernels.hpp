@@ -223,6 +223,7 @@ void launch_world_pixels(const camera_frame& f, uint32_t nx, uint32_t ny, float4
 void launch_project_points(const camera_frame& f, const float4* in, uint64_t n, float4* out,
                            hipStream_t s);
 void launch_vector_scale(float* x, float alpha, uint64_t n, hipStream_t s);
+void launch_set_w(float4* p4, const float* w, uint64_t n, hipStream_t s);
 void launch_retrieve(const double* acc, uint64_t n, float scale, float* out, hipStream_t s);
 void launch_quantity_scaled(const float* in, double scale, uint64_t n, float* w, hipStream_t s);
 // The reference's spectral X-ray emissivity (2-D Gaunt table on the device,

@@ -108,6 +108,13 @@ __global__ __launch_bounds__(kBlock) void k_project_points(proj_args pa, const f
 }
 
 // vector_scale (math.cl:94-102): x *= alpha in FP32.
+// p4[4 i + 3] = w[i]
+__global__ __launch_bounds__(kBlock) void k_set_w(float4* __restrict__ p4, const float* __restrict__ w,
+                                                   uint64_t n) {
+  const uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+  if (i < n) p4[i].w = w[i];
+}
+
 __global__ __launch_bounds__(kBlock) void k_vector_scale(float* x, float alpha, uint64_t n) {
   const uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
   if (i < n) x[i] *= alpha;
@@ -243,6 +250,12 @@ void launch_project_points(const camera_frame& f, const float4* in, uint64_t n, 
   if (!n) return;
   hipLaunchKernelGGL(k_project_points, dim3(nblocks(n)), dim3(kBlock), 0, s, make_proj(f), in, n,
                      out);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_set_w(float4* p4, const float* w, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_set_w, dim3(nblocks(n)), dim3(kBlock), 0, s, p4, w, n);
   ILC_HIP(hipGetLastError());
 }
 

@@ -13,8 +13,8 @@
 // with two substitutions of types the path does not need:
 //   cl::Buffer   -> illcrawl::device_buffer (HIP device memory, shared ownership
 //                   like cl::Buffer's reference counting)
-//   H5::DataSet  -> illcrawl::dataset_ref (the dataset's name; the data crawler
-//                   that reads HDF5 is not part of this library)
+//   H5::DataSet  -> illcrawl::dataset_ref (the dataset's name; the data
+//                   crawler of illcrawl_crawler.hpp opens it in the snapshot)
 // Errors throw illcrawl::error, a std::runtime_error carrying the ABI status
 // (the reference throws qcl::qcl_error / std::runtime_error, qcl.hpp:79-118).
 // All device work goes to one HIP stream per object (default: the null
@@ -355,6 +355,24 @@ class projective_smoothing : public reconstruction_backend {
           "project_particles");
     _backend->setup_projected_particles(_projected, particles.size(), additional_dataset);
   }
+
+  // The same for particles already on the device (float4 x, y, z, w, world
+  // frame), e.g. a block the data crawler compacted on the device; the
+  // caller's buffer is not modified.
+  void setup_particles(const device_buffer& particles, std::size_t num_particles,
+                       const std::vector<device_buffer>& additional_dataset) {
+    if (particles.count() < num_particles)
+      throw error(ILC_EINVAL, "setup_particles: buffer shorter than num_particles");
+    if (_projected.count() < num_particles)
+      _projected = device_buffer::create<device_vector4>(num_particles);
+    const ilc_camera c = _cam.c_struct(_depth);
+    check(ilc_project_particles(&c, particles.as<float>(), num_particles, _projected.as<float>(),
+                                _stream),
+          "project_particles");
+    _backend->setup_projected_particles(_projected, num_particles, additional_dataset);
+  }
+
+  hipStream_t stream() const { return _stream; }
 
   // world-frame points (e.g. camera::generate_pixel_coordinates); the
   // caller's buffer is not modified

@@ -261,6 +261,11 @@ ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t
                            float* particles4_dev, float* const* columns_out_dev,
                            uint64_t* index_out_dev, uint64_t* n_out, void* stream);
 
+/* Combine particles with their quantity value
+ * (reconstructing_data_crawler.cpp:233-239): particles4_dev[4i+3] = w_dev[i]. */
+ilc_status ilc_particles_set_w(float* particles4_dev, const float* w_dev, uint64_t n,
+                               void* stream);
+
 /* ------------------------------------------------- quantity transformation */
 
 /* The reference's X-ray quantities (quantities.cl:39-239, 354-430): thermal
