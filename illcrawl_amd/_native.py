@@ -93,6 +93,15 @@ SIGNATURES = {
     "ilc_camera_generate_pixel_coordinates": (_ST, [ctypes.POINTER(CameraC), _P, _P]),
     "ilc_project_evaluation_points": (_ST, [ctypes.POINTER(CameraC), _P, _U64, _P, _P]),
     "ilc_vector_scale": (_ST, [_P, _U64, ctypes.c_float, _P]),
+    "ilc_grid3d_create": (_ST, [ctypes.POINTER(PsgConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    "ilc_grid3d_destroy": (_ST, [_P]),
+    "ilc_grid3d_init_backend": (_ST, [_P, _U64]),
+    "ilc_grid3d_setup_particles": (_ST, [_P, _P, _U64, _P, _P]),
+    "ilc_grid3d_setup_evaluation_points": (_ST, [_P, _P, _U64, ctypes.c_uint32, _P]),
+    "ilc_grid3d_run": (_ST, [_P, _P]),
+    "ilc_grid3d_retrieve_results": (_ST, [_P, ctypes.c_float, _P, _P]),
+    "ilc_grid3d_retrieve_results_f64": (_ST, [_P, _P, _P]),
+    "ilc_grid3d_grid_info": (_ST, [_P, ctypes.POINTER(GridInfo)]),
     "ilc_particles_set_w": (_ST, [_P, _P, _U64, _P]),
     "ilc_crawl_block": (_ST, [ctypes.POINTER(VolumeC), _P, _U64, _P, ctypes.c_int32, _P, _P,
                               _P, ctypes.POINTER(_U64), _P]),

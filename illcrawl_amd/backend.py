@@ -155,6 +155,102 @@ class ProjectiveSmoothingGrid:
 
 # ------------------------------------------------------------ camera wrapper
 
+class GridSmoothing:
+    """reconstruction_backends::dm::grid (dm_reconstruction_backend_grid.cpp:57-134):
+    volumetric smoothing of 3-D particles at 3-D points with the 3-D quartic
+    (smoothing_kernel.cl:27-37), e.g. for volumetric slices.  Results
+    accumulate over run() calls until the next setup_evaluation_points."""
+
+    def __init__(self, device=0, target_particles_per_cell=1000, mode="reference",
+                 kernel="quartic_ref"):
+        cfg = N.PsgConfig()
+        cfg.device = int(device)
+        cfg.target_particles_per_cell = int(target_particles_per_cell)
+        cfg.mode = _MODES[mode]
+        cfg.kernel = _KERNELS[kernel]
+        self._h = ctypes.c_void_p()
+        N.call("ilc_grid3d_create", ctypes.byref(cfg), ctypes.byref(self._h))
+        self.device = torch.device("cuda", int(device))
+        self.mode, self.kernel = mode, kernel
+        self._keep = []
+        self._npoints = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                N.lib().ilc_grid3d_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def get_backend_name(self):
+        return "dm_smoothing/grid3d (hip gfx950)"
+
+    def init_backend(self, blocksize):
+        N.call("ilc_grid3d_init_backend", self._h, int(blocksize))
+
+    def setup_particles(self, particles4, additional_datasets, stream=None):
+        """particles4 (n, 4): x, y, z, w; additional_datasets: [h]."""
+        if len(additional_datasets) != 1:
+            raise ValueError("the grid backend takes exactly one additional dataset (h)")
+        p = _dev_f32(particles4, self.device, 4)
+        h = _dev_f32(additional_datasets[0], self.device).reshape(-1)
+        if h.shape[0] != p.shape[0]:
+            raise ValueError("one smoothing length per particle is required")
+        N.call("ilc_grid3d_setup_particles", self._h, _ptr(p), p.shape[0], _ptr(h),
+               _stream(stream))
+
+    def setup_evaluation_points(self, points4, layout_nx=0, stream=None):
+        pts = _dev_f32(points4, self.device, 4)
+        self._keep = [pts]
+        self._npoints = pts.shape[0]
+        N.call("ilc_grid3d_setup_evaluation_points", self._h, _ptr(pts), pts.shape[0],
+               int(layout_nx), _stream(stream))
+
+    def run(self, stream=None):
+        N.call("ilc_grid3d_run", self._h, _stream(stream))
+
+    def retrieve_results(self, scale=1.0, stream=None):
+        out = torch.empty(self._npoints, dtype=torch.float32, device=self.device)
+        N.call("ilc_grid3d_retrieve_results", self._h, ctypes.c_float(scale), _ptr(out),
+               _stream(stream))
+        return out
+
+    def retrieve_results_f64(self, stream=None):
+        out = torch.empty(self._npoints, dtype=torch.float64, device=self.device)
+        N.call("ilc_grid3d_retrieve_results_f64", self._h, _ptr(out), _stream(stream))
+        return out
+
+    def grid_info(self):
+        gi = N.GridInfo()
+        N.call("ilc_grid3d_grid_info", self._h, ctypes.byref(gi))
+        return dict(num_cells=list(gi.num_cells), min_corner=list(gi.min_corner),
+                    cell_size=list(gi.cell_size), max_h=gi.max_h,
+                    num_particles=gi.num_particles)
+
+
+def slice_points(center, normal, width, nx, ny=None, up=(0.0, 0.0, 1.0)):
+    """Pixel centres of a square slice of the volume (volumetric_slice.cpp):
+    nx x ny points, row-major, on the plane through ``center`` normal to
+    ``normal``, ``width`` wide.  Returns (nx*ny, 4) float32 (w = 0)."""
+    ny = nx if ny is None else ny
+    n = np.asarray(normal, np.float64)
+    n = n / np.linalg.norm(n)
+    u = np.cross(np.asarray(up, np.float64), n)
+    if np.linalg.norm(u) < 1e-12:
+        u = np.cross(np.array([0.0, 1.0, 0.0]), n)
+    u /= np.linalg.norm(u)
+    v = np.cross(n, u)
+    ps = float(width) / nx
+    ix = (np.arange(nx) + 0.5 - 0.5 * nx) * ps
+    iy = (np.arange(ny) + 0.5 - 0.5 * ny) * ps
+    X, Y = np.meshgrid(ix, iy)
+    pts = (np.asarray(center, np.float64)[None, :] + X.reshape(-1, 1) * u[None, :]
+           + Y.reshape(-1, 1) * v[None, :])
+    return np.concatenate([pts, np.zeros((pts.shape[0], 1))], 1).astype(np.float32)
+
+
 def camera_basis(cam):
     c = N.camera_struct(cam)
     b0, b1, mp = (ctypes.c_double * 3)(), (ctypes.c_double * 3)(), (ctypes.c_double * 3)()

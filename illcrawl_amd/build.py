@@ -26,7 +26,7 @@ CRAWL = os.path.join(LIBDIR, "crawl_demo")
 IO_PREFIX = os.environ.get("ILC_IO_PREFIX", "/opt/conda")  # HDF5 C++ and cfitsio
 
 SOURCES = ["host.cpp", "gridbuild.cpp", "capi.cpp", "batch.cpp", "crawl.hip", "grid.hip",
-           "gather.hip", "xray.hip",
+           "gather.hip", "xray.hip", "smooth3d.hip",
            "project.hip", "batch.hip"]
 
 ARCH = "gfx950"

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
@@ -50,6 +51,11 @@ struct ilc_psg {
   }
 };
 
+// dm::grid: the volumetric smoothing shares the single-view state
+struct ilc_grid3d {
+  ilc_psg p;
+};
+
 namespace {
 
 template <class F>
@@ -83,6 +89,14 @@ struct device_guard {
     if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
   }
 };
+
+#define ILC_GUARD_NULL(h)                       \
+  do {                                          \
+    if (!(h)) {                                 \
+      set_last_error("null handle");            \
+      return ILC_EINVAL;                        \
+    }                                           \
+  } while (0)
 
 void check_handle(const ilc_psg* h) { ILC_REQUIRE(h != nullptr, "null handle"); }
 
@@ -409,6 +423,127 @@ ilc_status ilc_vector_scale(float* x_dev, uint64_t n, float alpha, void* stream)
     ILC_REQUIRE(x_dev || n == 0, "null argument");
     launch_vector_scale(x_dev, alpha, n, as_stream(stream));
   });
+}
+
+// ------------------------------------------------ volumetric smoothing (dm::grid)
+
+ilc_status ilc_grid3d_create(const ilc_psg_config* cfg, ilc_grid3d** out) {
+  return guarded([&] {
+    ILC_REQUIRE(out != nullptr, "out is null");
+    *out = nullptr;
+    ilc_psg_config c{};
+    if (cfg) c = *cfg;
+    ILC_REQUIRE(c.mode == ILC_MODE_REFERENCE || c.mode == ILC_MODE_EXACT, "bad mode %d", c.mode);
+    ILC_REQUIRE(c.kernel == ILC_KERNEL_QUARTIC_REF || c.kernel == ILC_KERNEL_QUARTIC,
+                "the volumetric smoothing uses the quartic kernel (got %d)", c.kernel);
+    int ndev = 0;
+    ILC_HIP(hipGetDeviceCount(&ndev));
+    ILC_REQUIRE(c.device >= 0 && c.device < ndev, "device %d not present (%d devices)", c.device,
+                ndev);
+    std::unique_ptr<ilc_grid3d> h(new ilc_grid3d());
+    h->p.device = c.device;
+    h->p.target = c.target_particles_per_cell ? c.target_particles_per_cell : 1000;
+    h->p.mode = c.mode;
+    h->p.kernel = c.kernel;
+    h->p.ts = tile_shape{16, 1};  // the point layout's default width
+    *out = h.release();
+  });
+}
+
+ilc_status ilc_grid3d_destroy(ilc_grid3d* h) {
+  return guarded([&] {
+    if (!h) return;
+    device_guard dg(h->p.device);
+    delete h;
+  });
+}
+
+ilc_status ilc_grid3d_init_backend(ilc_grid3d* h, uint64_t blocksize) {
+  ILC_GUARD_NULL(h);
+  return ilc_psg_init_backend(&h->p, blocksize);
+}
+
+ilc_status ilc_grid3d_setup_particles(ilc_grid3d* g3, const float* particles4_dev, uint64_t n,
+                                      const float* hsml_dev, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(g3 != nullptr, "null handle");
+    ilc_psg* h = &g3->p;
+    if (h->blocksize == 0) fail(ILC_ENOTREADY, "init_backend was not called");
+    ILC_REQUIRE(n > 0, "no particles");
+    ILC_REQUIRE(n <= h->blocksize, "%llu particles exceed the blocksize %llu",
+                (unsigned long long)n, (unsigned long long)h->blocksize);
+    ILC_REQUIRE(n < (uint64_t(1) << 31), "block of %llu particles exceeds 2^31",
+                (unsigned long long)n);
+    ILC_REQUIRE(particles4_dev && hsml_dev, "null particle or smoothing-length pointer");
+    device_guard dg(h->device);
+    hipStream_t s = as_stream(stream);
+    const float4* p4 = reinterpret_cast<const float4*>(particles4_dev);
+    if (!h->bbox_host) {
+      ILC_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->bbox_host), sizeof(bbox_result)));
+      ILC_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->flags_host), 2 * sizeof(uint32_t)));
+    }
+    grid_scratch& sc = h->scratch;
+    // bounding box (particle_grid.cpp:43-67), then the grid parameters
+    const int nparts = static_cast<int>(std::min<uint64_t>(1024, (n + kBlock - 1) / kBlock));
+    sc.bbox_partial.ensure(size_t(nparts) * 8);
+    sc.bbox_dev.ensure(1);
+    sc.flags.ensure(2);
+    launch_bbox(p4, hsml_dev, n, sc.bbox_partial.p, nparts, sc.flags.p, sc.bbox_dev.p, s);
+    ILC_HIP(hipMemcpyAsync(h->bbox_host, sc.bbox_dev.p, sizeof(bbox_result),
+                           hipMemcpyDeviceToHost, s));
+    ILC_HIP(hipStreamSynchronize(s));
+    const bbox_result bb = *h->bbox_host;
+    if (bb.bad)
+      fail(ILC_EINVAL, "%u particles have non-finite coordinates/w or h outside [1.1e-19, 1.8e19]",
+           bb.bad);
+    const grid_params g = compute_grid_params(bb, n, h->target);
+    // the masked list packs a cell's x and y into 16 bits each
+    if (h->mode == ILC_MODE_REFERENCE && (g.nc[0] > 65536 || g.nc[1] > 65536))
+      fail(ILC_ERANGE, "grid of %d x %d cells in x, y (at most 65536 each)", g.nc[0], g.nc[1]);
+    launch_fill_u32(sc.flags.p, 0u, 1, s);
+    enqueue_grid_build(h->grid, sc, p4, hsml_dev, n, g, 0, bb.max_h, h->kernel, sc.flags.p, s,
+                       /*volumetric=*/true);
+    ILC_HIP(hipMemcpyAsync(h->flags_host, sc.flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    ILC_HIP(hipStreamSynchronize(s));
+    if (h->flags_host[0]) {
+      h->grid.ready = false;
+      fail(ILC_ERANGE, "%u particles fall outside the grid", h->flags_host[0]);
+    }
+  });
+}
+
+ilc_status ilc_grid3d_setup_evaluation_points(ilc_grid3d* h, const float* points4_dev,
+                                              uint64_t n, uint32_t layout_nx, void* stream) {
+  ILC_GUARD_NULL(h);
+  return ilc_psg_setup_evaluation_points(&h->p, points4_dev, n, layout_nx, stream);
+}
+
+ilc_status ilc_grid3d_run(ilc_grid3d* g3, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(g3 != nullptr, "null handle");
+    ilc_psg* h = &g3->p;
+    if (!h->grid.ready) fail(ILC_ENOTREADY, "run before setup_particles");
+    if (!h->pts) fail(ILC_ENOTREADY, "run before setup_evaluation_points");
+    device_guard dg(h->device);
+    point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
+    launch_smooth3d(h->grid.view(), p, h->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT,
+                    h->acc.p, as_stream(stream));
+  });
+}
+
+ilc_status ilc_grid3d_retrieve_results(ilc_grid3d* h, float scale, float* out_dev, void* stream) {
+  ILC_GUARD_NULL(h);
+  return ilc_psg_retrieve_results(&h->p, scale, out_dev, stream);
+}
+
+ilc_status ilc_grid3d_retrieve_results_f64(ilc_grid3d* h, double* out_dev, void* stream) {
+  ILC_GUARD_NULL(h);
+  return ilc_psg_retrieve_results_f64(&h->p, out_dev, stream);
+}
+
+ilc_status ilc_grid3d_grid_info(const ilc_grid3d* h, ilc_grid_info* out) {
+  ILC_GUARD_NULL(h);
+  return ilc_psg_grid_info(&h->p, out);
 }
 
 ilc_status ilc_particles_set_w(float* particles4_dev, const float* w_dev, uint64_t n,

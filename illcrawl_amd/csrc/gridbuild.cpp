@@ -27,6 +27,11 @@ float kernel_norm(int kernel) {
   }
 }
 
+float kernel_norm_3d(int kernel) {
+  // smoothing_kernel.cl:27-37: norm = 105 / (32 pi h^3)
+  return kernel == ILC_KERNEL_QUARTIC_REF ? 1.04445f : static_cast<float>(105.0 / (32.0 * M_PI));
+}
+
 grid_view grid_store::view() const {
   grid_view g;
   g.part = part.p;
@@ -43,6 +48,7 @@ grid_view grid_store::view() const {
   g.cell_radius = cell_radius;
   g.kz0 = kz0;
   g.ncells = static_cast<uint32_t>(gp.ncells());
+  g.pz = volumetric ? pz.p : nullptr;
   const uint64_t dn = density_n ? density_n : n;
   g.sparse = dn * 5 < uint64_t(g.ncells) * 2 ? 1 : 0;
   return g;
@@ -65,16 +71,16 @@ grid_params plan_grid(const bbox_result& bb, uint64_t n, uint64_t target, int mo
 
 void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                         uint64_t n, const grid_params& g, int32_t kz0, float max_h, int kernel,
-                        uint32_t* err, hipStream_t s) {
+                        uint32_t* err, hipStream_t s, bool volumetric) {
   gs.key.ensure(n);
   gs.perm.ensure(n);
   launch_keys(p4, uint32_t(n), g, gs.key.p, gs.perm.p, err, s);
-  enqueue_grid_from_keys(gs, sc, p4, h, n, g, kz0, max_h, kernel, s);
+  enqueue_grid_from_keys(gs, sc, p4, h, n, g, kz0, max_h, kernel, s, volumetric);
 }
 
 void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                             uint64_t n, const grid_params& g, int32_t kz0, float max_h,
-                            int kernel, hipStream_t s) {
+                            int kernel, hipStream_t s, bool volumetric) {
   const uint64_t ncells = g.ncells();
   gs.ready = false;
   gs.key.ensure(n);
@@ -90,8 +96,11 @@ void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, 
   launch_radix_sort(gs.key.p, gs.perm.p, uint32_t(n), key_bits_for(ncells - 1), sc.temp.p, s);
   launch_fill_u32(sc.counts.p, 0u, ncells + 1, s);
   launch_fill_u32(gs.maxh_bits.p, 0u, ncells, s);
-  launch_build_soa(p4, h, gs.perm.p, gs.key.p, uint32_t(n), kernel_norm(kernel), gs.part.p,
-                   gs.hs.p, sc.counts.p, gs.maxh_bits.p, s);
+  if (volumetric) gs.pz.ensure(n);
+  gs.volumetric = volumetric;
+  launch_build_soa(p4, h, gs.perm.p, gs.key.p, uint32_t(n),
+                   volumetric ? kernel_norm_3d(kernel) : kernel_norm(kernel), gs.part.p, gs.hs.p,
+                   sc.counts.p, gs.maxh_bits.p, s, volumetric ? gs.pz.p : nullptr);
   launch_exclusive_scan(sc.counts.p, gs.start.p, ncells + 1, sc.temp.p, s);
   gs.gp = g;
   gs.n = n;

@@ -19,6 +19,7 @@ struct grid_store {
   dbuf<uint32_t> perm;        // sorted position -> input index
   dbuf<uint32_t> start;       // ncells + 1
   dbuf<uint32_t> maxh_bits;   // ncells
+  dbuf<float> pz;             // volumetric grids: sorted z
   grid_params gp{};
   float max_h = 0.f, cell_radius = 0.f;
   int32_t kz0 = 0;
@@ -27,6 +28,7 @@ struct grid_store {
   // 0 = n (a windowed grid sets the count of the full set it stands for, so
   // that it is enumerated like the full grid)
   uint64_t density_n = 0;
+  bool volumetric = false;  // 3-D smoothing: part.z holds w K / h^3, z in pz
   bool ready = false;
   grid_view view() const;
 };
@@ -46,17 +48,21 @@ grid_params plan_grid(const bbox_result& bb, uint64_t n, uint64_t target, int mo
 // Enqueue keys, stable sort, sorted SoA, per-cell counts / max h and cell
 // offsets for n float4 particles p4 with smoothing lengths h.  Out-of-grid
 // keys are counted into err (device uint32, not cleared here).
+// volumetric: the grid of the 3-D smoothing (z kept, coef = w K / h^3)
 void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                         uint64_t n, const grid_params& g, int32_t kz0, float max_h, int kernel,
-                        uint32_t* err, hipStream_t s);
+                        uint32_t* err, hipStream_t s, bool volumetric = false);
 
 // The same from keys already in gs.key (cell keys) and gs.perm (indices into
 // p4 / h), n of them, in the order the within-cell order must follow.
 void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                             uint64_t n, const grid_params& g, int32_t kz0, float max_h,
-                            int kernel, hipStream_t s);
+                            int kernel, hipStream_t s, bool volumetric = false);
 
 float kernel_norm(int kernel);
+// the 3-D quartic's normalisation: the reference's 1.04445f (ILC_KERNEL_QUARTIC_REF)
+// or 105 / (32 pi) (ILC_KERNEL_QUARTIC)
+float kernel_norm_3d(int kernel);
 
 // Bits of a radix key that holds values up to maxval.
 int key_bits_for(uint64_t maxval);

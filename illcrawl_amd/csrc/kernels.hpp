@@ -21,9 +21,11 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32
 uint64_t radix_temp_words(uint64_t n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
                        hipStream_t s);
+// pz (may be NULL): volumetric smoothing keeps z there and sets
+// coef = w norm / h^3 (projected: coef = w norm / h^2)
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
                       const uint32_t* skeys, uint32_t n, float norm, float4* part, float* hs,
-                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s);
+                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s, float* pz = nullptr);
 // Reorder (key, perm) pairs, n of them, by descending h[perm] (top 16 bits,
 // stable); horder holds 2n words, temp radix_temp_words(n).
 void launch_h_order(const float* h, uint32_t* key, uint32_t* perm, uint32_t n, uint32_t* horder,
@@ -156,6 +158,7 @@ struct grid_view {
   float cell_radius;
   int32_t kz0;  // z layer of the particles (single layer in reference mode)
   uint32_t ncells;
+  const float* pz;  // volumetric smoothing: sorted z (NULL for projected particles)
 };
 
 // Evaluation points laid out as an image (lnx x lny, row-major).
@@ -225,6 +228,9 @@ void launch_project_points(const camera_frame& f, const float4* in, uint64_t n, 
 void launch_vector_scale(float* x, float alpha, uint64_t n, hipStream_t s);
 void launch_set_w(float4* p4, const float* w, uint64_t n, hipStream_t s);
 void launch_retrieve(const double* acc, uint64_t n, float scale, float* out, hipStream_t s);
+
+// ---- smooth3d.hip: the volumetric smoothing gather (grid with g.pz set)
+void launch_smooth3d(const grid_view& g, const point_view& p, int op, double* acc, hipStream_t s);
 void launch_quantity_scaled(const float* in, double scale, uint64_t n, float* w, hipStream_t s);
 // The reference's spectral X-ray emissivity (2-D Gaunt table on the device,
 // t[i + nx j], i along ln(gamma^2) from -6, j along ln(u) from -16, step 0.2)

@@ -261,6 +261,38 @@ ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t
                            float* particles4_dev, float* const* columns_out_dev,
                            uint64_t* index_out_dev, uint64_t* n_out, void* stream);
 
+/* ------------------------------------------- volumetric smoothing (dm::grid) */
+
+/* reconstruction_backends::dm::grid (dm_reconstruction_backend_grid.hpp:32-60,
+ * dm_reconstruction_backend_grid.cpp:57-134): particles keep their 3-D
+ * position; every evaluation point (x, y, z) gets the sum of
+ * w * quartic_polynomial3d(|x_p - x|, h) (smoothing_kernel.cl:27-37) over the
+ * particles of the cells in its search box (smoothing_particle_grid.cl:31-127
+ * without SPG_PROJECTION), e.g. a volumetric slice (volumetric_slice.cpp).
+ * The configuration is ilc_psg_config with a quartic kernel
+ * (ILC_KERNEL_QUARTIC_REF: the reference's 1.04445f normalisation;
+ * ILC_KERNEL_QUARTIC: 105 / (32 pi)); reference mode keeps the reference's
+ * search box, exact mode sums every particle whose support contains the
+ * point.  Results accumulate over run() calls until the next
+ * setup_evaluation_points, as for ilc_psg. */
+typedef struct ilc_grid3d ilc_grid3d;
+ilc_status ilc_grid3d_create(const ilc_psg_config* cfg, ilc_grid3d** out);
+ilc_status ilc_grid3d_destroy(ilc_grid3d* h);
+ilc_status ilc_grid3d_init_backend(ilc_grid3d* h, uint64_t blocksize);
+/* dm::grid::setup_particles (dm_reconstruction_backend_grid.cpp:63-85):
+ * particles4_dev (x, y, z, w) world frame, hsml_dev one h per particle;
+ * builds the 3-D grid (particle_grid.cpp:43-222), synchronises. */
+ilc_status ilc_grid3d_setup_particles(ilc_grid3d* h, const float* particles4_dev, uint64_t n,
+                                      const float* hsml_dev, void* stream);
+/* points4_dev (x, y, z, -); layout_nx: width of the point image the tiles
+ * follow (0: 16). */
+ilc_status ilc_grid3d_setup_evaluation_points(ilc_grid3d* h, const float* points4_dev,
+                                              uint64_t n, uint32_t layout_nx, void* stream);
+ilc_status ilc_grid3d_run(ilc_grid3d* h, void* stream);
+ilc_status ilc_grid3d_retrieve_results(ilc_grid3d* h, float scale, float* out_dev, void* stream);
+ilc_status ilc_grid3d_retrieve_results_f64(ilc_grid3d* h, double* out_dev, void* stream);
+ilc_status ilc_grid3d_grid_info(const ilc_grid3d* h, ilc_grid_info* out);
+
 /* Combine particles with their quantity value
  * (reconstructing_data_crawler.cpp:233-239): particles4_dev[4i+3] = w_dev[i]. */
 ilc_status ilc_particles_set_w(float* particles4_dev, const float* w_dev, uint64_t n,

@@ -49,6 +49,11 @@ PINNING = {
     "orc_tabulated_2d": "hand-computed bilinear half-texel answers",
     "orc_temperature": "closed form (gamma-1) u mu 121.1475502",
     "orc_xray_bolometric": "closed form on table-node-free inputs",
+    "orc_quartic3d": "W(0) = 1.04445/h^3, W(r >= h) = 0, 4 pi int r^2 W dr = 1.04445 * 32 pi / 105",
+    "orc_smooth3d_exact": "single-particle closed form; lattice volume integral = w",
+    "orc_smooth3d_ref": "uniform-h scenes where the search box drops nothing equal "
+                        "orc_smooth3d_exact up to the 1.04445f constant; a hand-built scene "
+                        "where the search box drops a known particle",
     "orc_backward_exact": "FP64 central differences of orc_gather_exact in w, h, x, y "
                           "(quartic_ref, quartic, cubic)",
     "orc_xray_flux": "constant Gaunt table: band integral of exp(-E/kT) in closed form "
@@ -115,6 +120,14 @@ def _declare(L):
                                  ctypes.c_float, _f32p, _u64, ctypes.c_int, _f64p,
                                  ctypes.c_void_p]
     L.orc_gather_ref.restype = None
+    L.orc_quartic3d.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.orc_quartic3d.restype = ctypes.c_float
+    L.orc_smooth3d_ref.argtypes = [_i32p, _f32p, _f32p, _i32p, _f32p, _f32p, _f32p,
+                                   ctypes.c_float, _f32p, _u64, ctypes.c_int, ctypes.c_int,
+                                   _f64p]
+    L.orc_smooth3d_ref.restype = None
+    L.orc_smooth3d_exact.argtypes = [_f32p, _f32p, _u64, _f32p, _u64, _f64p]
+    L.orc_smooth3d_exact.restype = None
     L.orc_gather_exact.argtypes = [_f32p, _f32p, _u64, _f32p, _u64, ctypes.c_int, _f64p,
                                    ctypes.c_void_p]
     L.orc_gather_exact.restype = None
@@ -258,6 +271,31 @@ class Grid:
                              pts4.shape[0], int(acc_mode), out,
                              pairs.ctypes.data if pairs is not None else None)
         return (out, pairs) if with_pairs else out
+
+
+    def smooth3d_ref(self, pts4, acc_mode=1, form=0):
+        """Volumetric smoothing with the reference's search (dm::grid).
+        form 0: the reference's FP32 kernel expression; 1: the same kernel
+        in FP64 from the stable form (1 - r^2/h^2)^2."""
+        pts4 = _c(pts4, np.float32)
+        out = np.zeros(pts4.shape[0], np.float64)
+        lib().orc_smooth3d_ref(self.nc, self.minc, self.cs, self.cells, self.sorted_p4,
+                               self.sorted_h, self.maxh_cell, np.float32(self.max_h), pts4,
+                               pts4.shape[0], int(acc_mode), int(form), out)
+        return out
+
+
+def smooth3d_exact(p4, h, pts4):
+    p4 = _c(p4, np.float32)
+    h = _c(h, np.float32)
+    pts4 = _c(pts4, np.float32)
+    out = np.zeros(pts4.shape[0], np.float64)
+    lib().orc_smooth3d_exact(p4, h, p4.shape[0], pts4, pts4.shape[0], out)
+    return out
+
+
+def quartic3d(r, h):
+    return lib().orc_quartic3d(np.float32(r), np.float32(h))
 
 
 # ---------------------------------------------------------------- exact

@@ -454,6 +454,120 @@ void orc_gather_ref(const int32_t nc[3], const float minc[3], const float cs[3],
 }
 
 /* ------------------------------------------------------------------ */
+/* Volumetric smoothing (dm::grid): smoothing_kernel.cl:27-37 and       */
+/* spg_run_reconstruction without SPG_PROJECTION                        */
+/* ------------------------------------------------------------------ */
+static float f_quartic3d(float r, float h)
+{
+  float h_inv = 1.f / h;
+  float norm = 1.04445f * h_inv;
+  norm = norm * h_inv;
+  norm = norm * h_inv;
+  float q = fminf(r * h_inv, 1.f);
+  float q2 = q * q;
+  float q4 = q2 * q2;
+  float a = 1.f + q4;
+  a = a - 2.f * q2;
+  return norm * a;
+}
+
+float orc_quartic3d(float r, float h) { return f_quartic3d(r, h); }
+
+/* The 3-D grid reconstruction with the reference's search: as
+ * orc_gather_ref, with the particle term w * quartic_polynomial3d(r, h) of
+ * the 3-D distance r = distance(particle.xyz, point.xyz).
+ * form 0: the term as the reference evaluates it (FP32, 1 + q^4 - 2 q^2);
+ * form 1: the same kernel, 1.04445 / h^3 (1 - r^2/h^2)^2, in FP64 (the
+ * reference's expression cancels near r = h). */
+void orc_smooth3d_ref(const int32_t nc[3], const float minc[3], const float cs[3],
+                      const int32_t* cells2, const float* sorted_p4, const float* sorted_h,
+                      const float* maxh_cell, float max_h,
+                      const float* pts4, uint64_t npts, int acc_mode, int form, double* out)
+{
+  const uint64_t total = (uint64_t)nc[0] * (uint64_t)nc[1] * (uint64_t)nc[2];
+  float dd = cs[0] * cs[0];
+  dd = dd + cs[1] * cs[1];
+  dd = dd + cs[2] * cs[2];
+  const float cell_radius = 0.5f * sqrtf(dd);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t t = 0; t < (int64_t)npts; ++t) {
+    const float* ep = pts4 + 4 * t;
+    float res32 = 0.0f;
+    double res64 = 0.0;
+    int32_t ec[3];
+    f_cell_indices(minc, cs, ep, ec);
+    uint64_t ekey = cell_key(nc, ec);
+    float R = max_h;
+    if (ekey < total)
+      R = fminf(maxh_cell[ekey] + cell_radius, max_h);
+    float lo_pt[3] = {ep[0] - R, ep[1] - R, ep[2] - R};
+    float hi_pt[3] = {ep[0] + R, ep[1] + R, ep[2] + R};
+    int32_t lo[3], hi[3];
+    f_cell_indices(minc, cs, lo_pt, lo);
+    f_cell_indices(minc, cs, hi_pt, hi);
+    for (int64_t cz = lo[2]; cz <= hi[2]; ++cz)
+      for (int64_t cy = lo[1]; cy <= hi[1]; ++cy)
+        for (int64_t cx = lo[0]; cx <= hi[0]; ++cx) {
+          if (!(0 <= cx && cx < nc[0] && 0 <= cy && cy < nc[1] && 0 <= cz && cz < nc[2]))
+            continue;
+          int32_t c[3] = {(int32_t)cx, (int32_t)cy, (int32_t)cz};
+          uint64_t ck = cell_key(nc, c);
+          int32_t b = cells2[2 * ck], e = cells2[2 * ck + 1];
+          float cell_cutoff = maxh_cell[ck];
+          float center[3];
+          for (int i = 0; i < 3; ++i) {
+            float m = minc[i] + (float)c[i] * cs[i];
+            center[i] = m + 0.5f * cs[i];
+          }
+          float d[3] = {ep[0] - center[0], ep[1] - center[1], ep[2] - center[2]};
+          float dist = sqrtf(f_dot3(d, d));
+          if (!(dist < cell_cutoff + cell_radius))
+            continue;
+          for (int32_t i = b; i < e; ++i) {
+            float hs = sorted_h[i];
+            const float* p = sorted_p4 + 4 * (int64_t)i;
+            if (form == 1) {
+              double dx = (double)p[0] - ep[0], dy = (double)p[1] - ep[1];
+              double dz = (double)p[2] - ep[2];
+              double hh = (double)hs;
+              double s = 1.0 - (dx * dx + dy * dy + dz * dz) / (hh * hh);
+              double term = s > 0.0 ? (double)p[3] * (double)1.04445f / (hh * hh * hh) * s * s : 0.0;
+              res32 += (float)term;
+              res64 += term;
+            } else {
+              float dv[3] = {p[0] - ep[0], p[1] - ep[1], p[2] - ep[2]};
+              float r = sqrtf(f_dot3(dv, dv));
+              float term = p[3] * f_quartic3d(r, hs);
+              res32 += term;
+              res64 += (double)term;
+            }
+          }
+        }
+    out[t] = acc_mode == 0 ? (double)res32 : res64;
+  }
+}
+
+/* Every particle whose support contains the point, in FP64:
+ * w * 105 / (32 pi h^3) (1 - r^2/h^2)^2. */
+void orc_smooth3d_exact(const float* p4, const float* h, uint64_t n, const float* pts4,
+                        uint64_t npts, double* out)
+{
+  const double K = 105.0 / (32.0 * M_PI);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t t = 0; t < (int64_t)npts; ++t) {
+    double acc = 0.0;
+    for (uint64_t j = 0; j < n; ++j) {
+      double dx = (double)p4[4 * j] - pts4[4 * t], dy = (double)p4[4 * j + 1] - pts4[4 * t + 1];
+      double dz = (double)p4[4 * j + 2] - pts4[4 * t + 2];
+      double hh = (double)h[j];
+      double s = 1.0 - (dx * dx + dy * dy + dz * dz) / (hh * hh);
+      if (s > 0.0) acc += (double)p4[4 * j + 3] * K / (hh * hh * hh) * s * s;
+    }
+    out[t] = acc;
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* FP64 exact oracle                                                   */
 /* ------------------------------------------------------------------ */
 

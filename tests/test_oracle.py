@@ -513,3 +513,84 @@ def test_xray_flux_band_integral(orc):
         errs.append(float(np.max(np.abs(got - want) / want)))
         steps_mean.append(float(steps.mean()))
     assert errs[0] < 5e-3 and errs[1] < 5e-3
+
+
+# ------------------------------------------------- volumetric smoothing (dm::grid)
+
+def test_quartic3d_closed_forms(orc):
+    """quartic_polynomial3d (smoothing_kernel.cl:27-37): W(0) = 1.04445/h^3,
+    W = 0 from r = h on, and 4 pi int_0^h r^2 W dr = 1.04445 * 32 pi / 105
+    (the shape (1 - q^2)^2 integrates to 32 pi / 105 h^3)."""
+    h = 1.7
+    assert orc.quartic3d(0.0, h) == pytest.approx(1.04445 / h ** 3, rel=1e-6)
+    assert orc.quartic3d(h, h) == 0.0 and orc.quartic3d(1.3 * h, h) == 0.0
+    r = (np.arange(200000) + 0.5) * (h / 200000)
+    w = np.array([orc.quartic3d(x, h) for x in r[::100]])
+    integral = float(np.sum(4.0 * np.pi * r[::100] ** 2 * w) * (h / 2000))
+    assert integral == pytest.approx(1.04445 * 32.0 * np.pi / 105.0, rel=2e-5)
+    # a dropped q^4 or a wrong sign on 2 q^2 breaks the half-radius value
+    q = 0.5
+    assert orc.quartic3d(q * h, h) == pytest.approx(1.04445 / h ** 3 * (1 - q * q) ** 2, rel=1e-6)
+
+
+def test_smooth3d_exact_single_particle_and_volume_integral(orc):
+    p4 = np.array([[1.0, 2.0, 3.0, 2.5]], np.float32)
+    h = np.array([0.8], np.float32)
+    pts = np.array([[1.0, 2.0, 3.0, 0], [1.4, 2.0, 3.0, 0], [1.0, 2.0, 3.9, 0]], np.float32)
+    got = orc.smooth3d_exact(p4, h, pts)
+    K = 105.0 / (32.0 * np.pi)
+    h8 = float(np.float32(0.8))
+    assert got[0] == pytest.approx(2.5 * K / h8 ** 3, rel=1e-12)
+    s = 1.0 - (float(np.float32(1.4)) - 1.0) ** 2 / h8 ** 2
+    assert got[1] == pytest.approx(2.5 * K / h8 ** 3 * s * s, rel=1e-6)
+    assert got[2] == 0.0
+    # sum over a fine lattice times the cell volume is the particle's weight
+    d = 0.04
+    ax = np.arange(-1.0, 1.0 + 1e-9, d)
+    X, Y, Z = np.meshgrid(ax + 1.0, ax + 2.0, ax + 3.0, indexing="ij")
+    lat = np.stack([X.ravel(), Y.ravel(), Z.ravel(), np.zeros(X.size)], 1).astype(np.float32)
+    vol = orc.smooth3d_exact(p4, h, lat).sum() * d ** 3
+    assert vol == pytest.approx(2.5, rel=2e-3)
+
+
+def test_smooth3d_ref_equals_exact_when_nothing_is_dropped(orc):
+    """Equal h everywhere: every point's search radius (max h of its cell +
+    cell half-diagonal) covers every particle in reach, so the reference sum
+    is the exact sum with the 1.04445f constant.  The reference's FP32 form
+    1 + q^4 - 2 q^2 cancels near q = 1 (up to ~5e-5 of a point's value
+    here); a dropped or extra particle is an error of 1e-3 or more."""
+    rng = np.random.default_rng(8)
+    n = 3000
+    p4 = np.concatenate([rng.uniform(0, 20, (n, 3)), rng.uniform(0.5, 1.5, (n, 1))],
+                        1).astype(np.float32)
+    h = np.full(n, 1.3, np.float32)
+    g = orc.Grid(p4, h, target=20)
+    assert int(g.nc.min()) > 1
+    pts = np.concatenate([rng.uniform(1, 19, (400, 3)), np.zeros((400, 1))], 1).astype(np.float32)
+    ref = g.smooth3d_ref(pts)
+    ex = orc.smooth3d_exact(p4, h, pts) * (float(np.float32(1.04445)) / (105.0 / (32.0 * np.pi)))
+    m = ex > 0
+    assert m.sum() > 300
+    assert np.max(np.abs(ref[m] - ex[m]) / ex[m]) < 1e-4
+    assert np.all(ref[~m] == 0.0)
+    # the stable FP64 form of the same search agrees with the exact sum to rounding
+    ref1 = g.smooth3d_ref(pts, form=1)
+    assert np.max(np.abs(ref1[m] - ex[m]) / ex[m]) < 1e-12
+
+
+def test_smooth3d_ref_drops_what_the_search_box_excludes(orc):
+    """The search radius comes from the point's own cell: a big particle in
+    another cell whose support reaches the point is dropped when it lies
+    beyond max h (point's cell) + cell half-diagonal (the F3 heuristic)."""
+    cs = np.float32(1.0)
+    params = (np.array([10, 10, 10], np.int32), np.zeros(3, np.float32), np.full(3, cs))
+    # small particles at the point's cell, one big one 3.2 away (h = 4)
+    p4 = np.array([[2.5, 2.5, 2.5, 1.0], [2.6, 2.4, 2.5, 1.0], [5.7, 2.5, 2.5, 1.0]], np.float32)
+    h = np.array([0.6, 0.6, 4.0], np.float32)
+    g = orc.Grid(p4, h, params=params)
+    pts = np.array([[2.5, 2.5, 2.5, 0.0]], np.float32)
+    ref = g.smooth3d_ref(pts)[0]
+    near = sum(orc.quartic3d(np.linalg.norm(p4[k, :3] - pts[0, :3]), h[k]) for k in range(2))
+    big = orc.quartic3d(np.linalg.norm(p4[2, :3] - pts[0, :3]), h[2])
+    assert big > 0                         # the big particle's support reaches the point
+    assert ref == pytest.approx(near, rel=1e-6)   # but the search box drops it
