@@ -1,6 +1,6 @@
 """Benchmark of the projective SPH smoothing path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1|c4|c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1|c4|c5|v1]
 
 Workloads (synthetic, seeded; see illcrawl_amd/synth.py):
   c3 (default)  the batched multi-view workload: the 16M-cell X-ray snapshot
@@ -16,6 +16,11 @@ Workloads (synthetic, seeded; see illcrawl_amd/synth.py):
                 cells (exact mode, --kernel quartic|cubic): adjoint cost
                 relative to forward and gradient error against the FP64 oracle
                 on a sample.
+  v1            volumetric smoothing (the 3-D grid reconstruction): gas
+                density on 8 slices of 2048^2 points, one through each halo
+                of the 16M-cell snapshot, 3000 kpc/h wide; a step is the 3-D
+                grid build over all 16M cells, one gather launch over the
+                8 slices and the retrieval.
 
 A step is one pass of the hot path over the workload, with the snapshot
 already resident in HBM: per view the quantity functor, the camera projection
@@ -40,6 +45,8 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-kernel evals/s, SPH projection: batched vs plain on MI355X @1e-5 rel err"
 PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (the spec figure counts packed FP32)
 FLOP_PER_EVAL = 12
+# 3-D pair: dx, dy, dz; r^2 (mul + 2 FMA); s = 1 - r^2/h^2 (FMA); s^2; w s^2 + a (FMA)
+FLOP_PER_EVAL_3D = 13
 # HBM bytes per k_gather_batch launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs of this workload scaled by a 1 GiB copy (profiles/k_gather_batch_xray_16M_64views.md);
 # keyed by (workload, mode, tile)
@@ -315,6 +322,129 @@ def batch_adjoint_main(args):
     print(json.dumps(line), flush=True)
 
 
+def volumetric_slices(snap, nx=2048, width=3000.0):
+    """One slice through each halo, normals cycling through z, y, x and two
+    oblique directions; all slices as one point array of nx-wide rows."""
+    import numpy as np
+    from illcrawl_amd import backend as B
+    normals = [(0, 0, 1), (0, 1, 0), (1, 0, 0), (1, 1, 1), (1, -2, 0.5)]
+    pts = [B.slice_points(np.asarray(c, np.float64), normals[k % len(normals)], width, nx)
+           for k, (c, _) in enumerate(snap.halos)]
+    return np.concatenate(pts, 0)
+
+
+def volumetric_main(args):
+    import numpy as np
+    import torch
+    import oracle
+    from illcrawl_amd import backend as B, synth
+    t0 = time.time()
+    snap = synth.cached("c2")
+    nx, width = 2048, 3000.0
+    pts = volumetric_slices(snap, nx, width)
+    log(f"[bench] v1: {snap.n} cells, {pts.shape[0] // (nx * nx)} slices of {nx}^2 "
+        f"({time.time() - t0:.1f}s)")
+    dev = torch.device("cuda", 0)
+    p4 = torch.from_numpy(snap.particles4()).to(dev)
+    h = torch.from_numpy(snap.hsml).to(dev)
+    pts_d = torch.from_numpy(pts).to(dev)
+    out = torch.empty(pts.shape[0], dtype=torch.float32, device=dev)
+    g = B.GridSmoothing(mode=args.mode, kernel="quartic_ref" if args.mode == "reference"
+                        else "quartic")
+    g.init_backend(snap.n)
+    g.setup_particles(p4, [h])
+    g.setup_evaluation_points(pts_d, layout_nx=nx)
+    total, per = g.count_evals(per_point=True)
+    log(f"[bench] exact evals per step: {total:.6g}")
+
+    def step(ev=None):
+        g.setup_particles(p4, [h])
+        g.setup_evaluation_points(pts_d, layout_nx=nx)
+        if ev:
+            ev[0].record()
+        g.run()
+        if ev:
+            ev[1].record()
+        return g.retrieve_results()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    gather_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    log(f"[bench] v1: {ms_per_step:.2f} ms/step, gather {gather_ms:.2f} ms")
+    achieved = total * FLOP_PER_EVAL_3D / (gather_ms * 1e-3) / 1e12
+    gi = g.grid_info()
+    line = {
+        "metric": "particle-kernel evals/s, volumetric SPH smoothing (3-D grid) on MI355X",
+        "value": total / (ms_per_step * 1e-3), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 per-point accumulation)",
+        "data": "synthetic (seeded: uniform + NFW halos, 32-NN density, lognormal masses)",
+        "config": dict(workload="volumetric smoothing: gas density on 8 slices through the "
+                                "halos of the 16M-cell snapshot", n_cells=snap.n,
+                       box_kpc_h=snap.box, slices=len(snap.halos), resolution=[nx, nx],
+                       width_kpc_h=width, mode=args.mode, evals_per_step=total,
+                       points_per_step=int(pts.shape[0]), grid_cells=gi["num_cells"],
+                       parallelism="single GPU"),
+        "points_per_s": pts.shape[0] / (ms_per_step * 1e-3),
+        "gather_ms": gather_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_smooth3d", "flop_per_eval": FLOP_PER_EVAL_3D,
+                     "evals_per_launch": total},
+    }
+    rng = np.random.default_rng(5)
+    ocache = {}
+
+    def ogrid():
+        if "g" not in ocache:
+            t = time.perf_counter()
+            ocache["g"] = oracle.Grid(snap.particles4(), snap.hsml, target=1000)
+            ocache["t"] = time.perf_counter() - t
+        return ocache["g"]
+
+    if not args.no_parity:
+        log("[bench] sampled parity vs oracle ...")
+        img = g.retrieve_results_f64().cpu().numpy()
+        idx = rng.choice(pts.shape[0], 3000, replace=False)
+        ref = (ogrid().smooth3d_ref(pts[idx], form=1) if args.mode == "reference"
+               else oracle.smooth3d_exact(snap.particles4(), snap.hsml, pts[idx]))
+        m = ref > 1e-6 * ref.max()
+        line["parity"] = dict(points=int(idx.size),
+                              linf=float(np.max(np.abs(img[idx][m] - ref[m]) / ref[m])),
+                              sum=float(abs(img[idx].sum() - ref.sum()) / ref.sum()))
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count())
+        oracle.set_num_threads(threads)
+        og = ogrid()
+        counts = per.cpu().numpy()
+        order = rng.permutation(pts.shape[0])
+        done, evals, elapsed, batch = 0, 0, 0.0, 64
+        while elapsed < args.cpu_seconds and done < order.size:
+            sel = order[done:done + batch]
+            t = time.perf_counter()
+            og.smooth3d_ref(pts[sel], acc_mode=0)
+            elapsed += time.perf_counter() - t
+            evals += int(counts[sel].sum())
+            done += sel.size
+            batch = min(batch * 2, 16384)
+        line["cpu_baseline"] = dict(
+            value=evals / elapsed, unit="evals/s", cores=threads, kind="port",
+            sample=f"{done} random points of the 8 slices ({elapsed:.1f} s of the reference "
+                   f"3-D grid gather restated in C, FP32 accumulator, {threads} OpenMP "
+                   f"threads); its grid build ({ocache['t']:.2f} s, one thread) not included")
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,6 +478,8 @@ def main():
         raise SystemExit("bench.py needs a GPU")
     if args.workload == "c4":
         return adjoint_main(args)
+    if args.workload == "v1":
+        return volumetric_main(args)
     if args.adjoint:
         return batch_adjoint_main(args)
 

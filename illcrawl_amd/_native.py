@@ -102,6 +102,7 @@ SIGNATURES = {
     "ilc_grid3d_retrieve_results": (_ST, [_P, ctypes.c_float, _P, _P]),
     "ilc_grid3d_retrieve_results_f64": (_ST, [_P, _P, _P]),
     "ilc_grid3d_grid_info": (_ST, [_P, ctypes.POINTER(GridInfo)]),
+    "ilc_grid3d_count_evals": (_ST, [_P, _P, ctypes.POINTER(_U64), _P]),
     "ilc_particles_set_w": (_ST, [_P, _P, _U64, _P]),
     "ilc_crawl_block": (_ST, [ctypes.POINTER(VolumeC), _P, _U64, _P, ctypes.c_int32, _P, _P,
                               _P, ctypes.POINTER(_U64), _P]),

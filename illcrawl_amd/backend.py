@@ -229,6 +229,16 @@ class GridSmoothing:
                     cell_size=list(gi.cell_size), max_h=gi.max_h,
                     num_particles=gi.num_particles)
 
+    def count_evals(self, per_point=False, stream=None):
+        """Exact count of (point, particle) pairs with w != 0 and the point
+        inside the particle's support, for the current grid."""
+        total = ctypes.c_uint64()
+        pp = torch.empty(self._npoints, dtype=torch.int64, device=self.device) \
+            if per_point else None
+        N.call("ilc_grid3d_count_evals", self._h, _ptr(pp), ctypes.byref(total),
+               _stream(stream))
+        return (total.value, pp) if per_point else total.value
+
 
 def slice_points(center, normal, width, nx, ny=None, up=(0.0, 0.0, 1.0)):
     """Pixel centres of a square slice of the volume (volumetric_slice.cpp):

@@ -546,6 +546,28 @@ ilc_status ilc_grid3d_grid_info(const ilc_grid3d* h, ilc_grid_info* out) {
   return ilc_psg_grid_info(&h->p, out);
 }
 
+ilc_status ilc_grid3d_count_evals(ilc_grid3d* g3, uint64_t* counts_dev, uint64_t* total,
+                                  void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(g3 != nullptr, "null handle");
+    ilc_psg* h = &g3->p;
+    if (!h->grid.ready) fail(ILC_ENOTREADY, "no grid");
+    if (!h->pts) fail(ILC_ENOTREADY, "no evaluation points");
+    device_guard dg(h->device);
+    hipStream_t s = as_stream(stream);
+    h->cnt.ensure(h->npts);
+    h->cnt_scratch.ensure(1);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->cnt.p), 0u, 2 * h->npts, s);
+    point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
+    launch_smooth3d(h->grid.view(), p, OP_COUNT, nullptr, s, h->cnt.p);
+    if (counts_dev)
+      ILC_HIP(hipMemcpyAsync(counts_dev, h->cnt.p, sizeof(unsigned long long) * h->npts,
+                             hipMemcpyDeviceToDevice, s));
+    const unsigned long long tot = launch_sum_u64(h->cnt.p, h->npts, h->cnt_scratch.p, s);
+    if (total) *total = tot;
+  });
+}
+
 ilc_status ilc_particles_set_w(float* particles4_dev, const float* w_dev, uint64_t n,
                                void* stream) {
   return guarded([&] {

@@ -229,8 +229,10 @@ void launch_vector_scale(float* x, float alpha, uint64_t n, hipStream_t s);
 void launch_set_w(float4* p4, const float* w, uint64_t n, hipStream_t s);
 void launch_retrieve(const double* acc, uint64_t n, float scale, float* out, hipStream_t s);
 
-// ---- smooth3d.hip: the volumetric smoothing gather (grid with g.pz set)
-void launch_smooth3d(const grid_view& g, const point_view& p, int op, double* acc, hipStream_t s);
+// ---- smooth3d.hip: the volumetric smoothing gather (grid with g.pz set);
+// op OP_REFERENCE / OP_EXACT add to acc, OP_COUNT adds evaluation counts to cnt
+void launch_smooth3d(const grid_view& g, const point_view& p, int op, double* acc, hipStream_t s,
+                     unsigned long long* cnt = nullptr);
 void launch_quantity_scaled(const float* in, double scale, uint64_t n, float* w, hipStream_t s);
 // The reference's spectral X-ray emissivity (2-D Gaunt table on the device,
 // t[i + nx j], i along ln(gamma^2) from -6, j along ln(u) from -16, step 0.2)

@@ -22,6 +22,9 @@
 //   5. evaluates each list for its point in FP32 over 32 particles, then into
 //      an FP64 accumulator.
 //
+// The count variant (exact search) adds 1 per pair with w != 0 and s > 0
+// instead of the kernel value: the number of particle-kernel evaluations.
+//
 // Kernel arithmetic: W = K / h^3 (1 - q^2)^2 for q = r / h < 1 and 0 beyond
 // (the reference's 1 + q^4 - 2 q^2 with q = min(r / h, 1)); the grid stores
 // coef = w K / h^3 and 1 / h^2, so a pair costs s = max(1 - r^2 / h^2, 0) and
@@ -115,9 +118,11 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
   return off + incl - v;
 }
 
-template <bool REF>
+template <bool REF, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, uint32_t tiles_x,
-                                                      double* __restrict__ acc) {
+                                                      double* __restrict__ acc,
+                                                      unsigned long long* __restrict__ cnt) {
+  static_assert(!(REF && COUNT), "evaluations are counted with the exact search");
   __shared__ smem3 sm;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -245,7 +250,10 @@ __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, 
         const float dx = e.x - q.x, dy = e.y - q.y, dz = e.z - q.z;
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         const float s = fmaxf(fmaf(-r2, e.w, 1.0f), 0.0f);
-        a = fmaf(w, s * s, a);
+        if constexpr (COUNT)
+          a += (s > 0.0f && w != 0.0f) ? 1.0f : 0.0f;
+        else
+          a = fmaf(w, s * s, a);
       }
       a64 += double(a);
     }
@@ -398,23 +406,32 @@ __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, 
     eval_full(ns);
     eval_masked(nm);
   }
-  if (pv) acc[pid] += a64;
+  if (pv) {
+    if constexpr (COUNT)
+      cnt[pid] += static_cast<unsigned long long>(a64);
+    else
+      acc[pid] += a64;
+  }
 }
 
 }  // namespace
 
-void launch_smooth3d(const grid_view& g, const point_view& p, int op, double* acc, hipStream_t s) {
+void launch_smooth3d(const grid_view& g, const point_view& p, int op, double* acc, hipStream_t s,
+                     unsigned long long* cnt) {
   if (!p.n) return;
   ILC_REQUIRE(g.pz != nullptr, "the grid is not a volumetric grid");
+  ILC_REQUIRE(op == OP_REFERENCE || op == OP_EXACT || op == OP_COUNT, "bad op %d", op);
+  ILC_REQUIRE(op == OP_COUNT ? cnt != nullptr : acc != nullptr, "null output");
   const uint32_t tiles_x = (p.lnx + kTile3 - 1) / kTile3;
   const uint64_t tiles = uint64_t(tiles_x) * ((p.lny + kTile3 - 1) / kTile3);
   ILC_REQUIRE(tiles < (uint64_t(1) << 31), "too many tiles");
+  const dim3 grid{static_cast<uint32_t>(tiles)}, block{kBlock};
   if (op == OP_REFERENCE)
-    hipLaunchKernelGGL(k_smooth3d<true>, dim3(uint32_t(tiles)), dim3(kBlock), 0, s, g, p, tiles_x,
-                       acc);
+    hipLaunchKernelGGL((k_smooth3d<true, false>), grid, block, 0, s, g, p, tiles_x, acc, cnt);
+  else if (op == OP_EXACT)
+    hipLaunchKernelGGL((k_smooth3d<false, false>), grid, block, 0, s, g, p, tiles_x, acc, cnt);
   else
-    hipLaunchKernelGGL(k_smooth3d<false>, dim3(uint32_t(tiles)), dim3(kBlock), 0, s, g, p, tiles_x,
-                       acc);
+    hipLaunchKernelGGL((k_smooth3d<false, true>), grid, block, 0, s, g, p, tiles_x, acc, cnt);
   ILC_HIP(hipGetLastError());
 }
 

@@ -292,6 +292,12 @@ ilc_status ilc_grid3d_run(ilc_grid3d* h, void* stream);
 ilc_status ilc_grid3d_retrieve_results(ilc_grid3d* h, float scale, float* out_dev, void* stream);
 ilc_status ilc_grid3d_retrieve_results_f64(ilc_grid3d* h, double* out_dev, void* stream);
 ilc_status ilc_grid3d_grid_info(const ilc_grid3d* h, ilc_grid_info* out);
+/* Exact particle-kernel evaluation count: per point (counts_dev, n points,
+ * optional) the particles with w != 0 whose support contains it, and the
+ * total, as ilc_psg_count_evals.  No reference counterpart (measurement
+ * only); synchronises the stream. */
+ilc_status ilc_grid3d_count_evals(ilc_grid3d* h, uint64_t* counts_dev, uint64_t* total,
+                                  void* stream);
 
 /* Combine particles with their quantity value
  * (reconstructing_data_crawler.cpp:233-239): particles4_dev[4i+3] = w_dev[i]. */

@@ -153,3 +153,25 @@ def test_rejects_other_kernels_and_empty_blocks(B):
         g.run()
     with pytest.raises(N.IllcrawlError):
         g.setup_particles(np.zeros((0, 4), np.float32), [np.zeros(0, np.float32)])
+
+
+def test_count_evals_equals_brute_force(B, scene):
+    """The evaluation count (the bench's unit of work) against a numpy brute
+    force of r^2 < h^2 over every particle, on a small slice; pairs within
+    FP32 rounding of the support edge may fall either way."""
+    p4, h, c = scene
+    pts = B.slice_points(c, (0, 0, 1), 600.0, 24)
+    g = B.GridSmoothing(mode="exact", kernel="quartic")
+    g.init_backend(p4.shape[0])
+    g.setup_particles(p4, [h])
+    g.setup_evaluation_points(pts)
+    total, per = g.count_evals(per_point=True)
+    per = per.cpu().numpy()
+    xyz, h2 = p4[:, :3].astype(np.float64), h.astype(np.float64) ** 2
+    live = p4[:, 3] != 0
+    ref = np.array([int(((((xyz - q[:3]) ** 2).sum(1) < h2) & live).sum()) for q in pts])
+    say(f"smooth3d count: {total} pairs (brute force {int(ref.sum())})")
+    assert total == int(per.sum())
+    assert total > 10 * pts.shape[0]
+    assert np.all(np.abs(per - ref) <= np.maximum(2, 1e-4 * ref))
+    assert abs(total - int(ref.sum())) <= 1e-5 * ref.sum() + 4
