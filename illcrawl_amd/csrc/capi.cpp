@@ -526,7 +526,8 @@ ilc_status ilc_grid3d_run(ilc_grid3d* g3, void* stream) {
     if (!h->pts) fail(ILC_ENOTREADY, "run before setup_evaluation_points");
     device_guard dg(h->device);
     point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
-    launch_smooth3d(h->grid.view(), p, h->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT,
+    launch_smooth3d(h->grid.view(), h->grid.levels(), p,
+                    h->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT,
                     h->acc.p, as_stream(stream));
   });
 }
@@ -559,7 +560,7 @@ ilc_status ilc_grid3d_count_evals(ilc_grid3d* g3, uint64_t* counts_dev, uint64_t
     h->cnt_scratch.ensure(1);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->cnt.p), 0u, 2 * h->npts, s);
     point_view p{h->pts, static_cast<uint32_t>(h->npts), h->lnx, h->lny};
-    launch_smooth3d(h->grid.view(), p, OP_COUNT, nullptr, s, h->cnt.p);
+    launch_smooth3d(h->grid.view(), h->grid.levels(), p, OP_COUNT, nullptr, s, h->cnt.p);
     if (counts_dev)
       ILC_HIP(hipMemcpyAsync(counts_dev, h->cnt.p, sizeof(unsigned long long) * h->npts,
                              hipMemcpyDeviceToDevice, s));

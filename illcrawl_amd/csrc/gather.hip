@@ -165,8 +165,8 @@ struct gather_smem {
   float dtab[KERNEL == 2 && ADJ ? kCubicTable : 1];
   uint32_t cell_b[kBlock];
   uint32_t cell_off[kBlock];
-  int cell_info[kBlock];  // cells: packed cell (cx | cy << 16) << 1 | full; rows: base key
-  int cell_y[kBlock];     // rows: the row's cy
+  int cell_info[kBlock];  // cells: packed cell cx | cy << 16 (unsigned); rows: base key
+  int cell_y[kBlock];     // cells: 1 when in every point's search box; rows: the row's cy
   uint32_t scan_ws[4];
   uint32_t Lidx[ADJ ? kCap : 1];    // adjoint: sorted particle index
   float4 Pt[ADJ ? kBlock * 4 : 1];  // adjoint: tile points (x, y, dL/dvalue, -), row-major
@@ -516,7 +516,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
         byw[k] = ok ? uint32_t(y1 - y0) : 0u;
       }
       auto add = [&](float* a, const float4& q, int c) {
-        const int cx = c & 0xffff, cy = c >> 16;
+        const int cx = c & 0xffff, cy = int(uint32_t(c) >> 16);
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
           const bool in = uint32_t(cx - bx0[k]) <= bxw[k] && uint32_t(cy - by0[k]) <= byw[k];
@@ -701,7 +701,8 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       sm.cell_info[t] = ccx;
       sm.cell_y[t] = ccy;
     } else {
-      sm.cell_info[t] = ((ccx | (ccy << 16)) << 1) | cfull;
+      sm.cell_info[t] = int(uint32_t(ccx) | (uint32_t(ccy) << 16));
+      sm.cell_y[t] = cfull;
     }
     __syncthreads();
     tot = uint32_t(uniform(int(tot)));
@@ -709,7 +710,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       const uint32_t v = v0 + uint32_t(t);
       float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
       bool keep = false, full = false, big = false, cover = false;
-      int cinf = 0;
+      int cinf = 0, cf = 0;
       uint32_t adj_idx = 0;
       if (v < tot) {
         // last cell whose offset is <= v (it has a nonzero count)
@@ -720,13 +721,15 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
         const uint32_t sidx = sm.cell_b[lo] + (v - sm.cell_off[lo]);
         cinf = sm.cell_info[lo];
         if constexpr (REF) {
+          if (!rows_mode) cf = sm.cell_y[lo];
           if (rows_mode) {
             // the particle's cell from its key and the row's base key
             const int px_c = int(g.key[sidx] - uint32_t(cinf));
             const int py_c = sm.cell_y[lo];
             const int pf = (allz_u && py_c >= r_ily && py_c <= r_ihy && px_c >= r_ilx &&
                             px_c <= r_ihx) ? 1 : 0;
-            cinf = ((px_c | (py_c << 16)) << 1) | pf;
+            cinf = int(uint32_t(px_c) | (uint32_t(py_c) << 16));
+            cf = pf;
           }
         }
         const float4 q = g.part[sidx];  // (x, y, coef, 1/h^2)
@@ -736,7 +739,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
           const float d2 = fmaf(dy, dy, dx * dx);
           keep = d2 * q.w < cull2;
         }
-        full = keep && (!REF || (cinf & 1));
+        full = keep && (!REF || cf);
         if (keep) {
           if constexpr (COUNT) {
             e = make_float4(q.x, q.y, g.hs[sidx], 0.0f);
@@ -804,7 +807,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
         if (keep && !full) {
           const int pos = basem + prefix_count(bm);
           sm.Lm[pos] = e;
-          sm.Lmc[pos] = cinf >> 1;
+          sm.Lmc[pos] = cinf;
         }
       }
       nb += totb;

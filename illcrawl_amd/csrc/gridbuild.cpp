@@ -32,6 +32,30 @@ float kernel_norm_3d(int kernel) {
   return kernel == ILC_KERNEL_QUARTIC_REF ? 1.04445f : static_cast<float>(105.0 / (32.0 * M_PI));
 }
 
+level_view grid_store::levels() const {
+  level_view lv;
+  lv.key = lv_key.p;
+  lv.pos = lv_pos.p;
+  lv.w = lv_w.p;
+  lv.cell = lv_cell.p;
+  lv.meta = lv_meta.p;
+  for (int l = 0; l < kLevels3; ++l) lv.cs[l] = lv_cs[l];
+  return lv;
+}
+
+void level_cell_sizes(const grid_params& g, float max_h, float cs[kLevels3]) {
+  // level L: h <= H_L = max_h 2^(L - 23), cells of H_L, at least the grid
+  // extent over the 19-bit fine coordinate range
+  double ext = 0.0;
+  for (int a = 0; a < 3; ++a) ext = std::max(ext, double(g.nc[a]) * double(g.cs[a]));
+  const float floor_cs = static_cast<float>(ext / 524000.0);
+  for (int l = 0; l < kLevels3; ++l) {
+    const float hl = std::ldexp(max_h, l - (kLevels3 - 1));
+    cs[l] = std::max(hl, floor_cs);
+    if (!(cs[l] > 0.0f) || !std::isfinite(cs[l])) cs[l] = std::max(max_h, 1e-30f);
+  }
+}
+
 grid_view grid_store::view() const {
   grid_view g;
   g.part = part.p;
@@ -110,6 +134,17 @@ void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, 
   dd = dd + g.cs[2] * g.cs[2];
   gs.cell_radius = 0.5f * std::sqrt(dd);
   gs.kz0 = kz0;
+  if (volumetric) {
+    gs.lv_key.ensure(n);
+    gs.lv_pos.ensure(n);
+    gs.lv_w.ensure(n);
+    gs.lv_cell.ensure(n);
+    gs.lv_meta.ensure(kLvMetaWords);
+    sc.lv_tmp.ensure(3 * std::max<uint64_t>(n, 1));
+    level_cell_sizes(g, max_h, gs.lv_cs);
+    launch_level_index(gs.view(), uint32_t(n), gs.levels(), gs.lv_key.p, gs.lv_pos.p, gs.lv_w.p,
+                       gs.lv_cell.p, gs.lv_meta.p, sc.lv_tmp.p, sc.temp.p, s);
+  }
   gs.ready = true;
 }
 

@@ -20,6 +20,13 @@ struct grid_store {
   dbuf<uint32_t> start;       // ncells + 1
   dbuf<uint32_t> maxh_bits;   // ncells
   dbuf<float> pz;             // volumetric grids: sorted z
+  // volumetric grids: the level index the gather searches
+  dbuf<unsigned long long> lv_key;
+  dbuf<float4> lv_pos;
+  dbuf<float> lv_w;
+  dbuf<int2> lv_cell;
+  dbuf<uint32_t> lv_meta;
+  float lv_cs[kLevels3] = {};
   grid_params gp{};
   float max_h = 0.f, cell_radius = 0.f;
   int32_t kz0 = 0;
@@ -31,11 +38,12 @@ struct grid_store {
   bool volumetric = false;  // 3-D smoothing: part.z holds w K / h^3, z in pz
   bool ready = false;
   grid_view view() const;
+  level_view levels() const;
 };
 
 // Scratch shared by successive builds on one stream.
 struct grid_scratch {
-  dbuf<uint32_t> temp, counts, flags, horder;
+  dbuf<uint32_t> temp, counts, flags, horder, lv_tmp;
   dbuf<float> bbox_partial;
   dbuf<bbox_result> bbox_dev;
 };

@@ -229,10 +229,33 @@ void launch_vector_scale(float* x, float alpha, uint64_t n, hipStream_t s);
 void launch_set_w(float4* p4, const float* w, uint64_t n, hipStream_t s);
 void launch_retrieve(const double* acc, uint64_t n, float scale, float* out, hipStream_t s);
 
-// ---- smooth3d.hip: the volumetric smoothing gather (grid with g.pz set);
+// ---- smooth3d.hip: the volumetric smoothing gather (grid with g.pz set)
+//
+// The level index of a volumetric grid: particles binned by h into levels
+// (level L holds h <= max_h 2^(L - 23)), each level sorted by a fine cell key
+// (L << 57 | z << 38 | y << 19 | x) of cell size cs[L] ~ the level's h.
+constexpr int kLevels3 = 24;
+constexpr uint32_t kFineMax3 = (1u << 19) - 1;
+// meta layout: count[24], max h bits[24], start[25]
+constexpr int kLvMetaWords = 3 * kLevels3 + 1;
+struct level_view {
+  const unsigned long long* key;  // sorted fine keys
+  const float4* pos;              // (x, y, z, 1/h^2)
+  const float* w;                 // w K / h^3
+  const int2* cell;               // reference cell (cx | cy << 16, cz)
+  const uint32_t* meta;
+  float cs[kLevels3];
+};
+// Fine cell sizes of the levels for a grid of that extent and max h (host).
+void level_cell_sizes(const grid_params& g, float max_h, float cs[kLevels3]);
+// Builds the level index from the grid's sorted arrays (part, pz, hs, key).
+// tmp: 3 n words; radix_temp: radix_temp_words(n).
+void launch_level_index(const grid_view& g, uint32_t n, const level_view& lv,
+                        unsigned long long* key, float4* pos, float* w, int2* cell,
+                        uint32_t* meta, uint32_t* tmp, uint32_t* radix_temp, hipStream_t s);
 // op OP_REFERENCE / OP_EXACT add to acc, OP_COUNT adds evaluation counts to cnt
-void launch_smooth3d(const grid_view& g, const point_view& p, int op, double* acc, hipStream_t s,
-                     unsigned long long* cnt = nullptr);
+void launch_smooth3d(const grid_view& g, const level_view& lv, const point_view& p, int op,
+                     double* acc, hipStream_t s, unsigned long long* cnt = nullptr);
 void launch_quantity_scaled(const float* in, double scale, uint64_t n, float* w, hipStream_t s);
 // The reference's spectral X-ray emissivity (2-D Gaunt table on the device,
 // t[i + nx j], i along ln(gamma^2) from -6, j along ln(u) from -16, step 0.2)

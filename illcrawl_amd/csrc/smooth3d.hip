@@ -5,6 +5,15 @@
 // (smoothing_kernel.cl:27-37) over the particles of the cells in its search
 // box.
 //
+// Candidates come from the level index of the grid, not from the reference
+// cells: particles binned by h into levels (level L holds h <= max_h 2^(L-23)),
+// each level sorted by a fine cell key (z, y, x) whose cell size is about the
+// level's h.  The reference's cells are coarse (about 1000 particles each on
+// average, far more in clustered regions), and a tile inside a cell would
+// stream all of them; a level row holds only the particles near the tile.
+// The result is the reference's, since the search box test is made on each
+// particle's reference cell.
+//
 // One 256-thread workgroup owns a 16 x 16 tile of the point array seen as an
 // image (one point per thread, e.g. a volumetric slice).  It
 //   1. loads its points and, in reference mode, each point's search box as
@@ -13,24 +22,17 @@
 //      x -/+ radius on each axis);
 //   2. reduces the tile's 3-D bounding box and the union and intersection of
 //      the search boxes;
-//   3. visits the candidate cells 256 at a time, drops those whose max h
-//      cannot reach the tile box, and reads only the prefix of each cell
-//      (particles by descending h) that can;
-//   4. culls each particle against the tile box (support sphere vs box) and
-//      compacts the survivors into two LDS lists by wave ballot: full (the
-//      cell is in every point's search box) and masked (in some);
+//   3. per level, takes the fine cells within the level's max h of the tile
+//      box: one key range per (y, z) row of them (per z plane, or one range,
+//      when the rows are too many), found by binary search in the level's
+//      sorted keys;
+//   4. streams the ranges, culls each particle against the tile box (support
+//      sphere vs box) and, in reference mode, drops it when its reference cell
+//      is in no point's search box; the survivors go by wave ballot to two LDS
+//      lists: full (the cell is in every point's box) and masked (in some);
 //   5. evaluates each list for its point in FP32 over 32 particles, then into
 //      an FP64 accumulator.
 //
-// The count variant (exact search) adds 1 per pair with w != 0 and s > 0
-// instead of the kernel value: the number of particle-kernel evaluations.
-//
-// Kernel arithmetic: W = K / h^3 (1 - q^2)^2 for q = r / h < 1 and 0 beyond
-// (the reference's 1 + q^4 - 2 q^2 with q = min(r / h, 1)); the grid stores
-// coef = w K / h^3 and 1 / h^2, so a pair costs s = max(1 - r^2 / h^2, 0) and
-// coef * s^2.  The cell-distance cull of the reference never removes a
-// particle whose support reaches the point (its cell centre is within the
-// cell half-diagonal of it), so it is not repeated.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -42,6 +44,7 @@ namespace {
 
 constexpr int kCap3 = 384;
 constexpr int kFlush3 = 128;
+constexpr uint32_t kRowCap3 = 64;  // key rows per level and tile before coarser ranges
 static_assert(kFlush3 - 1 + kBlock <= kCap3, "list capacity");
 constexpr int kTile3 = 16;  // 16 x 16 points per workgroup
 
@@ -88,16 +91,56 @@ struct smem3 {
   float Lw[kCap3];   // full: coef
   float4 Lm[kCap3];  // masked: (x, y, z, 1/h^2)
   float Lmw[kCap3];  // masked: coef
-  int2 Lmc[kCap3];   // masked: cell (cx | cy << 16, cz)
-  uint32_t cell_b[kBlock];
-  uint32_t cell_off[kBlock];
-  int cell_xy[kBlock];  // cx | cy << 16
-  int cell_zf[kBlock];  // cz << 1 | full
+  int2 Lmc[kCap3];   // masked: reference cell (cx | cy << 16, cz)
+  uint32_t row_b[kBlock];
+  uint32_t row_off[kBlock];
   uint32_t scan_ws[4];
   uint32_t wcnt[2][8];
   float redf[4][6];
   int redi[4][13];
+  // the tile's key rows per level: fine box, row mode, first row, start
+  int lv_f0[kLevels3][3];
+  int lv_f1[kLevels3][3];
+  int lv_mode[kLevels3];
+  uint32_t lv_row0[kLevels3 + 1];
+  uint32_t lv_s[kLevels3 + 1];
 };
+
+// level of a particle: the smallest L with h <= max_h 2^(L - 23) (by the
+// exponent of max_h / h; any assignment is correct, the gather searches
+// each level with its particles' largest h)
+__device__ inline int level_of(float h, float max_h) {
+  const float r = max_h / h;
+  const int e = int((__float_as_uint(r) >> 23) & 0xffu) - 127;
+  return min(max(kLevels3 - 1 - e, 0), kLevels3 - 1);
+}
+
+// fine cell coordinate, clamped to [0, kFineMax3]; monotone in x
+__device__ inline uint32_t fine_of(float x, float mn, float cs) {
+#pragma clang fp contract(off)
+  const float f = floorf((x - mn) / cs);
+  if (!(f >= 0.0f)) return 0u;
+  if (f >= float(kFineMax3)) return kFineMax3;
+  return uint32_t(f);
+}
+
+__device__ inline unsigned long long fine_key(int l, uint32_t fx, uint32_t fy, uint32_t fz) {
+  return (static_cast<unsigned long long>(l) << 57) | (static_cast<unsigned long long>(fz) << 38) |
+         (static_cast<unsigned long long>(fy) << 19) | static_cast<unsigned long long>(fx);
+}
+
+// first index in [a, b) whose key is >= k (keys sorted ascending)
+__device__ inline uint32_t lower_bound_u64(const unsigned long long* __restrict__ key, uint32_t a,
+                                           uint32_t b, unsigned long long k) {
+  while (a < b) {
+    const uint32_t m = (a + b) >> 1;
+    if (key[m] < k)
+      a = m + 1;
+    else
+      b = m;
+  }
+  return a;
+}
 
 __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* ws) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -119,8 +162,8 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
 }
 
 template <bool REF, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, uint32_t tiles_x,
-                                                      double* __restrict__ acc,
+__global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, level_view lv, point_view p,
+                                                      uint32_t tiles_x, double* __restrict__ acc,
                                                       unsigned long long* __restrict__ cnt) {
   static_assert(!(REF && COUNT), "evaluations are counted with the exact search");
   __shared__ smem3 sm;
@@ -198,27 +241,6 @@ __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, 
   __syncthreads();
   if (!uniform(any)) return;
 
-  // ---- candidate cell box
-  int c0[3], c1[3];
-  for (int a = 0; a < 3; ++a) {
-    if (REF) {
-      c0[a] = max(ulo[a], 0);
-      c1[a] = min(uhi[a], g.nc[a] - 1);
-    } else {
-      // one extra cell on each side absorbs rounding in x -/+ reach
-      c0[a] = max(max(cell_of(bmin[a], -g.max_h, g.minc[a], g.cs[a]), INT32_MIN + 1) - 1, 0);
-      c1[a] = min(min(cell_of(bmax[a], g.max_h, g.minc[a], g.cs[a]), INT32_MAX - 1) + 1,
-                  g.nc[a] - 1);
-    }
-    c0[a] = uniform(c0[a]);
-    c1[a] = uniform(c1[a]);
-  }
-  const int64_t ncx = c1[0] >= c0[0] ? c1[0] - c0[0] + 1 : 0;
-  const int64_t ncy = c1[1] >= c0[1] ? c1[1] - c0[1] + 1 : 0;
-  const int64_t ncz = c1[2] >= c0[2] ? c1[2] - c0[2] + 1 : 0;
-  const int64_t ncand = ncx * ncy * ncz;
-  const uint32_t nxny = uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
-
   // per point: its box clamped to the cells a list entry can carry
   // (0..65535 per axis); an empty box fails every cell
   int bx0 = INT32_MAX, by0 = 0, bz0 = 0;
@@ -268,7 +290,8 @@ __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, 
           const float4 e = sm.Lm[j0 + j];
           const float w = sm.Lmw[j0 + j];
           const int2 c = sm.Lmc[j0 + j];
-          const int cx = uniform(c.x) & 0xffff, cy = uniform(c.x) >> 16, cz = uniform(c.y);
+          const int cx = uniform(c.x) & 0xffff, cy = int(uint32_t(uniform(c.x)) >> 16);
+          const int cz = uniform(c.y);
           const bool in = uint32_t(cx - bx0) <= bxw && uint32_t(cy - by0) <= byw &&
                           uint32_t(cz - bz0) <= bzw;
           const float dx = e.x - q.x, dy = e.y - q.y, dz = e.z - q.z;
@@ -282,54 +305,86 @@ __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, 
   };
 
   const float sup2 = 1.00002f;  // support 1 h, slack for rounding
-  for (int64_t cbase = 0; cbase < ncand; cbase += kBlock) {
-    const int64_t ci = cbase + t;
-    uint32_t cb = 0, cn = 0;
-    int cxy = 0, czf = 0;
-    if (ci < ncand) {
-      const int cx = c0[0] + int(ci % ncx);
-      const int64_t rest = ci / ncx;
-      const int cy = c0[1] + int(rest % ncy);
-      const int cz = c0[2] + int(rest / ncy);
-      const uint32_t key = uint32_t(cx) + uint32_t(cy) * uint32_t(g.nc[0]) + uint32_t(cz) * nxny;
-      cb = g.start[key];
-      const uint32_t ce = g.start[key + 1];
-      if (ce > cb) {
-        const float mh = __uint_as_float(g.maxh_bits[key]);
-        const int cc[3] = {cx, cy, cz};
-        float d2c = 0.0f;
-        for (int a = 0; a < 3; ++a) {
-          const float lo_a = g.minc[a] + float(cc[a]) * g.cs[a];
-          const float mg = 1e-4f * g.cs[a] + 4e-7f * (fabsf(g.minc[a]) + fabsf(lo_a) + g.cs[a]);
-          const float d = fmaxf(fmaxf((lo_a - mg) - bmax[a], bmin[a] - (lo_a + g.cs[a] + mg)), 0.0f);
-          d2c = fmaf(d, d, d2c);
-        }
-        if (d2c < mh * mh * sup2) {
-          // the cell's particles come by descending h (top 16 bits): the
-          // prefix whose h, rounded up to its bucket, reaches the tile
-          uint32_t a = 0, b = ce - cb;
-          while (a < b) {
-            const uint32_t m = (a + b) >> 1;
-            const float hu = __uint_as_float(__float_as_uint(g.hs[cb + m]) | 0xffffu);
-            if (d2c < hu * hu * sup2)
-              a = m + 1;
-            else
-              b = m;
-          }
-          cn = a;
-        }
-        const bool full = REF && cx >= ilo[0] && cx <= ihi[0] && cy >= ilo[1] && cy <= ihi[1] &&
-                          cz >= ilo[2] && cz <= ihi[2];
-        cxy = cx | (cy << 16);
-        czf = (cz << 1) | (full ? 1 : 0);
+
+  // ---- the tile's key rows, level by level
+  if (t < kLevels3) {
+    const int l = t;
+    int f0[3] = {0, 0, 0}, f1[3] = {-1, -1, -1};
+    int mode = 0;
+    uint32_t rows = 0;
+    if (lv.meta[l] != 0u) {
+      // every particle whose support reaches a point of the tile lies within
+      // the level's largest h of the tile box; the margin covers rounding
+      const float R = __uint_as_float(lv.meta[kLevels3 + l]);
+      const float c = lv.cs[l];
+      for (int a = 0; a < 3; ++a) {
+        const float m = R + 1e-5f * (fabsf(bmin[a]) + fabsf(bmax[a]) + R + fabsf(g.minc[a]));
+        f0[a] = int(fine_of(bmin[a] - m, g.minc[a], c));
+        f1[a] = int(fine_of(bmax[a] + m, g.minc[a], c));
+      }
+      const uint32_t ny = uint32_t(f1[1] - f0[1] + 1), nz = uint32_t(f1[2] - f0[2] + 1);
+      if (ny <= kRowCap3 && nz <= kRowCap3 && ny * nz <= kRowCap3) {
+        mode = 0;
+        rows = ny * nz;
+      } else if (nz <= kRowCap3) {
+        mode = 1;  // one range per z plane over rows f0y..f1y
+        rows = nz;
+      } else {
+        mode = 2;  // one range over the whole box
+        rows = 1;
       }
     }
+    for (int a = 0; a < 3; ++a) {
+      sm.lv_f0[l][a] = f0[a];
+      sm.lv_f1[l][a] = f1[a];
+    }
+    sm.lv_mode[l] = mode;
+    sm.lv_row0[l] = rows;
+    sm.lv_s[l] = lv.meta[2 * kLevels3 + l];
+    if (l == kLevels3 - 1) sm.lv_s[kLevels3] = lv.meta[3 * kLevels3];
+  }
+  __syncthreads();
+  if (t == 0) {
+    uint32_t r0 = 0;
+    for (int l = 0; l < kLevels3; ++l) {
+      const uint32_t r = sm.lv_row0[l];
+      sm.lv_row0[l] = r0;
+      r0 += r;
+    }
+    sm.lv_row0[kLevels3] = r0;
+  }
+  __syncthreads();
+  const uint32_t nrows = uint32_t(uniform(int(sm.lv_row0[kLevels3])));
+
+  for (uint32_t rbase = 0; rbase < nrows; rbase += kBlock) {
+    const uint32_t r = rbase + uint32_t(t);
+    uint32_t rb = 0, rn = 0;
+    if (r < nrows) {
+      int l = 0;
+      while (sm.lv_row0[l + 1] <= r) ++l;  // r < lv_row0[kLevels3] ends the walk
+      const uint32_t k = r - sm.lv_row0[l];
+      const int mode = sm.lv_mode[l];
+      const int* f0 = sm.lv_f0[l];
+      const int* f1 = sm.lv_f1[l];
+      int y0 = f0[1], y1 = f1[1], z0 = f0[2], z1 = f1[2];
+      if (mode == 0) {
+        const uint32_t ny = uint32_t(f1[1] - f0[1] + 1);
+        y0 = y1 = f0[1] + int(k % ny);
+        z0 = z1 = f0[2] + int(k / ny);
+      } else if (mode == 1) {
+        z0 = z1 = f0[2] + int(k);
+      }
+      const unsigned long long klo = fine_key(l, uint32_t(f0[0]), uint32_t(y0), uint32_t(z0));
+      const unsigned long long khi = fine_key(l, uint32_t(f1[0]), uint32_t(y1), uint32_t(z1));
+      const uint32_t e_ = sm.lv_s[l + 1];
+      const uint32_t a_ = lower_bound_u64(lv.key, sm.lv_s[l], e_, klo);
+      rb = a_;
+      rn = lower_bound_u64(lv.key, a_, e_, khi + 1ull) - a_;
+    }
     uint32_t tot;
-    const uint32_t coff = block_scan_excl(cn, &tot, sm.scan_ws);
-    sm.cell_b[t] = cb;
-    sm.cell_off[t] = coff;
-    sm.cell_xy[t] = cxy;
-    sm.cell_zf[t] = czf;
+    const uint32_t roff = block_scan_excl(rn, &tot, sm.scan_ws);
+    sm.row_b[t] = rb;
+    sm.row_off[t] = roff;
     __syncthreads();
     tot = uint32_t(uniform(int(tot)));
     for (uint32_t v0 = 0; v0 < tot; v0 += kBlock) {
@@ -339,26 +394,37 @@ __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, 
       float w = 0.0f;
       int2 cinf = make_int2(0, 0);
       if (v < tot) {
+        // last row whose offset is <= v (it has a nonzero count)
         int l = 0;
 #pragma unroll
         for (int step = kBlock / 2; step > 0; step >>= 1)
-          if (sm.cell_off[l + step] <= v) l += step;
-        const uint32_t sidx = sm.cell_b[l] + (v - sm.cell_off[l]);
-        const float4 pp = g.part[sidx];  // (x, y, coef, 1/h^2)
-        const float pzv = g.pz[sidx];
-        if (pp.z != 0.0f) {
-          const float pc[3] = {pp.x, pp.y, pzv};
+          if (sm.row_off[l + step] <= v) l += step;
+        const uint32_t sidx = sm.row_b[l] + (v - sm.row_off[l]);
+        e = lv.pos[sidx];  // (x, y, z, 1/h^2)
+        w = lv.w[sidx];
+        if (w != 0.0f) {
+          const float pc[3] = {e.x, e.y, e.z};
           float d2 = 0.0f;
           for (int a = 0; a < 3; ++a) {
             const float d = fmaxf(fmaxf(bmin[a] - pc[a], pc[a] - bmax[a]), 0.0f);
             d2 = fmaf(d, d, d2);
           }
-          keep = d2 * pp.w < sup2;
+          keep = d2 * e.w < sup2;
         }
-        full = keep && (!REF || (sm.cell_zf[l] & 1));
-        e = make_float4(pp.x, pp.y, pzv, pp.w);
-        w = pp.z;
-        cinf = make_int2(sm.cell_xy[l], sm.cell_zf[l] >> 1);
+        full = keep;
+        if (REF && keep) {
+          // the reference sums a particle only when its cell is in the
+          // point's search box: none (drop), every (full) or some (masked)
+          cinf = lv.cell[sidx];
+          const int cc[3] = {cinf.x & 0xffff, int(uint32_t(cinf.x) >> 16), cinf.y};
+          bool inu = true, ini = true;
+          for (int a = 0; a < 3; ++a) {
+            inu = inu && cc[a] >= ulo[a] && cc[a] <= uhi[a];
+            ini = ini && cc[a] >= ilo[a] && cc[a] <= ihi[a];
+          }
+          keep = inu;
+          full = inu && ini;
+        }
       }
       const uint64_t bs = __ballot(full);
       const uint64_t bm = __ballot(keep && !full);
@@ -399,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, 
         ns = nm = 0;
       }
     }
-    __syncthreads();  // the cell table is rewritten by the next chunk
+    __syncthreads();  // the row table is rewritten by the next chunk
   }
   if (ns > 0 || nm > 0) {
     __syncthreads();
@@ -414,12 +480,117 @@ __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, point_view p, 
   }
 }
 
+// ---- level index build
+
+// level and fine key of each particle (grid order), level counts and max h
+__global__ __launch_bounds__(kBlock) void k_lv_keys(grid_view g, level_view lv, uint32_t n,
+                                                     uint32_t* __restrict__ lo,
+                                                     uint32_t* __restrict__ hi,
+                                                     uint32_t* __restrict__ idx,
+                                                     uint32_t* __restrict__ meta) {
+  __shared__ uint32_t cnt[kLevels3], mh[kLevels3];
+  __shared__ float scs[kLevels3];
+  const int t = threadIdx.x;
+  if (t == 0) {
+#pragma unroll
+    for (int l = 0; l < kLevels3; ++l) {
+      cnt[l] = 0u;
+      mh[l] = 0u;
+      scs[l] = lv.cs[l];
+    }
+  }
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kBlock + uint32_t(t);
+  if (i < n) {
+    const float4 pp = g.part[i];
+    const float h = g.hs[i];
+    const int l = level_of(h, g.max_h);
+    const float c = scs[l];
+    const unsigned long long k = fine_key(l, fine_of(pp.x, g.minc[0], c),
+                                          fine_of(pp.y, g.minc[1], c),
+                                          fine_of(g.pz[i], g.minc[2], c));
+    lo[i] = uint32_t(k);
+    hi[i] = uint32_t(k >> 32);
+    idx[i] = i;
+    atomicAdd(&cnt[l], 1u);
+    atomicMax(&mh[l], __float_as_uint(h));
+  }
+  __syncthreads();
+  if (t < kLevels3 && cnt[t]) {
+    atomicAdd(&meta[t], cnt[t]);
+    atomicMax(&meta[kLevels3 + t], mh[t]);
+  }
+}
+
+__global__ void k_lv_starts(uint32_t* __restrict__ meta) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t s = 0;
+  for (int l = 0; l < kLevels3; ++l) {
+    meta[2 * kLevels3 + l] = s;
+    s += meta[l];
+  }
+  meta[3 * kLevels3] = s;
+}
+
+// the level-ordered arrays from the sorted order
+__global__ __launch_bounds__(kBlock) void k_lv_soa(grid_view g, level_view lv, uint32_t n,
+                                                    const uint32_t* __restrict__ order,
+                                                    unsigned long long* __restrict__ key,
+                                                    float4* __restrict__ pos,
+                                                    float* __restrict__ w,
+                                                    int2* __restrict__ cell) {
+  __shared__ float scs[kLevels3];
+  const int t = threadIdx.x;
+  if (t == 0) {
+#pragma unroll
+    for (int l = 0; l < kLevels3; ++l) scs[l] = lv.cs[l];
+  }
+  __syncthreads();
+  const uint32_t j = blockIdx.x * kBlock + uint32_t(t);
+  if (j >= n) return;
+  const uint32_t i = order[j];
+  const float4 pp = g.part[i];  // (x, y, coef, 1/h^2)
+  const float z = g.pz[i];
+  const int l = level_of(g.hs[i], g.max_h);
+  const float c = scs[l];
+  key[j] = fine_key(l, fine_of(pp.x, g.minc[0], c), fine_of(pp.y, g.minc[1], c),
+                    fine_of(z, g.minc[2], c));
+  pos[j] = make_float4(pp.x, pp.y, z, pp.w);
+  w[j] = pp.z;
+  const uint32_t ck = g.key[i];
+  const uint32_t nx = uint32_t(g.nc[0]), nxy = nx * uint32_t(g.nc[1]);
+  const uint32_t cz = ck / nxy, rem = ck - cz * nxy, cy = rem / nx, cx = rem - cy * nx;
+  cell[j] = make_int2(int(cx | (cy << 16)), int(cz));
+}
+
 }  // namespace
 
-void launch_smooth3d(const grid_view& g, const point_view& p, int op, double* acc, hipStream_t s,
-                     unsigned long long* cnt) {
+void launch_level_index(const grid_view& g, uint32_t n, const level_view& lv,
+                        unsigned long long* key, float4* pos, float* w, int2* cell,
+                        uint32_t* meta, uint32_t* tmp, uint32_t* radix_temp, hipStream_t s) {
+  launch_fill_u32(meta, 0u, kLvMetaWords, s);
+  if (n) {
+    uint32_t* lo = tmp;
+    uint32_t* hi = tmp + n;
+    uint32_t* idx = tmp + 2 * uint64_t(n);
+    const dim3 grid{(n + kBlock - 1) / kBlock}, block{kBlock};
+    hipLaunchKernelGGL(k_lv_keys, grid, block, 0, s, g, lv, n, lo, hi, idx, meta);
+    ILC_HIP(hipGetLastError());
+    // (hi, lo) order by two stable sorts: low word, then high word
+    launch_radix_sort(lo, idx, n, 32, radix_temp, s);
+    launch_gather_u32(hi, idx, n, lo, s);
+    launch_radix_sort(lo, idx, n, 30, radix_temp, s);
+    hipLaunchKernelGGL(k_lv_soa, grid, block, 0, s, g, lv, n, idx, key, pos, w, cell);
+    ILC_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_lv_starts, dim3(1), dim3(64), 0, s, meta);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_smooth3d(const grid_view& g, const level_view& lv, const point_view& p, int op,
+                     double* acc, hipStream_t s, unsigned long long* cnt) {
   if (!p.n) return;
-  ILC_REQUIRE(g.pz != nullptr, "the grid is not a volumetric grid");
+  ILC_REQUIRE(g.pz != nullptr && lv.key != nullptr, "the grid is not a volumetric grid");
   ILC_REQUIRE(op == OP_REFERENCE || op == OP_EXACT || op == OP_COUNT, "bad op %d", op);
   ILC_REQUIRE(op == OP_COUNT ? cnt != nullptr : acc != nullptr, "null output");
   const uint32_t tiles_x = (p.lnx + kTile3 - 1) / kTile3;
@@ -427,11 +598,11 @@ void launch_smooth3d(const grid_view& g, const point_view& p, int op, double* ac
   ILC_REQUIRE(tiles < (uint64_t(1) << 31), "too many tiles");
   const dim3 grid{static_cast<uint32_t>(tiles)}, block{kBlock};
   if (op == OP_REFERENCE)
-    hipLaunchKernelGGL((k_smooth3d<true, false>), grid, block, 0, s, g, p, tiles_x, acc, cnt);
+    hipLaunchKernelGGL((k_smooth3d<true, false>), grid, block, 0, s, g, lv, p, tiles_x, acc, cnt);
   else if (op == OP_EXACT)
-    hipLaunchKernelGGL((k_smooth3d<false, false>), grid, block, 0, s, g, p, tiles_x, acc, cnt);
+    hipLaunchKernelGGL((k_smooth3d<false, false>), grid, block, 0, s, g, lv, p, tiles_x, acc, cnt);
   else
-    hipLaunchKernelGGL((k_smooth3d<false, true>), grid, block, 0, s, g, p, tiles_x, acc, cnt);
+    hipLaunchKernelGGL((k_smooth3d<false, true>), grid, block, 0, s, g, lv, p, tiles_x, acc, cnt);
   ILC_HIP(hipGetLastError());
 }
 
