@@ -391,7 +391,8 @@ def volumetric_main(args):
         "config": dict(workload="volumetric smoothing: gas density on 8 slices through the "
                                 "halos of the 16M-cell snapshot", n_cells=snap.n,
                        box_kpc_h=snap.box, slices=len(snap.halos), resolution=[nx, nx],
-                       width_kpc_h=width, mode=args.mode, evals_per_step=total,
+                       width_kpc_h=width, mode=args.mode, particles_per_cell=64,
+                       evals_per_step=total,
                        points_per_step=int(pts.shape[0]), grid_cells=gi["num_cells"],
                        parallelism="single GPU"),
         "points_per_s": pts.shape[0] / (ms_per_step * 1e-3),
@@ -407,7 +408,7 @@ def volumetric_main(args):
     def ogrid():
         if "g" not in ocache:
             t = time.perf_counter()
-            ocache["g"] = oracle.Grid(snap.particles4(), snap.hsml, target=1000)
+            ocache["g"] = oracle.Grid(snap.particles4(), snap.hsml, target=64)
             ocache["t"] = time.perf_counter() - t
         return ocache["g"]
 

@@ -159,9 +159,11 @@ class GridSmoothing:
     """reconstruction_backends::dm::grid (dm_reconstruction_backend_grid.cpp:57-134):
     volumetric smoothing of 3-D particles at 3-D points with the 3-D quartic
     (smoothing_kernel.cl:27-37), e.g. for volumetric slices.  Results
-    accumulate over run() calls until the next setup_evaluation_points."""
+    accumulate over run() calls until the next setup_evaluation_points.  The
+    reference's default is 64 particles per cell
+    (dm_reconstruction_backend_grid.hpp:39)."""
 
-    def __init__(self, device=0, target_particles_per_cell=1000, mode="reference",
+    def __init__(self, device=0, target_particles_per_cell=64, mode="reference",
                  kernel="quartic_ref"):
         cfg = N.PsgConfig()
         cfg.device = int(device)
@@ -238,6 +240,107 @@ class GridSmoothing:
         N.call("ilc_grid3d_count_evals", self._h, _ptr(pp), ctypes.byref(total),
                _stream(stream))
         return (total.value, pp) if per_point else total.value
+
+
+def slice_pixel_points(cam, num_additional_samples=0, seed=0):
+    """Evaluation points of volumetric_slice::create_slice
+    (volumetric_slice.cpp:46-84): camera::get_pixel_coordinate(x, y) =
+    min_position + x ps b0 + y ps b1 in double, stored as float32 (w = 0),
+    row-major; then num_additional_samples more copies jittered by up to half
+    a pixel (numpy's seeded generator; the reference draws from
+    std::random_device).  Returns (samples * nx * ny, 4) float32."""
+    b0, b1, mn, ps = camera_basis(cam)
+    nx, ny = int(cam["nx"]), int(cam.get("ny", cam["nx"]))
+    xs, ys = np.meshgrid(np.arange(nx, dtype=np.float64), np.arange(ny, dtype=np.float64))
+    xs, ys = [xs.reshape(-1)], [ys.reshape(-1)]
+    rng = np.random.default_rng(seed)
+    for _ in range(int(num_additional_samples)):
+        j = rng.uniform(-0.5, 0.5, (ny * nx, 2))
+        xs.append(xs[0] + j[:, 0])
+        ys.append(ys[0] + j[:, 1])
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    pts = (mn[None, :] + (x * ps)[:, None] * b0[None, :]) + (y * ps)[:, None] * b1[None, :]
+    out = np.zeros((x.size, 4), np.float32)
+    out[:, :3] = pts
+    return out
+
+
+def tomography_points(cam, initial_z_step, num_steps):
+    """Evaluation points of the slices of
+    volumetric_tomography::create_tomographic_cube (volumetric_tomography.cpp:
+    66-95): slice k has the camera moved to position + (initial_z_step + k)
+    ps look_at.  Returns (num_steps * nx * ny, 4) float32, slice-major."""
+    ps = float(cam["width"]) / int(cam["nx"])
+    pos = np.asarray(cam["position"], np.float64)
+    la = np.asarray(cam["look_at"], np.float64)
+    out = []
+    for k in range(int(num_steps)):
+        c = dict(cam)
+        c["position"] = tuple(pos + (float(initial_z_step + k) * ps) * la)
+        out.append(slice_pixel_points(c))
+    return np.concatenate(out, 0) if out else np.zeros((0, 4), np.float32)
+
+
+def _smooth_blocks(points, nx, blocks, blocksize, device, grid_kwargs):
+    """The crawler's loop: one grid per block, values accumulated."""
+    blocks = blocks() if callable(blocks) else blocks
+    if blocksize is None:
+        blocks = list(blocks)
+        blocksize = max([int(p.shape[0]) for p, _ in blocks] + [1])
+    g = GridSmoothing(device=device, **grid_kwargs)
+    g.init_backend(int(blocksize))
+    g.setup_evaluation_points(points, layout_nx=nx)
+    for p4, h in blocks:
+        if int(p4.shape[0]) == 0:
+            continue
+        g.setup_particles(p4, [h])
+        g.run()
+    return g.retrieve_results_f64().cpu().numpy()
+
+
+def volumetric_slice(cam, blocks, dV=1.0, num_additional_samples=0, seed=0, blocksize=None,
+                     device=0, **grid_kwargs):
+    """volumetric_slice::create_slice over particle blocks: ``blocks`` yields
+    (particles4, h) per block (world frame, the crawler's output; or a
+    callable returning such an iterable); every block builds its own grid
+    and the values accumulate.  dV: the quantity's effective dV of a pixel
+    volume; blocksize: the largest block (default: found by listing the
+    blocks).  Returns the (ny, nx) float32 slice, the mean over the samples
+    of dV times the reconstruction."""
+    nx, ny = int(cam["nx"]), int(cam.get("ny", cam["nx"]))
+    samples = int(num_additional_samples) + 1
+    pts = slice_pixel_points(cam, num_additional_samples, seed)
+    vals = _smooth_blocks(pts, nx, blocks, blocksize, device, grid_kwargs).astype(np.float32)
+    out = np.zeros(nx * ny, np.float32)
+    for i in range(samples):
+        # float += double, as output[idx] += dV * contribution / samples
+        out = (out.astype(np.float64) + float(dV) * vals[i * nx * ny:(i + 1) * nx * ny]
+               .astype(np.float64) / samples).astype(np.float32)
+    return out.reshape(ny, nx)
+
+
+def volumetric_tomography(cam, blocks, z_range=None, initial_z_step=0, num_steps=None, dV=1.0,
+                          blocksize=None, device=0, max_points_per_pass=1 << 27, **grid_kwargs):
+    """volumetric_tomography::create_tomographic_cube: num_steps slices
+    (default int(z_range / pixel size), at least 1), the camera moved one
+    pixel along look_at per slice.  The points of many slices go to one pass
+    over the blocks (at most max_points_per_pass points), so ``blocks`` must
+    be re-iterable (a sequence or a callable) when the cube needs several.
+    Returns (num_steps, ny, nx) float32."""
+    nx, ny = int(cam["nx"]), int(cam.get("ny", cam["nx"]))
+    ps = float(cam["width"]) / nx
+    if num_steps is None:
+        num_steps = max(1, int(float(z_range) / ps))
+    npix = nx * ny
+    per = max(1, int(max_points_per_pass) // npix)
+    out = np.zeros((int(num_steps), ny, nx), np.float32)
+    for z0 in range(0, int(num_steps), per):
+        nz = min(per, int(num_steps) - z0)
+        pts = tomography_points(cam, initial_z_step + z0, nz)
+        vals = _smooth_blocks(pts, nx, blocks, blocksize, device, grid_kwargs).astype(np.float32)
+        out[z0:z0 + nz] = (float(dV) * vals.astype(np.float64)).astype(np.float32) \
+            .reshape(nz, ny, nx)
+    return out
 
 
 def slice_points(center, normal, width, nx, ny=None, up=(0.0, 0.0, 1.0)):

@@ -442,7 +442,8 @@ ilc_status ilc_grid3d_create(const ilc_psg_config* cfg, ilc_grid3d** out) {
                 ndev);
     std::unique_ptr<ilc_grid3d> h(new ilc_grid3d());
     h->p.device = c.device;
-    h->p.target = c.target_particles_per_cell ? c.target_particles_per_cell : 1000;
+    // dm::grid's default of 64 particles per cell (dm_reconstruction_backend_grid.hpp:39)
+    h->p.target = c.target_particles_per_cell ? c.target_particles_per_cell : 64;
     h->p.mode = c.mode;
     h->p.kernel = c.kernel;
     h->p.ts = tile_shape{16, 1};  // the point layout's default width

@@ -10,6 +10,8 @@
 //                                            projective_smoothing_backend_grid.hpp:28-44
 //   reconstruction_backends::dm::projective_smoothing
 //                                            projective_smoothing_reconstruction.hpp:33-89
+//   reconstruction_backends::dm::grid       dm_reconstruction_backend_grid.hpp:30-70
+//                                            (volumetric smoothing at 3-D points)
 // with two substitutions of types the path does not need:
 //   cl::Buffer   -> illcrawl::device_buffer (HIP device memory, shared ownership
 //                   like cl::Buffer's reference counting)
@@ -129,9 +131,19 @@ class reconstruction_backend {
   virtual void init_backend(std::size_t blocksize) = 0;
   virtual void setup_particles(const std::vector<particle>& particles,
                                const std::vector<device_buffer>& additional_dataset) = 0;
+  // the same for particles already on the device (float4 x, y, z, w, world
+  // frame), e.g. a block the data crawler compacted there
+  virtual void setup_particles(const device_buffer& particles, std::size_t num_particles,
+                               const std::vector<device_buffer>& additional_dataset) = 0;
   virtual void setup_evaluation_points(const device_buffer& evaluation_points,
                                        std::size_t num_points) = 0;
   virtual void run() = 0;
+  // the stream all of the backend's device work goes to
+  virtual hipStream_t stream() const = 0;
+  // Row length of the evaluation points when they form an image, which the
+  // gather tiles; 0 = one row.  Backends that lay out their own points
+  // ignore it.
+  virtual void set_evaluation_layout(std::size_t /*row_length*/) {}
   virtual ~reconstruction_backend() {}
 };
 
@@ -161,6 +173,20 @@ class camera {
 
   std::size_t get_num_pixels(int dim) const { return dim == 0 ? _c.nx : _c.ny; }
   double get_pixel_size() const { return _c.width / _c.nx; }
+  const double* get_position() const { return _c.position; }
+  const double* get_look_at() const { return _c.look_at; }
+  // camera::set_position (camera.cpp:88-92); the screen moves with it
+  void set_position(const double pos[3]) {
+    for (int i = 0; i < 3; ++i) _c.position[i] = pos[i];
+  }
+  // camera::get_pixel_coordinate (camera.cpp:49-61), in double:
+  // min_position + x ps b0 + y ps b1, x and y in pixel units
+  void get_pixel_coordinate(double x, double y, double out[3]) const {
+    double b0[3], b1[3], mn[3], ps = 0.0;
+    check(ilc_camera_basis(&_c, b0, b1, mn, &ps), "camera::get_pixel_coordinate");
+    const double sx = x * ps, sy = y * ps;
+    for (int i = 0; i < 3; ++i) out[i] = (mn[i] + sx * b0[i]) + sy * b1[i];
+  }
   // camera::get_pixel_area (camera.cpp:191-195)
   double get_pixel_area() const { return get_pixel_size() * get_pixel_size(); }
 
@@ -244,6 +270,12 @@ class projective_smoothing_grid : public projective_smoothing_backend {
     setup_projected_particles(_upload, particles.size(), additional_dataset);
   }
 
+  // device particles, already in the camera frame
+  void setup_particles(const device_buffer& particles, std::size_t num_particles,
+                       const std::vector<device_buffer>& additional_dataset) override {
+    setup_projected_particles(particles, num_particles, additional_dataset);
+  }
+
   void setup_projected_particles(const device_buffer& particles, std::size_t num_particles,
                                  const std::vector<device_buffer>& additional_datasets) override {
     if (additional_datasets.size() != 1)
@@ -270,7 +302,7 @@ class projective_smoothing_grid : public projective_smoothing_backend {
 
   // Row length of the evaluation points when they form an image (the gather
   // tiles the image); 0 = one row.  The camera wrapper sets it.
-  void set_evaluation_layout(std::size_t row_length) {
+  void set_evaluation_layout(std::size_t row_length) override {
     _layout_nx = static_cast<uint32_t>(row_length);
   }
 
@@ -289,7 +321,7 @@ class projective_smoothing_grid : public projective_smoothing_backend {
   }
 
   ilc_psg* handle() const noexcept { return _h.get(); }
-  hipStream_t stream() const noexcept { return _o.stream; }
+  hipStream_t stream() const noexcept override { return _o.stream; }
 
  private:
   std::shared_ptr<ilc_psg> _h;
@@ -360,7 +392,7 @@ class projective_smoothing : public reconstruction_backend {
   // frame), e.g. a block the data crawler compacted on the device; the
   // caller's buffer is not modified.
   void setup_particles(const device_buffer& particles, std::size_t num_particles,
-                       const std::vector<device_buffer>& additional_dataset) {
+                       const std::vector<device_buffer>& additional_dataset) override {
     if (particles.count() < num_particles)
       throw error(ILC_EINVAL, "setup_particles: buffer shorter than num_particles");
     if (_projected.count() < num_particles)
@@ -372,7 +404,7 @@ class projective_smoothing : public reconstruction_backend {
     _backend->setup_projected_particles(_projected, num_particles, additional_dataset);
   }
 
-  hipStream_t stream() const { return _stream; }
+  hipStream_t stream() const override { return _stream; }
 
   // world-frame points (e.g. camera::generate_pixel_coordinates); the
   // caller's buffer is not modified
@@ -408,6 +440,101 @@ class projective_smoothing : public reconstruction_backend {
   hipStream_t _stream;
   device_buffer _projected, _points;
   std::size_t _num_points = 0;
+};
+
+// dm::grid: volumetric smoothing of 3-D particles at 3-D evaluation points
+// with the 3-D quartic (dm_reconstruction_backend_grid.cpp:57-134,
+// smoothing_kernel.cl:27-37).  One additional dataset: the smoothing
+// lengths.  Results accumulate over run() calls until the next
+// setup_evaluation_points.
+class grid : public reconstruction_backend {
+ public:
+  struct options {
+    int device = 0;
+    std::size_t target_particles_per_cell = 64;  // dm_reconstruction_backend_grid.hpp:39
+    ilc_mode mode = ILC_MODE_REFERENCE;
+    ilc_kernel kernel = ILC_KERNEL_QUARTIC_REF;
+    hipStream_t stream = nullptr;
+  };
+
+  explicit grid(const dataset_ref& smoothing_lengths) : grid(smoothing_lengths, options()) {}
+
+  grid(const dataset_ref& smoothing_lengths, const options& o) : _hsml(smoothing_lengths), _o(o) {
+    ilc_psg_config cfg{};
+    cfg.device = o.device;
+    cfg.target_particles_per_cell = o.target_particles_per_cell;
+    cfg.mode = o.mode;
+    cfg.kernel = o.kernel;
+    ilc_grid3d* h = nullptr;
+    check(ilc_grid3d_create(&cfg, &h), "dm::grid");
+    _h = std::shared_ptr<ilc_grid3d>(h, [](ilc_grid3d* q) { (void)ilc_grid3d_destroy(q); });
+  }
+
+  std::vector<dataset_ref> get_required_additional_datasets() const override { return {_hsml}; }
+
+  // FP32 values, one per evaluation point; valid until the next
+  // setup_evaluation_points.  Synchronises the stream.
+  const device_buffer& retrieve_results() override {
+    if (_results.count() < _num_points) _results = device_buffer::create<float>(_num_points);
+    check(ilc_grid3d_retrieve_results(_h.get(), 1.0f, _results.as<float>(), _o.stream),
+          "retrieve_results");
+    check_hip(hipStreamSynchronize(_o.stream), "retrieve_results");
+    return _results;
+  }
+
+  std::string get_backend_name() const override { return "dm_smoothing/grid3d (hip gfx950)"; }
+
+  void init_backend(std::size_t blocksize) override {
+    check(ilc_grid3d_init_backend(_h.get(), blocksize), "init_backend");
+  }
+
+  void setup_particles(const std::vector<particle>& particles,
+                       const std::vector<device_buffer>& additional_dataset) override {
+    if (_upload.count() < particles.size())
+      _upload = device_buffer::create<device_vector4>(particles.size());
+    if (!particles.empty())
+      _upload.upload(particles.data(), particles.size() * sizeof(particle), _o.stream);
+    setup_particles(_upload, particles.size(), additional_dataset);
+  }
+
+  void setup_particles(const device_buffer& particles, std::size_t num_particles,
+                       const std::vector<device_buffer>& additional_dataset) override {
+    if (additional_dataset.size() != 1)
+      throw error(ILC_EINVAL, "dm::grid needs one additional dataset (h)");
+    if (particles.count() < num_particles || additional_dataset[0].count() < num_particles)
+      throw error(ILC_EINVAL, "setup_particles: buffers shorter than num_particles");
+    check(ilc_grid3d_setup_particles(_h.get(), particles.as<float>(), num_particles,
+                                     additional_dataset[0].as<float>(), _o.stream),
+          "setup_particles");
+  }
+
+  void setup_evaluation_points(const device_buffer& evaluation_points,
+                               std::size_t num_points) override {
+    if (evaluation_points.count() < num_points)
+      throw error(ILC_EINVAL, "setup_evaluation_points: buffer shorter than num_points");
+    _points = evaluation_points;  // borrowed by the library until the next call
+    _num_points = num_points;
+    check(ilc_grid3d_setup_evaluation_points(_h.get(), evaluation_points.as<float>(), num_points,
+                                             _layout_nx, _o.stream),
+          "setup_evaluation_points");
+  }
+
+  void run() override { check(ilc_grid3d_run(_h.get(), _o.stream), "run"); }
+
+  void set_evaluation_layout(std::size_t row_length) override {
+    _layout_nx = static_cast<uint32_t>(row_length);
+  }
+
+  hipStream_t stream() const noexcept override { return _o.stream; }
+  ilc_grid3d* handle() const noexcept { return _h.get(); }
+
+ private:
+  std::shared_ptr<ilc_grid3d> _h;
+  dataset_ref _hsml;
+  options _o;
+  device_buffer _results, _upload, _points;
+  std::size_t _num_points = 0;
+  uint32_t _layout_nx = 0;
 };
 
 }  // namespace dm

@@ -10,7 +10,9 @@
 // the reference's per-row body on the device (ilc_crawl_block: periodic
 // correction, render-volume cutout, order-preserving compaction of the kept
 // rows and their columns), then the quantity transformation, and hands the
-// block to a projective smoothing reconstruction (setup_particles + run).
+// block to a reconstruction backend (setup_particles + run): the projective
+// smoothing of dm::projective_smoothing or the volumetric smoothing of
+// dm::grid.
 // Blocks accumulate in the backend until retrieve_results, as in the
 // reference.
 #ifndef ILLCRAWL_CRAWLER_HPP
@@ -28,7 +30,8 @@ namespace reconstruction_quantity {
 // A reconstructed quantity: the particle group and datasets it reads, their
 // scaling factors (the unit conversion: input = float(factor * value),
 // quantity.cpp:84-93), the transformation of the scaled inputs into the
-// particle weight w, and the effective dA of the line-of-sight integration.
+// particle weight w, and the effective dA of the line-of-sight integration
+// and dV of a volume integration.
 class quantity {
  public:
   virtual ~quantity() = default;
@@ -41,6 +44,7 @@ class quantity {
   virtual void transform(const float* const* in_dev, std::size_t n, float* w_dev,
                          hipStream_t s) const = 0;
   virtual double effective_line_of_sight_integration_dA(double dA, double range) const = 0;
+  virtual double effective_volume_integration_dV(double dV, double volume) const = 0;
 };
 
 // the gas density as is (unprocessed_quantity, quantities.cl:701-711)
@@ -67,6 +71,10 @@ class projected_density : public density_based_quantity {
  public:
   using density_based_quantity::density_based_quantity;
   double effective_line_of_sight_integration_dA(double, double) const override { return 1.0; }
+  // the mean density over the volume
+  double effective_volume_integration_dV(double dV, double volume) const override {
+    return dV / volume;
+  }
 };
 
 // mass: the same integral times the pixel area (quantity.hpp:577-604)
@@ -74,6 +82,7 @@ class mass : public density_based_quantity {
  public:
   using density_based_quantity::density_based_quantity;
   double effective_line_of_sight_integration_dA(double dA, double) const override { return dA; }
+  double effective_volume_integration_dV(double dV, double) const override { return dV; }
 };
 
 // xray_flux: bremsstrahlung in the band [e_min, e_max] keV from density,
@@ -115,6 +124,8 @@ class xray_flux : public quantity {
           "quantity transformation");
   }
   double effective_line_of_sight_integration_dA(double dA, double) const override { return dA; }
+  // quantity.hpp:193-197
+  double effective_volume_integration_dV(double dV, double) const override { return dV; }
 
  private:
   std::vector<float> _tab;
@@ -127,14 +138,13 @@ class xray_flux : public quantity {
 }  // namespace reconstruction_quantity
 
 // reconstructing_data_crawler: streams the quantity's particle group through
-// a projective smoothing reconstruction in blocks of `blocksize` rows.
+// a reconstruction backend in blocks of `blocksize` rows.
 class reconstructing_data_crawler {
  public:
   // volume: the render volume, the periodic box and the additional border
   // region (reconstructing_data_crawler.cpp:188-193)
   reconstructing_data_crawler(const ilc_volume& volume, io::illustris_data_loader* data,
-                              std::size_t blocksize,
-                              reconstruction_backends::dm::projective_smoothing* backend)
+                              std::size_t blocksize, reconstruction_backend* backend)
       : _vol(volume), _data(data), _blocksize(blocksize), _backend(backend) {
     if (!_data || !_backend) throw error(ILC_EINVAL, "reconstructing_data_crawler: null argument");
     if (!_blocksize) throw error(ILC_EINVAL, "reconstructing_data_crawler: zero blocksize");
@@ -199,6 +209,11 @@ class reconstructing_data_crawler {
     check_hip(hipStreamSynchronize(s), "crawler");
   }
 
+  // the backend's results of the last run (reconstructing_data_crawler.hpp:127);
+  // synchronises
+  const device_buffer& get_reconstruction() { return _backend->retrieve_results(); }
+  reconstruction_backend& get_backend() { return *_backend; }
+
  private:
   void ensure(std::size_t ncol) {
     if (_xyz.count() < 3 * _blocksize) {
@@ -217,7 +232,7 @@ class reconstructing_data_crawler {
   ilc_volume _vol;
   io::illustris_data_loader* _data;
   std::size_t _blocksize;
-  reconstruction_backends::dm::projective_smoothing* _backend;
+  reconstruction_backend* _backend;
   device_buffer _xyz, _particles, _w;
   std::vector<device_buffer> _in, _out;
 };

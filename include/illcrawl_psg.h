@@ -62,8 +62,11 @@ typedef struct ilc_psg ilc_psg;
 
 typedef struct {
   int device;                        /* HIP device ordinal */
-  uint64_t target_particles_per_cell; /* 0 -> 1000, the projective grid's
-                                        (projective_smoothing_backend_grid.cpp:29) */
+  uint64_t target_particles_per_cell; /* 0 -> the reference's default: 1000
+                                        for the projective grid
+                                        (projective_smoothing_backend_grid.cpp:29),
+                                        64 for the volumetric grid
+                                        (dm_reconstruction_backend_grid.hpp:39) */
   int mode;                          /* ilc_mode */
   int kernel;                        /* ilc_kernel */
   int tile_w, tile_h;                /* gather tile in points; 0 -> autotuned default */
@@ -271,7 +274,8 @@ ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t
  * without SPG_PROJECTION), e.g. a volumetric slice (volumetric_slice.cpp).
  * The configuration is ilc_psg_config with a quartic kernel
  * (ILC_KERNEL_QUARTIC_REF: the reference's 1.04445f normalisation;
- * ILC_KERNEL_QUARTIC: 105 / (32 pi)); reference mode keeps the reference's
+ * ILC_KERNEL_QUARTIC: 105 / (32 pi)); target 0 means the reference's 64
+ * particles per cell; reference mode keeps the reference's
  * search box, exact mode sums every particle whose support contains the
  * point.  Results accumulate over run() calls until the next
  * setup_evaluation_points, as for ilc_psg. */

@@ -12,6 +12,12 @@
 //       no border; camera along z through the box centre) into
 //       <dir>/image_<quantity>.fits; xray reads <dir>/gaunt.f32 (gnx*gny,
 //       dims in <dir>/gaunt.dims)
+//   crawl_demo volume <dir> <blocksize> <quantity: density|mass> <nx> <width> <box> <nz> <samples>
+//       volumetric smoothing (dm::grid) of <dir>/snap.hdf5 through the
+//       crawler: camera along z through the box centre; nz = 0 writes the
+//       slice with <samples> additional jittered samples per pixel (seed 7)
+//       to <dir>/slice_<quantity>.fits, nz > 0 the tomographic cube of nz
+//       slices to <dir>/cube_<quantity>.fits
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -20,6 +26,7 @@
 #include <vector>
 
 #include "illcrawl_crawler.hpp"
+#include "illcrawl_volumetric.hpp"
 
 namespace {
 
@@ -112,6 +119,49 @@ int render(const std::string& dir, std::size_t blocksize, const std::string& qna
   return 0;
 }
 
+int volume(const std::string& dir, std::size_t blocksize, const std::string& qname, int nx,
+           double width, double box, int nz, int samples) {
+  namespace dm = illcrawl::reconstruction_backends::dm;
+  illcrawl::io::illustris_data_loader data(dir + "/snap.hdf5", 0u);
+  const double c = 0.5 * box;
+  const double position[3] = {c, c, c}, look_at[3] = {0.0, 0.0, 1.0};
+  illcrawl::camera cam(position, look_at, 0.0, width, nx, nx);
+  dm::grid grid(
+      illcrawl::dataset_ref{illcrawl::io::illustris_data_loader::get_smoothing_length_identifier()});
+  std::unique_ptr<illcrawl::reconstruction_quantity::quantity> q;
+  if (qname == "density")
+    q.reset(new illcrawl::reconstruction_quantity::projected_density());
+  else if (qname == "mass")
+    q.reset(new illcrawl::reconstruction_quantity::mass());
+  else
+    throw std::runtime_error("unknown quantity " + qname);
+  ilc_volume vol{};
+  for (int a = 0; a < 3; ++a) {
+    vol.center[a] = c;
+    vol.extent[a] = box;
+    vol.periodic_wraparound[a] = box;
+  }
+  vol.border = 0.0;
+  illcrawl::reconstructing_data_crawler crawler(vol, &data, blocksize, &grid);
+  std::vector<float> out;
+  std::string path;
+  if (nz == 0) {
+    illcrawl::volumetric_slice(cam).create_slice(crawler, *q, out, std::size_t(samples), 7);
+    path = dir + "/slice_" + qname + ".fits";
+    illcrawl::io::fits<float>(path).save(out.data(), {nx, nx});
+  } else {
+    illcrawl::volumetric_tomography(cam).create_tomographic_cube(crawler, *q, 0, std::size_t(nz),
+                                                                 out);
+    path = dir + "/cube_" + qname + ".fits";
+    illcrawl::io::fits<float>(path).save(out.data(), {nx, nx, nz});
+  }
+  double s = 0;
+  for (float v : out) s += v;
+  std::printf("crawl_demo: volume %s %dx%dx%d, samples %d, blocksize %zu, sum %.9g\n",
+              qname.c_str(), nx, nx, nz, samples, blocksize, s);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -140,6 +190,9 @@ int main(int argc, char** argv) {
       std::printf("\n");
       return f ? 0 : 1;
     }
+    if (mode == "volume" && argc == 10)
+      return volume(argv[2], std::stoul(argv[3]), argv[4], std::stoi(argv[5]), std::stod(argv[6]),
+                    std::stod(argv[7]), std::stoi(argv[8]), std::stoi(argv[9]));
     if (mode == "render" && argc == 8)
       return render(argv[2], std::stoul(argv[3]), argv[4], std::stoi(argv[5]), std::stod(argv[6]),
                     std::stod(argv[7]));

@@ -117,3 +117,52 @@ def test_crawler_from_hdf5_equals_block_replay(B, snap, quantity):
     ssum = abs(got.sum() - ref.sum()) / ref.sum()
     say(f"  rel L-inf {linf:.2e}, rel sum {ssum:.2e}, max {ref.max():.4e}")
     assert linf < 1e-5 and ssum < 1e-6
+
+
+def volume_replay(B, s, nx, nz):
+    """The crawler's blocks as (particles4, h) for the volumetric mirror."""
+    c = 0.5 * BOX
+    out = []
+    for first in range(0, N, BLOCK):
+        sl = slice(first, first + BLOCK)
+        xyz = s["pos"][sl].astype(np.float32)
+        p4, (h, rho) = B.crawl_block((c, c, c), (BOX,) * 3, (BOX,) * 3, 0.0, xyz,
+                                     [s["hsml"][sl], s["rho"][sl]])
+        if p4.shape[0]:
+            p4[:, 3] = B.quantity_scaled(rho, 1.0)
+            out.append((p4, h))
+    return out
+
+
+@pytest.mark.parametrize("quantity,nz", [("density", 0), ("mass", 0), ("density", 5)])
+def test_volumetric_crawler_from_hdf5_equals_block_replay(B, snap, quantity, nz):
+    """volumetric_slice / volumetric_tomography (include/illcrawl_volumetric.hpp)
+    over the crawler with dm::grid, against the same blocks through the
+    Python mirror: the points are the same doubles rounded to float on both
+    sides and every block builds the same grid, so the values agree to the
+    FP32 rounding of the dV scaling."""
+    from illcrawl_amd import io as I
+    nx = 48
+    say(f"case: C++ volumetric crawler over HDF5, quantity={quantity}, nz={nz}")
+    r = subprocess.run([DEMO, "volume", snap["dir"], str(BLOCK), quantity, str(nx), str(WIDTH),
+                        str(BOX), str(nz), "0"], capture_output=True, text=True, timeout=120)
+    say("  " + (r.stdout.strip() or r.stderr.strip()))
+    assert r.returncode == 0, r.stderr
+    name = ("slice_" if nz == 0 else "cube_") + quantity + ".fits"
+    got = I.read_fits(os.path.join(snap["dir"], name)).astype(np.float64)
+    c = 0.5 * BOX
+    cam = dict(position=(c, c, c), look_at=(0.0, 0.0, 1.0), roll=0.0, width=WIDTH, nx=nx, ny=nx)
+    ps = WIDTH / nx
+    dV = 1.0 if quantity == "density" else ps ** 3
+    blocks = volume_replay(B, snap, nx, nz)
+    if nz == 0:
+        ref = B.volumetric_slice(cam, blocks, dV=dV).astype(np.float64)
+    else:
+        ref = B.volumetric_tomography(cam, blocks, num_steps=nz, dV=dV).astype(np.float64)
+    assert got.shape == ref.shape
+    m = ref > 1e-6 * ref.max()
+    assert m.sum() > ref.size // 2
+    linf = float(np.max(np.abs(got[m] - ref[m]) / ref[m]))
+    ssum = abs(got.sum() - ref.sum()) / ref.sum()
+    say(f"  rel L-inf {linf:.2e}, rel sum {ssum:.2e}")
+    assert linf < 1e-6 and ssum < 1e-6

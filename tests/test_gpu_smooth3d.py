@@ -80,7 +80,7 @@ def smooth(B, p4, h, pts, mode, kernel, target, blocks=1):
     return out, g
 
 
-@pytest.mark.parametrize("target", [1000, 150])
+@pytest.mark.parametrize("target", [64, 1000])
 def test_reference_mode_matches_oracle(B, orc, scene, target):
     p4, h, c = scene
     pts = slices(B, c)
@@ -108,7 +108,7 @@ def test_exact_mode_matches_brute_force(B, orc, scene):
     p4, h, c = scene
     pts = slices(B, c, nx=64)
     say(f"smooth3d exact n={p4.shape[0]} points={pts.shape[0]}")
-    got, _ = smooth(B, p4, h, pts, "exact", "quartic", 1000)
+    got, _ = smooth(B, p4, h, pts, "exact", "quartic", 64)
     ref = orc.smooth3d_exact(p4, h, pts)
     linf, s = rel_errors(got, ref)
     say(f"  linf {linf:.2e} sum {s:.2e}")
@@ -137,7 +137,7 @@ def test_points_outside_the_grid_and_scaled_retrieval(B, orc, scene):
     p4, h, c = scene
     far = np.array([[-5000.0, 0.0, 0.0, 0.0], [1e4, 1e4, 1e4, 0.0]], np.float32)
     pts = np.concatenate([far, slices(B, c, nx=32)], 0)
-    got, g = smooth(B, p4, h, pts, "reference", "quartic_ref", 1000)
+    got, g = smooth(B, p4, h, pts, "reference", "quartic_ref", 64)
     assert got[0] == 0.0 and got[1] == 0.0
     f32 = g.retrieve_results(scale=2.0).cpu().numpy()
     np.testing.assert_allclose(f32, (2.0 * got).astype(np.float32), rtol=1e-6)
@@ -175,3 +175,75 @@ def test_count_evals_equals_brute_force(B, scene):
     assert total > 10 * pts.shape[0]
     assert np.all(np.abs(per - ref) <= np.maximum(2, 1e-4 * ref))
     assert abs(total - int(ref.sum())) <= 1e-5 * ref.sum() + 4
+
+
+def blocks_of(p4, h, nblocks):
+    n = p4.shape[0]
+    step = (n + nblocks - 1) // nblocks
+    return [(p4[k:k + step], h[k:k + step]) for k in range(0, n, step)]
+
+
+def test_volumetric_slice_matches_oracle(B, orc, scene):
+    """volumetric_slice over three blocks (one grid each): dV times the sum
+    of the per-block reconstructions at the camera's pixel coordinates."""
+    p4, h, c = scene
+    cam = dict(position=tuple(c), look_at=(0.0, 0.6, 0.8), roll=0.2, width=900.0, nx=48, ny=40)
+    blocks = blocks_of(p4, h, 3)
+    say(f"volumetric slice 48x40, 3 blocks of n={p4.shape[0]}")
+    got = B.volumetric_slice(cam, blocks, dV=2.5)
+    assert got.shape == (40, 48) and got.dtype == np.float32
+    pts = B.slice_pixel_points(cam)
+    ref = np.zeros(pts.shape[0])
+    for bp, bh in blocks:
+        ref += orc.Grid(bp, bh, target=64).smooth3d_ref(pts, form=1)
+    linf, s = rel_errors(got.reshape(-1), 2.5 * ref)
+    say(f"  linf {linf:.2e} sum {s:.2e}")
+    assert linf < LINF_TOL and s < SUM_TOL
+
+
+def test_volumetric_slice_additional_samples_average(B, scene):
+    p4, h, c = scene
+    cam = dict(position=tuple(c), look_at=(0.0, 0.0, 1.0), roll=0.0, width=600.0, nx=32, ny=32)
+    blocks = blocks_of(p4, h, 2)
+    got = B.volumetric_slice(cam, blocks, dV=1.0, num_additional_samples=2, seed=3)
+    pts = B.slice_pixel_points(cam, 2, 3)
+    g = B.GridSmoothing()
+    g.init_backend(max(b[0].shape[0] for b in blocks))
+    g.setup_evaluation_points(pts, layout_nx=32)
+    for bp, bh in blocks:
+        g.setup_particles(bp, [bh])
+        g.run()
+    v = g.retrieve_results().cpu().numpy().astype(np.float64).reshape(3, 32 * 32)
+    # three float += double steps, each rounding to float32
+    np.testing.assert_allclose(got.reshape(-1), v.mean(0), rtol=5e-7, atol=0)
+
+
+def test_tomography_cube_equals_its_slices(B, orc, scene):
+    """The cube renders every slice in one pass over the blocks (and in
+    chunks of slices); each slice equals the slice rendered alone with the
+    camera moved, bit for bit (the 32-row slices keep the 16-row tiles
+    inside one slice), and the oracle within the parity bar."""
+    p4, h, c = scene
+    cam = dict(position=tuple(c - np.array([0.0, 0.0, 40.0])), look_at=(0.0, 0.0, 1.0),
+               roll=0.0, width=640.0, nx=32, ny=32)
+    blocks = blocks_of(p4, h, 2)
+    say("tomography cube 32x32x6, 2 blocks")
+    cube = B.volumetric_tomography(cam, blocks, num_steps=6, initial_z_step=1)
+    chunked = B.volumetric_tomography(cam, blocks, num_steps=6, initial_z_step=1,
+                                      max_points_per_pass=2 * 32 * 32)
+    np.testing.assert_array_equal(cube, chunked)
+    ps = 640.0 / 32
+    for k in range(6):
+        cz = dict(cam)
+        cz["position"] = tuple(np.asarray(cam["position"]) + (1 + k) * ps * np.array([0, 0, 1.0]))
+        np.testing.assert_array_equal(cube[k], B.volumetric_slice(cz, blocks))
+    pts = B.tomography_points(cam, 1, 6)
+    ref = np.zeros(pts.shape[0])
+    for bp, bh in blocks:
+        ref += orc.Grid(bp, bh, target=64).smooth3d_ref(pts, form=1)
+    linf, s = rel_errors(cube.reshape(-1), ref)
+    say(f"  linf {linf:.2e} sum {s:.2e}")
+    assert linf < LINF_TOL and s < SUM_TOL
+    assert cube.shape == (6, 32, 32)
+    with pytest.raises(TypeError):
+        B.volumetric_tomography(cam, blocks)   # neither z_range nor num_steps

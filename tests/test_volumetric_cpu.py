@@ -1,0 +1,61 @@
+"""Evaluation points of volumetric slices and tomographic cubes (host math,
+no GPU): camera::get_pixel_coordinate in double (camera.cpp:49-61), the
+jitter of additional samples (volumetric_slice.cpp:60-80) and the camera
+steps of a cube (volumetric_tomography.cpp:75-81), against closed forms of
+an axis-aligned camera."""
+import numpy as np
+import pytest
+
+B = pytest.importorskip("illcrawl_amd.backend")
+
+
+def cam_z(nx=4, ny=4, width=4.0, pos=(5.0, 6.0, 7.0)):
+    return dict(position=pos, look_at=(0.0, 0.0, 1.0), roll=0.0, width=width, nx=nx, ny=ny)
+
+
+def test_slice_points_axis_aligned_closed_form():
+    # looking along +z: b0 = x, b1 = y; min corner = pos - (nx/2, ny/2) * ps
+    pts = B.slice_pixel_points(cam_z(nx=4, ny=3, width=8.0))
+    assert pts.shape == (12, 4) and pts.dtype == np.float32
+    ps = 2.0
+    for y in range(3):
+        for x in range(4):
+            np.testing.assert_array_equal(pts[y * 4 + x],
+                                          [5.0 - 4.0 + x * ps, 6.0 - 3.0 + y * ps, 7.0, 0.0])
+
+
+def test_slice_points_rotated_camera_is_orthonormal_grid():
+    la = np.array([1.0, 2.0, 2.0]) / 3.0
+    cam = dict(position=(1.0, 2.0, 3.0), look_at=tuple(la), roll=0.3, width=10.0, nx=8, ny=8)
+    pts = B.slice_pixel_points(cam).astype(np.float64)[:, :3].reshape(8, 8, 3)
+    dx = pts[:, 1:] - pts[:, :-1]
+    dy = pts[1:, :] - pts[:-1, :]
+    np.testing.assert_allclose(np.linalg.norm(dx, axis=-1), 1.25, rtol=1e-5)
+    np.testing.assert_allclose(np.linalg.norm(dy, axis=-1), 1.25, rtol=1e-5)
+    np.testing.assert_allclose(dx.reshape(-1, 3) @ la, 0.0, atol=1e-5)
+    np.testing.assert_allclose(dy.reshape(-1, 3) @ la, 0.0, atol=1e-5)
+    # the screen passes through the camera position: its centre is the
+    # position (pixel (nx/2, ny/2) is the centre's lower-left corner)
+    np.testing.assert_allclose(pts[4, 4], cam["position"], atol=1e-5)
+
+
+def test_additional_samples_stay_within_half_a_pixel():
+    cam = cam_z(nx=16, ny=8, width=32.0)
+    pts = B.slice_pixel_points(cam, num_additional_samples=3, seed=5)
+    assert pts.shape == (4 * 128, 4)
+    base = pts[:128].astype(np.float64)
+    for i in range(1, 4):
+        d = (pts[i * 128:(i + 1) * 128].astype(np.float64) - base) / 2.0   # in pixels
+        assert np.all(np.abs(d[:, :2]) <= 0.5) and np.all(d[:, 2] == 0)
+        assert np.std(d[:, 0]) > 0.2                                       # it is jittered
+    again = B.slice_pixel_points(cam, num_additional_samples=3, seed=5)
+    np.testing.assert_array_equal(pts, again)                              # seeded
+
+
+def test_tomography_points_step_one_pixel_along_look_at():
+    cam = cam_z(nx=4, ny=4, width=4.0)
+    pts = B.tomography_points(cam, 2, 3).reshape(3, 16, 4)
+    base = B.slice_pixel_points(cam)
+    for k in range(3):
+        np.testing.assert_array_equal(pts[k][:, :2], base[:, :2])
+        assert np.all(pts[k][:, 2] == np.float32(7.0 + (2 + k) * 1.0))
