@@ -502,8 +502,7 @@ ilc_status ilc_grid3d_setup_particles(ilc_grid3d* g3, const float* particles4_de
     if (h->mode == ILC_MODE_REFERENCE && (g.nc[0] > 65536 || g.nc[1] > 65536))
       fail(ILC_ERANGE, "grid of %d x %d cells in x, y (at most 65536 each)", g.nc[0], g.nc[1]);
     launch_fill_u32(sc.flags.p, 0u, 1, s);
-    enqueue_grid_build(h->grid, sc, p4, hsml_dev, n, g, 0, bb.max_h, h->kernel, sc.flags.p, s,
-                       /*volumetric=*/true);
+    enqueue_volumetric_grid(h->grid, sc, p4, hsml_dev, n, g, bb.max_h, h->kernel, sc.flags.p, s);
     ILC_HIP(hipMemcpyAsync(h->flags_host, sc.flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     ILC_HIP(hipStreamSynchronize(s));
     if (h->flags_host[0]) {

@@ -311,7 +311,6 @@ __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__
                                                        uint32_t n, float norm,
                                                        float4* __restrict__ part,
                                                        float* __restrict__ hs,
-                                                       float* __restrict__ pz,
                                                        uint32_t* __restrict__ count,
                                                        uint32_t* __restrict__ maxh_bits) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
@@ -325,13 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__
     hh = h[i];
     key = skeys[s];
     const float ih2 = 1.0f / (hh * hh);
-    if (pz) {
-      // volumetric smoothing: coef = w K / h^3, z kept
-      part[s] = make_float4(q.x, q.y, q.w * (norm * (ih2 / hh)), ih2);
-      pz[s] = q.z;
-    } else {
-      part[s] = make_float4(q.x, q.y, q.w * (norm * ih2), ih2);
-    }
+    part[s] = make_float4(q.x, q.y, q.w * (norm * ih2), ih2);
     hs[s] = hh;
   }
   // run heads within the wave
@@ -493,10 +486,10 @@ void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits,
 
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
                       const uint32_t* skeys, uint32_t n, float norm, float4* part, float* hs,
-                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s, float* pz) {
+                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_build_soa, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, perm,
-                     skeys, n, norm, part, hs, pz, count, maxh_bits);
+                     skeys, n, norm, part, hs, count, maxh_bits);
   ILC_HIP(hipGetLastError());
 }
 

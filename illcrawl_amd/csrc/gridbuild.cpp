@@ -72,7 +72,6 @@ grid_view grid_store::view() const {
   g.cell_radius = cell_radius;
   g.kz0 = kz0;
   g.ncells = static_cast<uint32_t>(gp.ncells());
-  g.pz = volumetric ? pz.p : nullptr;
   const uint64_t dn = density_n ? density_n : n;
   g.sparse = dn * 5 < uint64_t(g.ncells) * 2 ? 1 : 0;
   return g;
@@ -95,16 +94,16 @@ grid_params plan_grid(const bbox_result& bb, uint64_t n, uint64_t target, int mo
 
 void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                         uint64_t n, const grid_params& g, int32_t kz0, float max_h, int kernel,
-                        uint32_t* err, hipStream_t s, bool volumetric) {
+                        uint32_t* err, hipStream_t s) {
   gs.key.ensure(n);
   gs.perm.ensure(n);
   launch_keys(p4, uint32_t(n), g, gs.key.p, gs.perm.p, err, s);
-  enqueue_grid_from_keys(gs, sc, p4, h, n, g, kz0, max_h, kernel, s, volumetric);
+  enqueue_grid_from_keys(gs, sc, p4, h, n, g, kz0, max_h, kernel, s);
 }
 
 void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                             uint64_t n, const grid_params& g, int32_t kz0, float max_h,
-                            int kernel, hipStream_t s, bool volumetric) {
+                            int kernel, hipStream_t s) {
   const uint64_t ncells = g.ncells();
   gs.ready = false;
   gs.key.ensure(n);
@@ -120,11 +119,9 @@ void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, 
   launch_radix_sort(gs.key.p, gs.perm.p, uint32_t(n), key_bits_for(ncells - 1), sc.temp.p, s);
   launch_fill_u32(sc.counts.p, 0u, ncells + 1, s);
   launch_fill_u32(gs.maxh_bits.p, 0u, ncells, s);
-  if (volumetric) gs.pz.ensure(n);
-  gs.volumetric = volumetric;
-  launch_build_soa(p4, h, gs.perm.p, gs.key.p, uint32_t(n),
-                   volumetric ? kernel_norm_3d(kernel) : kernel_norm(kernel), gs.part.p, gs.hs.p,
-                   sc.counts.p, gs.maxh_bits.p, s, volumetric ? gs.pz.p : nullptr);
+  gs.volumetric = false;
+  launch_build_soa(p4, h, gs.perm.p, gs.key.p, uint32_t(n), kernel_norm(kernel), gs.part.p,
+                   gs.hs.p, sc.counts.p, gs.maxh_bits.p, s);
   launch_exclusive_scan(sc.counts.p, gs.start.p, ncells + 1, sc.temp.p, s);
   gs.gp = g;
   gs.n = n;
@@ -134,17 +131,34 @@ void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, 
   dd = dd + g.cs[2] * g.cs[2];
   gs.cell_radius = 0.5f * std::sqrt(dd);
   gs.kz0 = kz0;
-  if (volumetric) {
-    gs.lv_key.ensure(n);
-    gs.lv_pos.ensure(n);
-    gs.lv_w.ensure(n);
-    gs.lv_cell.ensure(n);
-    gs.lv_meta.ensure(kLvMetaWords);
-    sc.lv_tmp.ensure(3 * std::max<uint64_t>(n, 1));
-    level_cell_sizes(g, max_h, gs.lv_cs);
-    launch_level_index(gs.view(), uint32_t(n), gs.levels(), gs.lv_key.p, gs.lv_pos.p, gs.lv_w.p,
-                       gs.lv_cell.p, gs.lv_meta.p, sc.lv_tmp.p, sc.temp.p, s);
-  }
+  gs.ready = true;
+}
+
+void enqueue_volumetric_grid(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
+                             uint64_t n, const grid_params& g, float max_h, int kernel,
+                             uint32_t* err, hipStream_t s) {
+  gs.ready = false;
+  gs.maxh_bits.ensure(g.ncells());
+  gs.lv_key.ensure(n);
+  gs.lv_pos.ensure(n);
+  gs.lv_w.ensure(n);
+  gs.lv_cell.ensure(n);
+  gs.lv_meta.ensure(kLvMetaWords);
+  sc.lv_tmp.ensure(3 * std::max<uint64_t>(n, 1));
+  sc.temp.ensure(radix_temp_words(n));
+  level_cell_sizes(g, max_h, gs.lv_cs);
+  launch_volumetric_index(p4, h, uint32_t(n), g, max_h, kernel_norm_3d(kernel), gs.levels(),
+                          gs.lv_key.p, gs.lv_pos.p, gs.lv_w.p, gs.lv_cell.p, gs.lv_meta.p,
+                          gs.maxh_bits.p, err, sc.lv_tmp.p, sc.temp.p, s);
+  gs.gp = g;
+  gs.n = n;
+  gs.max_h = max_h;
+  float dd = g.cs[0] * g.cs[0];
+  dd = dd + g.cs[1] * g.cs[1];
+  dd = dd + g.cs[2] * g.cs[2];
+  gs.cell_radius = 0.5f * std::sqrt(dd);
+  gs.kz0 = 0;
+  gs.volumetric = true;
   gs.ready = true;
 }
 

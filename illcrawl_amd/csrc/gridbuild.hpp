@@ -19,8 +19,8 @@ struct grid_store {
   dbuf<uint32_t> perm;        // sorted position -> input index
   dbuf<uint32_t> start;       // ncells + 1
   dbuf<uint32_t> maxh_bits;   // ncells
-  dbuf<float> pz;             // volumetric grids: sorted z
-  // volumetric grids: the level index the gather searches
+  // volumetric grids: the level index the gather searches (part, key, hs,
+  // perm and start are not built for them)
   dbuf<unsigned long long> lv_key;
   dbuf<float4> lv_pos;
   dbuf<float> lv_w;
@@ -35,7 +35,7 @@ struct grid_store {
   // 0 = n (a windowed grid sets the count of the full set it stands for, so
   // that it is enumerated like the full grid)
   uint64_t density_n = 0;
-  bool volumetric = false;  // 3-D smoothing: part.z holds w K / h^3, z in pz
+  bool volumetric = false;  // 3-D smoothing: the level index and maxh_bits only
   bool ready = false;
   grid_view view() const;
   level_view levels() const;
@@ -56,16 +56,22 @@ grid_params plan_grid(const bbox_result& bb, uint64_t n, uint64_t target, int mo
 // Enqueue keys, stable sort, sorted SoA, per-cell counts / max h and cell
 // offsets for n float4 particles p4 with smoothing lengths h.  Out-of-grid
 // keys are counted into err (device uint32, not cleared here).
-// volumetric: the grid of the 3-D smoothing (z kept, coef = w K / h^3)
 void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                         uint64_t n, const grid_params& g, int32_t kz0, float max_h, int kernel,
-                        uint32_t* err, hipStream_t s, bool volumetric = false);
+                        uint32_t* err, hipStream_t s);
 
 // The same from keys already in gs.key (cell keys) and gs.perm (indices into
 // p4 / h), n of them, in the order the within-cell order must follow.
 void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                             uint64_t n, const grid_params& g, int32_t kz0, float max_h,
-                            int kernel, hipStream_t s, bool volumetric = false);
+                            int kernel, hipStream_t s);
+
+// The grid of the 3-D smoothing: the reference grid's parameters and per-cell
+// max h, and the level index (coef = w K / h^3) the volumetric gather
+// searches.  Out-of-grid particles are counted into err.
+void enqueue_volumetric_grid(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
+                             uint64_t n, const grid_params& g, float max_h, int kernel,
+                             uint32_t* err, hipStream_t s);
 
 float kernel_norm(int kernel);
 // the 3-D quartic's normalisation: the reference's 1.04445f (ILC_KERNEL_QUARTIC_REF)

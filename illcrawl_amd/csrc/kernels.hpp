@@ -21,11 +21,10 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32
 uint64_t radix_temp_words(uint64_t n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
                        hipStream_t s);
-// pz (may be NULL): volumetric smoothing keeps z there and sets
-// coef = w norm / h^3 (projected: coef = w norm / h^2)
+// coef = w norm / h^2
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
                       const uint32_t* skeys, uint32_t n, float norm, float4* part, float* hs,
-                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s, float* pz = nullptr);
+                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s);
 // Reorder (key, perm) pairs, n of them, by descending h[perm] (top 16 bits,
 // stable); horder holds 2n words, temp radix_temp_words(n).
 void launch_h_order(const float* h, uint32_t* key, uint32_t* perm, uint32_t n, uint32_t* horder,
@@ -158,7 +157,6 @@ struct grid_view {
   float cell_radius;
   int32_t kz0;  // z layer of the particles (single layer in reference mode)
   uint32_t ncells;
-  const float* pz;  // volumetric smoothing: sorted z (NULL for projected particles)
 };
 
 // Evaluation points laid out as an image (lnx x lny, row-major).
@@ -248,11 +246,15 @@ struct level_view {
 };
 // Fine cell sizes of the levels for a grid of that extent and max h (host).
 void level_cell_sizes(const grid_params& g, float max_h, float cs[kLevels3]);
-// Builds the level index from the grid's sorted arrays (part, pz, hs, key).
-// tmp: 3 n words; radix_temp: radix_temp_words(n).
-void launch_level_index(const grid_view& g, uint32_t n, const level_view& lv,
-                        unsigned long long* key, float4* pos, float* w, int2* cell,
-                        uint32_t* meta, uint32_t* tmp, uint32_t* radix_temp, hipStream_t s);
+// Builds the level index of particles p4 (x, y, z, w) with smoothing lengths
+// h on grid g, and the grid's per-cell max h (maxh_bits, ncells words); err
+// counts particles outside the grid.  norm: the kernel's 3-D normalisation
+// (coef = w norm / h^3).  tmp: 3 n words; radix_temp: radix_temp_words(n).
+void launch_volumetric_index(const float4* p4, const float* h, uint32_t n, const grid_params& g,
+                             float max_h, float norm, const level_view& lv,
+                             unsigned long long* key, float4* pos, float* w, int2* cell,
+                             uint32_t* meta, uint32_t* maxh_bits, uint32_t* err, uint32_t* tmp,
+                             uint32_t* radix_temp, hipStream_t s);
 // op OP_REFERENCE / OP_EXACT add to acc, OP_COUNT adds evaluation counts to cnt
 void launch_smooth3d(const grid_view& g, const level_view& lv, const point_view& p, int op,
                      double* acc, hipStream_t s, unsigned long long* cnt = nullptr);

@@ -35,6 +35,8 @@
 //
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -480,14 +482,33 @@ __global__ __launch_bounds__(kBlock) void k_smooth3d(grid_view g, level_view lv,
   }
 }
 
-// ---- level index build
+// ---- level index build, from the particles in input order
 
-// level and fine key of each particle (grid order), level counts and max h
-__global__ __launch_bounds__(kBlock) void k_lv_keys(grid_view g, level_view lv, uint32_t n,
-                                                     uint32_t* __restrict__ lo,
-                                                     uint32_t* __restrict__ hi,
-                                                     uint32_t* __restrict__ idx,
-                                                     uint32_t* __restrict__ meta) {
+// grid3d_get_cell_indices of a particle (particle_grid.cl) as the grid build
+// computes it; false when the particle lies outside the grid
+__device__ inline bool cell_index_3d(const float4& q, const grid_params& g, int32_t idx[3]) {
+#pragma clang fp contract(off)
+  const float c[3] = {q.x, q.y, q.z};
+  bool ok = true;
+  for (int a = 0; a < 3; ++a) {
+    const float d = c[a] - g.minc[a];
+    idx[a] = floor_to_int(d / g.cs[a]);
+    ok = ok && idx[a] >= 0 && idx[a] < g.nc[a];
+  }
+  return ok;
+}
+
+// per particle: the reference cell's max h (atomic max), the level and its
+// fine key (lo, hi words), the level counts and max h
+__global__ __launch_bounds__(kBlock) void k_vol_keys(const float4* __restrict__ p4,
+                                                      const float* __restrict__ hsml, uint32_t n,
+                                                      grid_params g, float max_h, level_view lv,
+                                                      uint32_t* __restrict__ lo,
+                                                      uint32_t* __restrict__ hi,
+                                                      uint32_t* __restrict__ idx,
+                                                      uint32_t* __restrict__ meta,
+                                                      uint32_t* __restrict__ maxh_bits,
+                                                      uint32_t* __restrict__ err) {
   __shared__ uint32_t cnt[kLevels3], mh[kLevels3];
   __shared__ float scs[kLevels3];
   const int t = threadIdx.x;
@@ -500,20 +521,31 @@ __global__ __launch_bounds__(kBlock) void k_lv_keys(grid_view g, level_view lv, 
     }
   }
   __syncthreads();
-  const uint32_t i = blockIdx.x * kBlock + uint32_t(t);
-  if (i < n) {
-    const float4 pp = g.part[i];
-    const float h = g.hs[i];
-    const int l = level_of(h, g.max_h);
-    const float c = scs[l];
-    const unsigned long long k = fine_key(l, fine_of(pp.x, g.minc[0], c),
-                                          fine_of(pp.y, g.minc[1], c),
-                                          fine_of(g.pz[i], g.minc[2], c));
+  const uint32_t nx = uint32_t(g.nc[0]), nxy = nx * uint32_t(g.nc[1]);
+  for (uint32_t i = blockIdx.x * kBlock + uint32_t(t); i < n; i += gridDim.x * kBlock) {
+    const float4 q = p4[i];
+    const float h = hsml[i];
+    int32_t c[3];
+    if (cell_index_3d(q, g, c)) {
+      // clustered cells hold millions of particles: read the running max
+      // first and go to the atomic unit only with a larger h (a stale read
+      // is never above the true max, so nothing is skipped wrongly)
+      uint32_t* m = &maxh_bits[uint32_t(c[0]) + uint32_t(c[1]) * nx + uint32_t(c[2]) * nxy];
+      const uint32_t hb = __float_as_uint(h);
+      if (hb > __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(m, hb);
+    } else {
+      atomicAdd(err, 1u);
+    }
+    const int l = level_of(h, max_h);
+    const float cs = scs[l];
+    const unsigned long long k = fine_key(l, fine_of(q.x, g.minc[0], cs),
+                                          fine_of(q.y, g.minc[1], cs), fine_of(q.z, g.minc[2], cs));
     lo[i] = uint32_t(k);
     hi[i] = uint32_t(k >> 32);
     idx[i] = i;
     atomicAdd(&cnt[l], 1u);
-    atomicMax(&mh[l], __float_as_uint(h));
+    if (__float_as_uint(h) > mh[l]) atomicMax(&mh[l], __float_as_uint(h));
   }
   __syncthreads();
   if (t < kLevels3 && cnt[t]) {
@@ -532,13 +564,17 @@ __global__ void k_lv_starts(uint32_t* __restrict__ meta) {
   meta[3 * kLevels3] = s;
 }
 
-// the level-ordered arrays from the sorted order
-__global__ __launch_bounds__(kBlock) void k_lv_soa(grid_view g, level_view lv, uint32_t n,
-                                                    const uint32_t* __restrict__ order,
-                                                    unsigned long long* __restrict__ key,
-                                                    float4* __restrict__ pos,
-                                                    float* __restrict__ w,
-                                                    int2* __restrict__ cell) {
+// the level-ordered arrays: key, (x, y, z, 1/h^2), coef = w norm / h^3 and
+// the reference cell
+__global__ __launch_bounds__(kBlock) void k_vol_soa(const float4* __restrict__ p4,
+                                                     const float* __restrict__ hsml, uint32_t n,
+                                                     grid_params g, float max_h, float norm,
+                                                     level_view lv,
+                                                     const uint32_t* __restrict__ order,
+                                                     unsigned long long* __restrict__ key,
+                                                     float4* __restrict__ pos,
+                                                     float* __restrict__ w,
+                                                     int2* __restrict__ cell) {
   __shared__ float scs[kLevels3];
   const int t = threadIdx.x;
   if (t == 0) {
@@ -549,38 +585,46 @@ __global__ __launch_bounds__(kBlock) void k_lv_soa(grid_view g, level_view lv, u
   const uint32_t j = blockIdx.x * kBlock + uint32_t(t);
   if (j >= n) return;
   const uint32_t i = order[j];
-  const float4 pp = g.part[i];  // (x, y, coef, 1/h^2)
-  const float z = g.pz[i];
-  const int l = level_of(g.hs[i], g.max_h);
-  const float c = scs[l];
-  key[j] = fine_key(l, fine_of(pp.x, g.minc[0], c), fine_of(pp.y, g.minc[1], c),
-                    fine_of(z, g.minc[2], c));
-  pos[j] = make_float4(pp.x, pp.y, z, pp.w);
-  w[j] = pp.z;
-  const uint32_t ck = g.key[i];
-  const uint32_t nx = uint32_t(g.nc[0]), nxy = nx * uint32_t(g.nc[1]);
-  const uint32_t cz = ck / nxy, rem = ck - cz * nxy, cy = rem / nx, cx = rem - cy * nx;
-  cell[j] = make_int2(int(cx | (cy << 16)), int(cz));
+  const float4 q = p4[i];
+  const float h = hsml[i];
+  const int l = level_of(h, max_h);
+  const float cs = scs[l];
+  key[j] = fine_key(l, fine_of(q.x, g.minc[0], cs), fine_of(q.y, g.minc[1], cs),
+                    fine_of(q.z, g.minc[2], cs));
+  const float ih2 = 1.0f / (h * h);
+  pos[j] = make_float4(q.x, q.y, q.z, ih2);
+  w[j] = q.w * (norm * (ih2 / h));
+  int32_t c[3];
+  if (!cell_index_3d(q, g, c)) c[0] = c[1] = c[2] = 0;  // reported by k_vol_keys
+  cell[j] = make_int2(int(uint32_t(c[0]) | (uint32_t(c[1]) << 16)), c[2]);
 }
 
 }  // namespace
 
-void launch_level_index(const grid_view& g, uint32_t n, const level_view& lv,
-                        unsigned long long* key, float4* pos, float* w, int2* cell,
-                        uint32_t* meta, uint32_t* tmp, uint32_t* radix_temp, hipStream_t s) {
+void launch_volumetric_index(const float4* p4, const float* h, uint32_t n, const grid_params& g,
+                             float max_h, float norm, const level_view& lv,
+                             unsigned long long* key, float4* pos, float* w, int2* cell,
+                             uint32_t* meta, uint32_t* maxh_bits, uint32_t* err, uint32_t* tmp,
+                             uint32_t* radix_temp, hipStream_t s) {
   launch_fill_u32(meta, 0u, kLvMetaWords, s);
+  launch_fill_u32(maxh_bits, 0u, g.ncells(), s);
   if (n) {
     uint32_t* lo = tmp;
     uint32_t* hi = tmp + n;
     uint32_t* idx = tmp + 2 * uint64_t(n);
-    const dim3 grid{(n + kBlock - 1) / kBlock}, block{kBlock};
-    hipLaunchKernelGGL(k_lv_keys, grid, block, 0, s, g, lv, n, lo, hi, idx, meta);
+    // a capped grid: each workgroup adds its level counts to the same few
+    // global words once
+    const uint32_t nb = std::min<uint32_t>((n + kBlock - 1) / kBlock, 2048u);
+    hipLaunchKernelGGL(k_vol_keys, dim3(nb), dim3(kBlock), 0, s, p4, h, n, g, max_h, lv, lo, hi,
+                       idx, meta, maxh_bits, err);
     ILC_HIP(hipGetLastError());
     // (hi, lo) order by two stable sorts: low word, then high word
     launch_radix_sort(lo, idx, n, 32, radix_temp, s);
     launch_gather_u32(hi, idx, n, lo, s);
     launch_radix_sort(lo, idx, n, 30, radix_temp, s);
-    hipLaunchKernelGGL(k_lv_soa, grid, block, 0, s, g, lv, n, idx, key, pos, w, cell);
+    const dim3 grid{(n + kBlock - 1) / kBlock}, block{kBlock};
+    hipLaunchKernelGGL(k_vol_soa, grid, block, 0, s, p4, h, n, g, max_h, norm, lv, idx, key, pos,
+                       w, cell);
     ILC_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_lv_starts, dim3(1), dim3(64), 0, s, meta);
@@ -590,7 +634,7 @@ void launch_level_index(const grid_view& g, uint32_t n, const level_view& lv,
 void launch_smooth3d(const grid_view& g, const level_view& lv, const point_view& p, int op,
                      double* acc, hipStream_t s, unsigned long long* cnt) {
   if (!p.n) return;
-  ILC_REQUIRE(g.pz != nullptr && lv.key != nullptr, "the grid is not a volumetric grid");
+  ILC_REQUIRE(lv.key != nullptr, "the grid is not a volumetric grid");
   ILC_REQUIRE(op == OP_REFERENCE || op == OP_EXACT || op == OP_COUNT, "bad op %d", op);
   ILC_REQUIRE(op == OP_COUNT ? cnt != nullptr : acc != nullptr, "null output");
   const uint32_t tiles_x = (p.lnx + kTile3 - 1) / kTile3;
