@@ -244,7 +244,7 @@ class GridSmoothing:
 
 def slice_pixel_points(cam, num_additional_samples=0, seed=0):
     """Evaluation points of volumetric_slice::create_slice
-    (volumetric_slice.cpp:46-84): camera::get_pixel_coordinate(x, y) =
+    (volumetric_slice.cpp:51-76): camera::get_pixel_coordinate(x, y) =
     min_position + x ps b0 + y ps b1 in double, stored as float32 (w = 0),
     row-major; then num_additional_samples more copies jittered by up to half
     a pixel (numpy's seeded generator; the reference draws from
@@ -268,7 +268,7 @@ def slice_pixel_points(cam, num_additional_samples=0, seed=0):
 def tomography_points(cam, initial_z_step, num_steps):
     """Evaluation points of the slices of
     volumetric_tomography::create_tomographic_cube (volumetric_tomography.cpp:
-    66-95): slice k has the camera moved to position + (initial_z_step + k)
+    61-102): slice k has the camera moved to position + (initial_z_step + k)
     ps look_at.  Returns (num_steps * nx * ny, 4) float32, slice-major."""
     ps = float(cam["width"]) / int(cam["nx"])
     pos = np.asarray(cam["position"], np.float64)

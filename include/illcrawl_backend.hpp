@@ -10,7 +10,7 @@
 //                                            projective_smoothing_backend_grid.hpp:28-44
 //   reconstruction_backends::dm::projective_smoothing
 //                                            projective_smoothing_reconstruction.hpp:33-89
-//   reconstruction_backends::dm::grid       dm_reconstruction_backend_grid.hpp:30-70
+//   reconstruction_backends::dm::grid       dm_reconstruction_backend_grid.hpp:32-79
 //                                            (volumetric smoothing at 3-D points)
 // with two substitutions of types the path does not need:
 //   cl::Buffer   -> illcrawl::device_buffer (HIP device memory, shared ownership

@@ -121,7 +121,7 @@ ilc_status ilc_psg_init_backend(ilc_psg* h, uint64_t blocksize);
 /* projective_smoothing_backend::setup_projected_particles
  * (projective_smoothing_backend.hpp:35-42;
  * projective_smoothing_backend_grid.cpp:33-45, which continues into
- * dm::grid::setup_particles, dm_reconstruction_backend_grid.cpp:63-85, and
+ * dm::grid::setup_particles, dm_reconstruction_backend_grid.cpp:65-85, and
  * the smoothing_particle_grid / particle_grid constructors).
  * particles4_dev: n float4 (x, y, z, w) in the camera frame; hsml_dev: n
  * smoothing lengths (the backend's one additional dataset).  Builds the
@@ -283,7 +283,7 @@ typedef struct ilc_grid3d ilc_grid3d;
 ilc_status ilc_grid3d_create(const ilc_psg_config* cfg, ilc_grid3d** out);
 ilc_status ilc_grid3d_destroy(ilc_grid3d* h);
 ilc_status ilc_grid3d_init_backend(ilc_grid3d* h, uint64_t blocksize);
-/* dm::grid::setup_particles (dm_reconstruction_backend_grid.cpp:63-85):
+/* dm::grid::setup_particles (dm_reconstruction_backend_grid.cpp:65-85):
  * particles4_dev (x, y, z, w) world frame, hsml_dev one h per particle;
  * builds the 3-D grid (particle_grid.cpp:43-222), synchronises. */
 ilc_status ilc_grid3d_setup_particles(ilc_grid3d* h, const float* particles4_dev, uint64_t n,

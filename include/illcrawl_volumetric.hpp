@@ -1,8 +1,8 @@
 // illcrawl_volumetric.hpp -- volumetric slices and tomographic cubes over the
 // data crawler with a volumetric backend (reconstruction_backends::dm::grid).
 //
-//   illcrawl::volumetric_slice       volumetric_slice.hpp:30-45, volumetric_slice.cpp:29-115
-//   illcrawl::volumetric_tomography  volumetric_tomography.hpp, volumetric_tomography.cpp:27-110
+//   illcrawl::volumetric_slice       volumetric_slice.hpp, volumetric_slice.cpp:31-108
+//   illcrawl::volumetric_tomography  volumetric_tomography.hpp, volumetric_tomography.cpp:29-102
 //
 // A slice evaluates the reconstruction at the camera's pixel positions
 // (camera::get_pixel_coordinate, computed in double and stored as float) and
@@ -138,7 +138,7 @@ class volumetric_tomography {
       const std::size_t nz = std::min(per_pass, num_steps - z0);
       std::vector<device_vector4> pts(nz * npix);
       for (std::size_t k = 0; k < nz; ++k) {
-        // volumetric_tomography.cpp:79-80: position + step * pixel size * look_at
+        // volumetric_tomography.cpp:84-86: position + step * pixel size * look_at
         const double step = static_cast<double>(initial_z_step + z0 + k) * ps;
         double p[3];
         for (int i = 0; i < 3; ++i) p[i] = pos[i] + step * la[i];

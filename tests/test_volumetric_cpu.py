@@ -1,7 +1,7 @@
 """Evaluation points of volumetric slices and tomographic cubes (host math,
 no GPU): camera::get_pixel_coordinate in double (camera.cpp:49-61), the
-jitter of additional samples (volumetric_slice.cpp:60-80) and the camera
-steps of a cube (volumetric_tomography.cpp:75-81), against closed forms of
+jitter of additional samples (volumetric_slice.cpp:64-76) and the camera
+steps of a cube (volumetric_tomography.cpp:81-86), against closed forms of
 an axis-aligned camera."""
 import numpy as np
 import pytest
