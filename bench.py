@@ -50,7 +50,9 @@ FLOP_PER_EVAL_3D = 13
 # HBM bytes per k_gather_batch launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs of this workload scaled by a 1 GiB copy (profiles/k_gather_batch_xray_16M_64views.md);
 # keyed by (workload, mode, tile)
-MEASURED_TRAFFIC = {("c3", "reference", (16, 16)): dict(read=21.33e9, write=1.784e9)}
+MEASURED_TRAFFIC = {("c3", "reference", (16, 16)): dict(read=21.33e9, write=1.784e9),
+                    # k_smooth3d, profiles/k_smooth3d_slices_16M.md
+                    ("v1", "reference", (16, 16)): dict(read=1.397e9, write=0.268e9)}
 
 
 def log(msg):
@@ -380,6 +382,7 @@ def volumetric_main(args):
     gather_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
     log(f"[bench] v1: {ms_per_step:.2f} ms/step, gather {gather_ms:.2f} ms")
     achieved = total * FLOP_PER_EVAL_3D / (gather_ms * 1e-3) / 1e12
+    traffic = MEASURED_TRAFFIC.get(("v1", args.mode, (16, 16)))
     gi = g.grid_info()
     line = {
         "metric": "particle-kernel evals/s, volumetric SPH smoothing (3-D grid) on MI355X",
@@ -398,7 +401,9 @@ def volumetric_main(args):
         "points_per_s": pts.shape[0] / (ms_per_step * 1e-3),
         "gather_ms": gather_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic["read"] + traffic["write"] if traffic else None,
+                     "traffic_split": traffic,
                      "kernel": "k_smooth3d", "flop_per_eval": FLOP_PER_EVAL_3D,
                      "evals_per_launch": total},
     }

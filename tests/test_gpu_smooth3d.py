@@ -247,3 +247,37 @@ def test_tomography_cube_equals_its_slices(B, orc, scene):
     assert cube.shape == (6, 32, 32)
     with pytest.raises(TypeError):
         B.volumetric_tomography(cam, blocks)   # neither z_range nor num_steps
+
+
+@pytest.mark.parametrize("normal", [(0.0, 0.0, 1.0), (1.0, 1.0, 1.0)])
+def test_coarse_tiles_use_plane_and_box_ranges(B, orc, normal):
+    """Pixels far coarser than h: a tile spans more than 64 key rows of the
+    small-h levels, so the gather takes one key range per z plane (a slice
+    normal to z) or one range for the whole box (an oblique slice).  h spans
+    2^30, so the smallest particles fall into the clamped level 0."""
+    rng = np.random.default_rng(31)
+    n = 20000
+    p4 = np.concatenate([rng.uniform(0, 100, (n, 3)), rng.lognormal(0, 0.3, (n, 1))],
+                        1).astype(np.float32)
+    h = rng.uniform(0.8, 1.6, n).astype(np.float32)
+    h[:40] = 30.0
+    h[40:60] = 1e-7
+    # the slice, then one point on each tiny particle
+    on_tiny = np.concatenate([p4[40:60, :3], np.zeros((20, 1), np.float32)], 1)
+    pts = np.concatenate([B.slice_points((50.0, 50.0, 50.0), normal, 400.0, 32), on_tiny], 0)
+    say(f"coarse tiles normal={normal}: n={n}, points={pts.shape[0]}")
+    cases = [("reference", "quartic_ref",
+              lambda: orc.Grid(p4, h, target=64).smooth3d_ref(pts, form=1)),
+             ("exact", "quartic", lambda: orc.smooth3d_exact(p4, h, pts))]
+    for mode, kernel, oracle_values in cases:
+        got, _ = smooth(B, p4, h, pts, mode, kernel, 64)
+        ref = oracle_values()
+        # the slice (values of order 1) and the points on the tiny particles
+        # (~1e21 w) separately, so that neither hides the other
+        for part, sl in (("slice", slice(0, -20)), ("on tiny", slice(-20, None))):
+            linf, s = rel_errors(got[sl], ref[sl])
+            say(f"  {mode} {part}: linf {linf:.2e} sum {s:.2e}")
+            assert linf < LINF_TOL and s < SUM_TOL
+        # h ~ 1 against 12.5-unit pixels: a few dozen slice points see particles
+        assert (ref[:-20] > 0).sum() > 40
+        assert np.all(got[-20:] > 1e20 * p4[40:60, 3])
