@@ -71,6 +71,7 @@ struct view_state {
   int group = -1;         // slab group (views with identical projected x', y')
   window_f win{};
   uint32_t nwin = 0;      // particles kept by the filter (from the plan)
+  uint32_t nwin_plain = 0;  // the same without the screen test (single-view builds)
   dbuf<uint32_t> orig;    // filtered grids: snapshot index of each grid input row
   // the view's grid as the last render built it: its own grid, or a slice of
   // the batch's shared grid arrays
@@ -143,7 +144,7 @@ struct ilc_batch {
   pinned<view_build_desc> bdesc_host;
   dbuf<int32_t> fviews_dev;
   pinned<int32_t> fviews_host;
-  dbuf<uint32_t> wcounts_all;
+  dbuf<uint32_t> wcounts_all;  // per (view, window tile); the plain counts follow
   // shared build of every view's grid (persistent renders): one scatter of
   // every view's kept particles, one sort by (view, cell, descending h), one
   // SoA pass; a view's grid is a slice
@@ -372,6 +373,19 @@ void plan(ilc_batch* b, hipStream_t s) {
                         std::nextafter(float(hx + reach), INFINITY),
                         std::nextafter(float(-hy - reach), -INFINITY),
                         std::nextafter(float(hy + reach), INFINITY), 1, 0};
+      // the screen: every pixel point's camera-frame coordinates, with a
+      // margin for their FP32 rounding; rows whose support disk (the
+      // gather's cull, with the count pass's slack) misses it are not kept
+      double cmax = 0.0;
+      for (int a = 0; a < 3; ++a) cmax = std::max(cmax, std::fabs(vs.v.cam.position[a]));
+      const double mg = 1e-5 * (cmax + 2.0 * std::max(hx, hy) + double(vs.max_h));
+      const float sup = b->kernel == ILC_KERNEL_CUBIC ? 2.0f : 1.0f;
+      vs.win.use_screen = 1;
+      vs.win.sx0 = std::nextafter(float(-hx - mg), -INFINITY);
+      vs.win.sx1 = std::nextafter(float(hx + mg), INFINITY);
+      vs.win.sy0 = std::nextafter(float(-hy - mg), -INFINITY);
+      vs.win.sy1 = std::nextafter(float(hy + mg), INFINITY);
+      vs.win.cull2 = sup * sup * 1.0001f;
     } else if (vs.compact) {
       vs.win = window_f{0.f, 0.f, 0.f, 0.f, 0, 1};
     } else {
@@ -382,9 +396,9 @@ void plan(ilc_batch* b, hipStream_t s) {
     // kept counts per window tile of every view (unfiltered views keep
     // all), in one pass; the render scans and scatters from them
     const uint32_t nt = window_tiles(b->ps.n);
-    b->wcounts_all.ensure(nv * size_t(nt));
-    b->wtotal.ensure(nv);
-    b->wtotal_host.ensure(nv);
+    b->wcounts_all.ensure(2 * nv * size_t(nt));
+    b->wtotal.ensure(2 * nv);
+    b->wtotal_host.ensure(2 * nv);
     b->fviews_host.ensure(nv);
     b->fviews_dev.ensure(nv);
     int nf = 0;
@@ -397,10 +411,11 @@ void plan(ilc_batch* b, hipStream_t s) {
                            hipMemcpyHostToDevice, s));
     ILC_HIP(hipMemcpyAsync(b->fviews_dev.p, b->fviews_host.p, sizeof(int32_t) * size_t(nf),
                            hipMemcpyHostToDevice, s));
-    launch_fill_u32(b->wtotal.p, 0u, nv, s);
-    launch_multi_window_count(b->ps.pos3, b->ps.n, b->bdesc_dev.p, b->fviews_dev.p, nf,
-                              b->wcounts_all.p, b->wtotal.p, s);
-    ILC_HIP(hipMemcpyAsync(b->wtotal_host.p, b->wtotal.p, sizeof(uint32_t) * nv,
+    launch_fill_u32(b->wtotal.p, 0u, 2 * nv, s);
+    launch_multi_window_count(b->ps.pos3, b->ps.hsml, b->ps.n, b->bdesc_dev.p, b->fviews_dev.p,
+                              nf, b->wcounts_all.p, b->wtotal.p,
+                              b->wcounts_all.p + nv * size_t(nt), b->wtotal.p + nv, s);
+    ILC_HIP(hipMemcpyAsync(b->wtotal_host.p, b->wtotal.p, sizeof(uint32_t) * 2 * nv,
                            hipMemcpyDeviceToHost, s));
     ILC_HIP(hipStreamSynchronize(s));
   }
@@ -411,7 +426,9 @@ void plan(ilc_batch* b, hipStream_t s) {
     view_state& vs = *b->views[v];
     const bool filtered = vs.windowed || vs.compact;
     vs.nwin = filtered ? b->wtotal_host.p[v] : 0u;
+    vs.nwin_plain = filtered ? b->wtotal_host.p[nv + v] : 0u;
     sig.push_back(filtered ? uint64_t(vs.nwin) : ~uint64_t(0));
+    sig.push_back(uint64_t(vs.nwin_plain));
     sig.push_back(uint64_t(int64_t(vs.group)));
     vs.sbase = ntot;
     vs.cbase = ncells;
@@ -511,15 +528,15 @@ void build_view(ilc_batch* b, view_state& vs, const build_ctx& c) {
   if (vs.windowed || vs.compact) {
     // kept particles in input order, then the grid from their keys
     const uint32_t nt = window_tiles(b->ps.n);
-    const uint64_t m = vs.nwin;
+    const uint64_t m = vs.nwin_plain;
     c.woffs->ensure(size_t(nt) + 1);
     c.htmp->ensure(std::max<uint64_t>(m, 1));
     vs.grid.key.ensure(std::max<uint64_t>(m, 1));
     vs.grid.perm.ensure(std::max<uint64_t>(m, 1));
     c.sc->temp.ensure(scan_temp_words(uint64_t(nt) + 1));
     // the plan counted the kept particles of every window tile of this view
-    launch_exclusive_scan(b->wcounts_all.p + view_index(b, vs) * size_t(nt), c.woffs->p, nt,
-                          c.sc->temp.p, st);
+    launch_exclusive_scan(b->wcounts_all.p + (b->views.size() + view_index(b, vs)) * size_t(nt),
+                          c.woffs->p, nt, c.sc->temp.p, st);
     vs.orig.ensure(std::max<uint64_t>(m, 1));
     launch_window_scatter(b->ps, proj_of(b, vs), gaunt_of(b), vs.win, c.woffs->p, vs.gp,
                           c.p4tmp->p, c.htmp->p, vs.grid.key.p, vs.grid.perm.p, vs.orig.p,
@@ -566,10 +583,13 @@ void build_shared(ilc_batch* b, hipStream_t s) {
     b->sh_wx.ensure(b->ps.n);
     launch_particle_xray(b->ps, gaunt_of(b), b->sh_wx.p, s);
   }
+  // per-cell max h starts at zero: the scatter raises it for the rows its
+  // screen test leaves out, the SoA pass for the kept rows
+  launch_fill_u32(b->sh_maxh.p, 0u, nc, s);
   launch_multi_window_scatter(b->ps, any_xray ? b->sh_wx.p : nullptr, b->bdesc_dev.p,
                               b->fviews_dev.p, static_cast<int>(nv), b->sh_offs.p, b->sh_p4c.p,
                               b->sh_hc.p, b->sh_ckey.p, b->sh_orig.p, b->sh_vid.p,
-                              b->counter.p + 1, s);
+                              b->counter.p + 1, b->sh_cbase.p, b->sh_maxh.p, s);
   // stable sorts by descending h, cell, view: a view's rows end up in one
   // range, by cell, biggest h first, ties in input order
   launch_hq_keys(b->sh_hc.p, n32, b->sh_sk.p, b->sh_perm.p, s);
@@ -585,7 +605,6 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   launch_final_keys(b->sh_perm.p, b->sh_ckey.p, b->sh_vid.p, b->sh_cbase.p, n32, b->sh_key.p,
                     b->sh_sk.p, s);
   launch_fill_u32(b->sh_counts.p, 0u, nc + 1, s);
-  launch_fill_u32(b->sh_maxh.p, 0u, nc, s);
   launch_build_soa(b->sh_p4c.p, b->sh_hc.p, b->sh_perm.p, b->sh_sk.p, n32, kernel_norm(b->kernel),
                    b->sh_part.p, b->sh_hs.p, b->sh_counts.p, b->sh_maxh.p, s);
   launch_exclusive_scan(b->sh_counts.p, b->sh_start.p, nc + 1, b->sh_temp.p, s);

@@ -372,15 +372,27 @@ __global__ __launch_bounds__(kBlock) void k_window_count(const float* __restrict
 }
 
 // counts of k_window_count for every listed view, from one read of the tile
+// the support disk (radius^2 = cull2 h^2) of a particle at (ax, ay) meets
+// the screen rectangle: the gather's tile cull with the screen for the tile
+// box (every tile box lies inside the rectangle, so a particle any tile keeps
+// passes here)
+__device__ inline bool reaches_screen(const window_f& win, float ax, float ay, float ih2) {
+  const float dx = fmaxf(fmaxf(win.sx0 - ax, ax - win.sx1), 0.0f);
+  const float dy = fmaxf(fmaxf(win.sy0 - ay, ay - win.sy1), 0.0f);
+  const float d2 = fmaf(dy, dy, dx * dx);
+  return d2 * ih2 < win.cull2;
+}
+
 __global__ __launch_bounds__(kBlock) void k_multi_window_count(
-    const float* __restrict__ pos3, uint64_t n, const view_build_desc* __restrict__ vd,
-    const int32_t* __restrict__ views, int nviews, uint32_t ntiles, uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ totals) {
+    const float* __restrict__ pos3, const float* __restrict__ hsml, uint64_t n,
+    const view_build_desc* __restrict__ vd, const int32_t* __restrict__ views, int nviews,
+    uint32_t ntiles, uint32_t* __restrict__ counts, uint32_t* __restrict__ totals,
+    uint32_t* __restrict__ counts_plain, uint32_t* __restrict__ totals_plain) {
 #pragma clang fp contract(off)
-  __shared__ uint32_t wc[2][4];
+  __shared__ uint32_t wc[2][8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
-  float px[kWinItems], py[kWinItems], pz[kWinItems];
+  float px[kWinItems], py[kWinItems], pz[kWinItems], ih2[kWinItems];
   bool in[kWinItems];
 #pragma unroll
   for (int r = 0; r < kWinItems; ++r) {
@@ -389,38 +401,51 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
     px[r] = in[r] ? pos3[3 * i] : 0.0f;
     py[r] = in[r] ? pos3[3 * i + 1] : 0.0f;
     pz[r] = in[r] ? pos3[3 * i + 2] : 0.0f;
+    const float h = in[r] ? hsml[i] : 1.0f;
+    ih2[r] = 1.0f / (h * h);  // as the grid's SoA holds it
   }
   for (int k = 0; k < nviews; ++k) {
     const int v = views[k];
     const view_proj vp = vd[v].vp;
     const window_f win = vd[v].win;
-    uint32_t c = 0;
+    uint32_t c = 0, cp = 0;
     if (!win.use_window && !win.use_depth) {
 #pragma unroll
       for (int r = 0; r < kWinItems; ++r) c += __popcll(__ballot(in[r]));
+      cp = c;
     } else {
 #pragma unroll
       for (int r = 0; r < kWinItems; ++r) {
         const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
         bool keep = in[r];
+        float ax = 0.0f, ay = 0.0f;
         if (win.use_window) {
-          const float ax = dot3(vp.m + 0, vx, vy, vz), ay = dot3(vp.m + 3, vx, vy, vz);
+          ax = dot3(vp.m + 0, vx, vy, vz);
+          ay = dot3(vp.m + 3, vx, vy, vz);
           keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
         }
         if (win.use_depth) {
           const float az = dot3(vp.m + 6, vx, vy, vz);
           keep = keep && !(az < 0.0f || az > vp.depth);
         }
+        cp += __popcll(__ballot(keep));
+        if (win.use_screen) keep = keep && reaches_screen(win, ax, ay, ih2[r]);
         c += __popcll(__ballot(keep));
       }
     }
     const int slot = k & 1;
-    if (lane == 0) wc[slot][wave] = c;
+    if (lane == 0) {
+      wc[slot][wave] = c;
+      wc[slot][4 + wave] = cp;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
       const uint32_t t = wc[slot][0] + wc[slot][1] + wc[slot][2] + wc[slot][3];
+      const uint32_t tp = wc[slot][4] + wc[slot][5] + wc[slot][6] + wc[slot][7];
       counts[uint64_t(v) * ntiles + blockIdx.x] = t;
+      counts_plain[uint64_t(v) * ntiles + blockIdx.x] = tp;
       if (t) atomicAdd(totals + v, t);
+      if (tp) atomicAdd(totals_plain + v, tp);
     }
   }
 }
@@ -453,7 +478,8 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     const int32_t* __restrict__ views, int nviews, uint32_t ntiles,
     const uint32_t* __restrict__ offs, float4* __restrict__ p4c, float* __restrict__ hc,
     uint32_t* __restrict__ ckey, uint32_t* __restrict__ orig, uint32_t* __restrict__ vid,
-    uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ err, const uint32_t* __restrict__ cbase,
+    uint32_t* __restrict__ maxh_bits) {
 #pragma clang fp contract(off)
   __shared__ __align__(16) uint32_t wc[2][kWinItems][4];
   // the tile's positions, read once (SoA)
@@ -469,18 +495,55 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     sz[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i + 2] : 0.0f;
   }
   __syncthreads();
+  // cell key of (x', y', 0) in a view's grid: grid3d_get_cell_indices,
+  // particle_grid.cl:45-53
+  auto cell_key = [](const grid_params& g, float ax, float ay, bool* in_grid) {
+    const float c3[3] = {ax, ay, 0.0f};
+    int32_t idx[3];
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+      const float d = c3[a] - g.minc[a];
+      idx[a] = floor_to_int(d / g.cs[a]);
+      ok = ok && idx[a] >= 0 && idx[a] < g.nc[a];
+    }
+    *in_grid = ok;
+    return ok ? uint32_t(idx[0]) + uint32_t(idx[1]) * uint32_t(g.nc[0]) +
+                    uint32_t(idx[2]) * uint32_t(g.nc[0]) * uint32_t(g.nc[1])
+              : 0u;
+  };
   // the filter computes only the coordinates it tests (the flags are
-  // uniform); the kept path recomputes all three
-  auto kept = [](const view_proj& vp, const window_f& win, float x, float y, float z, bool ok) {
+  // uniform); the kept path recomputes all three.  A row inside the window
+  // and depth cut whose support cannot reach the screen is not kept; it
+  // raises its cell's max h, which the search box of the reference reads.
+  auto kept = [&](const view_proj& vp, const window_f& win, const grid_params& g, uint32_t cb,
+                  float x, float y, float z, bool ok, uint64_t i) {
     const float vx = x - vp.c[0], vy = y - vp.c[1], vz = z - vp.c[2];
     bool keep = ok;
+    float ax = 0.0f, ay = 0.0f;
     if (win.use_window) {
-      const float ax = dot3(vp.m + 0, vx, vy, vz), ay = dot3(vp.m + 3, vx, vy, vz);
+      ax = dot3(vp.m + 0, vx, vy, vz);
+      ay = dot3(vp.m + 3, vx, vy, vz);
       keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
     }
     if (win.use_depth) {
       const float az = dot3(vp.m + 6, vx, vy, vz);
       keep = keep && !(az < 0.0f || az > vp.depth);
+    }
+    if (win.use_screen && keep) {
+      const float h = ps.hsml[i];
+      if (!reaches_screen(win, ax, ay, 1.0f / (h * h))) {
+        keep = false;
+        bool in_grid;
+        const uint32_t key = cell_key(g, ax, ay, &in_grid);
+        if (in_grid) {
+          uint32_t* m = maxh_bits + cb + key;
+          const uint32_t hb = __float_as_uint(h);
+          if (hb > __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(m, hb);
+        } else {
+          atomicAdd(err, 1u);
+        }
+      }
     }
     return keep;
   };
@@ -488,13 +551,14 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     const int v = views[k];
     const view_proj vp = vd[v].vp;
     const window_f win = vd[v].win;
+    const uint32_t cb = win.use_screen ? cbase[v] : 0u;
     const int slot = k & 1;
     uint32_t bits = 0;  // this thread's kept rows
 #pragma unroll
     for (int r = 0; r < kWinItems; ++r) {
       const int t = r * kBlock + threadIdx.x;
       const bool ok = base + uint64_t(t) < ps.n;
-      const bool keep = kept(vp, win, sx[t], sy[t], sz[t], ok);
+      const bool keep = kept(vp, win, vd[v].g, cb, sx[t], sy[t], sz[t], ok, base + uint64_t(t));
       bits |= keep ? 1u << r : 0u;
       const uint32_t c = __popcll(__ballot(keep));
       if (lane == 0) wc[slot][r][wave] = c;
@@ -525,21 +589,9 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
         if (az < 0.0f || az > vp.depth) w = 0.0f;
         p4c[j] = make_float4(ax, ay, 0.0f, w);
         hc[j] = ps.hsml[i];
-        // cell key of (x', y', 0): grid3d_get_cell_indices, particle_grid.cl:45-53
-        const float c3[3] = {ax, ay, 0.0f};
-        int32_t idx[3];
-        bool in_grid = true;
-        for (int a = 0; a < 3; ++a) {
-          const float d = c3[a] - g.minc[a];
-          idx[a] = floor_to_int(d / g.cs[a]);
-          in_grid = in_grid && idx[a] >= 0 && idx[a] < g.nc[a];
-        }
-        uint32_t key = 0;
-        if (in_grid)
-          key = uint32_t(idx[0]) + uint32_t(idx[1]) * uint32_t(g.nc[0]) +
-                uint32_t(idx[2]) * uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
-        else
-          atomicAdd(err, 1u);
+        bool in_grid;
+        const uint32_t key = cell_key(g, ax, ay, &in_grid);
+        if (!in_grid) atomicAdd(err, 1u);
         ckey[j] = key;
         orig[j] = uint32_t(i);
         vid[j] = uint32_t(v);
@@ -704,11 +756,11 @@ void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  const view_build_desc* vd_dev, const int32_t* views_dev,
                                  int nviews, const uint32_t* offs, float4* p4c, float* hc,
                                  uint32_t* ckey, uint32_t* orig, uint32_t* vid, uint32_t* err,
-                                 hipStream_t s) {
+                                 const uint32_t* cbase, uint32_t* maxh_bits, hipStream_t s) {
   const uint32_t nb = window_tiles(ps.n);
   if (!nb || nviews <= 0) return;
   hipLaunchKernelGGL(k_multi_window_scatter, dim3(nb), dim3(kBlock), 0, s, ps, wx, vd_dev,
-                     views_dev, nviews, nb, offs, p4c, hc, ckey, orig, vid, err);
+                     views_dev, nviews, nb, offs, p4c, hc, ckey, orig, vid, err, cbase, maxh_bits);
   ILC_HIP(hipGetLastError());
 }
 
@@ -751,13 +803,14 @@ void launch_multi_bbox(const particle_soa& ps, const view_build_desc* vd_dev, in
   ILC_HIP(hipGetLastError());
 }
 
-void launch_multi_window_count(const float* pos3, uint64_t n, const view_build_desc* vd_dev,
-                               const int32_t* views_dev, int nviews, uint32_t* counts,
-                               uint32_t* totals, hipStream_t s) {
+void launch_multi_window_count(const float* pos3, const float* hsml, uint64_t n,
+                               const view_build_desc* vd_dev, const int32_t* views_dev,
+                               int nviews, uint32_t* counts, uint32_t* totals,
+                               uint32_t* counts_plain, uint32_t* totals_plain, hipStream_t s) {
   const uint32_t nb = window_tiles(n);
   if (!nb || nviews <= 0) return;
-  hipLaunchKernelGGL(k_multi_window_count, dim3(nb), dim3(kBlock), 0, s, pos3, n, vd_dev,
-                     views_dev, nviews, nb, counts, totals);
+  hipLaunchKernelGGL(k_multi_window_count, dim3(nb), dim3(kBlock), 0, s, pos3, hsml, n, vd_dev,
+                     views_dev, nviews, nb, counts, totals, counts_plain, totals_plain);
   ILC_HIP(hipGetLastError());
 }
 

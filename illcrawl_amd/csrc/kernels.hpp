@@ -82,6 +82,11 @@ void launch_crawl(const double period[3], const double gmin[3], const double gma
 struct window_f {
   float x0, x1, y0, y1;
   int use_window, use_depth;
+  // the batch's shared build: rows whose support cannot reach the screen
+  // rectangle [sx0, sx1] x [sy0, sy1] (d^2 / h^2 >= cull2 by the gather's own
+  // cull arithmetic) only raise their cell's max h
+  int use_screen;
+  float sx0, sx1, sy0, sy1, cull2;
 };
 // One view of a multi-view pass over the snapshot.
 struct view_build_desc {
@@ -98,9 +103,13 @@ void launch_multi_bbox(const particle_soa& ps, const view_build_desc* vd_dev, in
                        hipStream_t s);
 // Kept particles per window tile (window_tiles(n) of them) of every listed view
 // in one read: counts[v * ntiles + t], totals[v] += the same (not cleared).
-void launch_multi_window_count(const float* pos3, uint64_t n, const view_build_desc* vd_dev,
-                               const int32_t* views_dev, int nviews, uint32_t* counts,
-                               uint32_t* totals, hipStream_t s);
+// counts / totals: kept rows (window, depth cut and screen test);
+// counts_plain / totals_plain: the same without the screen test (the rows a
+// single view's window scatter keeps)
+void launch_multi_window_count(const float* pos3, const float* hsml, uint64_t n,
+                               const view_build_desc* vd_dev, const int32_t* views_dev,
+                               int nviews, uint32_t* counts, uint32_t* totals,
+                               uint32_t* counts_plain, uint32_t* totals_plain, hipStream_t s);
 uint32_t window_tiles(uint64_t n);
 // X-ray weight of every particle (the quantity of particle_of, view-free).
 void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* wx,
@@ -109,11 +118,13 @@ void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* 
 // positions offs[v * ntiles + t] (exclusive scan of the multi-window counts):
 // projected float4 (x', y', 0, w with the depth mask), h, the cell key in the
 // view's grid, the snapshot index and the view.  wx: launch_particle_xray.
+// cbase / maxh_bits: every view's first cell in, and the per-cell max h of,
+// the shared cell arrays (rows kept out by the screen test go there only)
 void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  const view_build_desc* vd_dev, const int32_t* views_dev,
                                  int nviews, const uint32_t* offs, float4* p4c, float* hc,
                                  uint32_t* ckey, uint32_t* orig, uint32_t* vid, uint32_t* err,
-                                 hipStream_t s);
+                                 const uint32_t* cbase, uint32_t* maxh_bits, hipStream_t s);
 // keys[j] = descending-h key of h[j] (as launch_h_order), vals[j] = j
 void launch_hq_keys(const float* h, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
 // out[i] = src[idx[i]]

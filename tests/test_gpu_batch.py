@@ -37,6 +37,22 @@ def scene():
     return snap, views
 
 
+# A batched view's grid keeps only the particles whose support reaches the
+# screen (the others only raise their cell's max h), so its tiles stream fewer
+# candidates than the single-view grid's and group the same list entries into
+# FP32 partial sums at other places: the images agree to the FP32 rounding
+# of those sums.  The plain path builds each view's grid alone and stays bit
+# for bit equal to the single-view backend.
+GROUPING_TOL = 1e-6
+
+
+def assert_same_sum(got, ref, what):
+    assert np.array_equal(got == 0, ref == 0), what
+    m = ref != 0
+    rel = float(np.max(np.abs(got[m] - ref[m]) / np.abs(ref[m]))) if m.any() else 0.0
+    assert rel < GROUPING_TOL, (what, rel)
+
+
 def single_view(B, snap, v, mode, tab):
     if v["quantity"] == 1:
         w = B.quantity_xray_bolometric(snap.rho, snap.temp, snap.xe, *tab)
@@ -73,10 +89,14 @@ def test_batch_equals_single_view(B, scene, mode):
     for k, v in enumerate(views):
         ref, _ = single_view(B, snap, v, mode, tab)
         got = r.image_f64(k).cpu().numpy()
-        # same grid, same per-tile accumulation order per point: identical
-        assert np.array_equal(got, ref), f"view {k}"
+        assert_same_sum(got, ref, f"view {k}")
         img = r.image(k).cpu().numpy()
-        assert np.array_equal(img, (ref * np.float64(np.float32(v["scale"]))).astype(np.float32))
+        assert np.array_equal(img, (got * np.float64(np.float32(v["scale"]))).astype(np.float32))
+    # the plain path: each view's grid alone, bit for bit the single view's
+    r.render(plain=True)
+    for k, v in enumerate(views):
+        ref, _ = single_view(B, snap, v, mode, tab)
+        assert np.array_equal(r.image_f64(k).cpu().numpy(), ref), f"plain view {k}"
 
 
 def test_batch_view_vs_oracle(B, scene, orc):
@@ -109,7 +129,7 @@ def test_plain_and_tiles_agree(B, scene):
     base = [r.image_f64(k).cpu().numpy() for k in range(len(views))]
     r.render(plain=True)
     for k in range(len(views)):
-        assert np.array_equal(r.image_f64(k).cpu().numpy(), base[k])
+        assert_same_sum(r.image_f64(k).cpu().numpy(), base[k], f"plain view {k}")
     for tile in ((16, 16), (32, 16), (64, 16)):
         r.set_tile(*tile)
         r.render()
@@ -166,10 +186,10 @@ def test_batch_counts_and_autotune(B, scene, orc):
 @pytest.mark.parametrize("mode", ["reference", "exact"])
 def test_windowed_grids_equal_single_view(B, mode):
     """Narrow views deep inside the snapshot build their grids from the
-    particles around the screen only; their images must stay bit-identical to
-    the single-view backend's full grid.  A view reaching past the particle
-    bounding box keeps every particle (reference mode: a point outside the grid
-    can alias any cell)."""
+    particles around the screen only; their images equal the single-view
+    backend's full grid to the grouping of the FP32 partial sums.  A view
+    reaching past the particle bounding box keeps every particle (reference
+    mode: a point outside the grid can alias any cell)."""
     from illcrawl_amd import synth
     say(f"case: windowed batch grids, mode={mode}")
     snap = synth.make_snapshot(1 << 16, 9, 10000.0, n_halos=2, rs_range=(200.0, 400.0),
@@ -196,18 +216,18 @@ def test_windowed_grids_equal_single_view(B, mode):
         info = r.view_info(k)
         say(f"  view {k}: {info['num_particles']} of {snap.n} particles, "
             f"max {float(ref.max()):.3e}")
-        assert np.array_equal(got, ref), f"view {k}"
+        assert_same_sum(got, ref, f"view {k}")
         assert ref.max() > 0
         n_windowed += info["num_particles"] < snap.n
     assert n_windowed >= 3
     if mode == "reference":
         assert r.view_info(len(views) - 1)["num_particles"] == snap.n
-    # graph replays give the same images
+    # graph replays give the eager images
+    eager = [r.image_f64(k).cpu().numpy() for k in range(len(views))]
     r.render(graph=True)
     r.render(graph=True)
-    for k, v in enumerate(views):
-        ref, _ = single_view(B, snap, v, mode, tab)
-        assert np.array_equal(r.image_f64(k).cpu().numpy(), ref), f"graph view {k}"
+    for k in range(len(views)):
+        assert np.array_equal(r.image_f64(k).cpu().numpy(), eager[k]), f"graph view {k}"
 
 
 @pytest.mark.parametrize("mode", ["reference", "exact"])
