@@ -59,9 +59,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def sampled_parity(snap, view, img_f64, n=1500, seed=7):
-    """Relative L-inf / sum error of the GPU image against the oracle's
-    reference-semantics gather on a random pixel sample (plus the maximum)."""
+KINDS = {"quartic_ref": 0, "quartic": 1, "cubic": 2}
+
+
+def sampled_parity(snap, view, img_f64, n=1500, seed=7, kernel="quartic_ref", mode="reference"):
+    """Relative L-inf / sum error of the GPU image against the oracle on a
+    random pixel sample (plus the maximum): the reference-semantics gather
+    (reference mode), or the FP64 sum over every particle (exact mode)."""
     import numpy as np
     import oracle
     from illcrawl_amd import synth
@@ -79,20 +83,27 @@ def sampled_parity(snap, view, img_f64, n=1500, seed=7):
     rng = np.random.default_rng(seed)
     idx = np.unique(np.concatenate([rng.choice(img.size, min(n, img.size), replace=False),
                                     [int(np.argmax(img))]]))
-    ref = g.gather_ref(pts[idx], acc_mode=1)
-    ref32 = g.gather_ref(pts[idx], acc_mode=0)
+    if mode == "reference":
+        ref = g.gather_ref(pts[idx], acc_mode=1)
+        ref32 = g.gather_ref(pts[idx], acc_mode=0)
+    else:
+        ref = oracle.gather_exact(pp, snap.hsml, pts[idx], kind=KINDS[kernel])
+        ref32 = None
     m = ref > 1e-6 * ref.max()
     linf = float(np.max(np.abs(img[idx][m] - ref[m]) / ref[m])) if m.any() else 0.0
-    l32 = float(np.max(np.abs(ref32[m] - ref[m]) / ref[m])) if m.any() else 0.0
+    l32 = float(np.max(np.abs(ref32[m] - ref[m]) / ref[m])) if m.any() and ref32 is not None \
+        else None
     return dict(pixels=int(idx.size), linf_vs_oracle=linf,
                 oracle_fp32_accumulator_vs_fp64=l32)
 
 
-def cpu_baseline(snap, view, per_point_counts, seconds=15.0):
+def cpu_baseline(snap, view, per_point_counts, seconds=15.0, kernel="quartic_ref"):
     """The oracle (the reference algorithm restated in C, OpenMP over
     pixels) on a bounded random sample of one view's pixels; evals from the
     exact per-pixel pair counts of the same pixels (GPU count kernel, which
-    the tests pin to the oracle's count)."""
+    the tests pin to the oracle's count).  Kernels other than the
+    reference's quartic have no reference algorithm: the oracle's FP64 sum
+    over every particle stands in."""
     import numpy as np
     import oracle
     from illcrawl_amd import synth
@@ -114,22 +125,29 @@ def cpu_baseline(snap, view, per_point_counts, seconds=15.0):
     order = rng.permutation(pts.shape[0])
     counts = per_point_counts.cpu().numpy()
     done, evals, elapsed, batch = 0, 0, 0.0, 16
+    ref_kernel = kernel == "quartic_ref"
+    if not ref_kernel:
+        batch = 1
     while elapsed < seconds and done < order.size:
         idx = order[done:done + batch]
         t0 = time.perf_counter()
-        g.gather_ref(pts[idx], acc_mode=0)
+        if ref_kernel:
+            g.gather_ref(pts[idx], acc_mode=0)
+        else:
+            oracle.gather_exact(pp, snap.hsml, pts[idx], kind=KINDS[kernel])
         elapsed += time.perf_counter() - t0
         evals += int(counts[idx].sum())
         done += idx.size
         batch = min(batch * 2, 4096)
+    what = ("the reference gather restated in C, FP32 accumulator" if ref_kernel else
+            f"the oracle's FP64 {kernel} sum over every particle (no reference algorithm)")
     return dict(value=evals / elapsed, unit="evals/s", cores=threads, kind="port",
                 sample=f"{done} random pixels of one {view['nx']}x{view['ny']} view "
-                       f"({elapsed:.1f} s of the reference gather restated in C, FP32 "
-                       f"accumulator, {threads} OpenMP threads); its grid build "
-                       f"({t_grid:.2f} s, one thread) not included")
+                       f"({elapsed:.1f} s of {what}, {threads} OpenMP threads); its grid "
+                       f"build ({t_grid:.2f} s, one thread) not included")
 
 
-def single_view_counts(snap, view):
+def single_view_counts(snap, view, kernel="quartic_ref"):
     """Per-pixel exact eval counts of one view (single-view backend)."""
     import torch
     from illcrawl_amd import backend as B, synth
@@ -143,7 +161,7 @@ def single_view_counts(snap, view):
     p4[:, 3] = w
     proj = B.project_particles(view, p4)
     del p4
-    b = B.ProjectiveSmoothingGrid(mode="exact")
+    b = B.ProjectiveSmoothingGrid(mode="exact", kernel=kernel)
     b.init_backend(snap.n)
     b.setup_projected_particles(proj, snap.hsml)
     b.setup_evaluation_points(B.camera_pixel_points(view), layout_nx=view["nx"])
@@ -460,8 +478,9 @@ def main():
     ap.add_argument("--mode", default="reference", choices=["reference", "exact"])
     ap.add_argument("--adjoint", action="store_true",
                     help="batched forward + adjoint timing of the workload (exact mode)")
-    ap.add_argument("--kernel", default="quartic", choices=["quartic", "quartic_ref", "cubic"],
-                    help="projected kernel of the adjoint workload (c4)")
+    ap.add_argument("--kernel", default=None, choices=["quartic", "quartic_ref", "cubic"],
+                    help="projected kernel: default quartic_ref (the reference's), quartic "
+                         "for c4; a kernel other than quartic_ref renders in exact mode")
     ap.add_argument("--tile", default=None, help="WxH gather tile; default: autotuned")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -483,7 +502,12 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     if args.workload == "c4":
+        args.kernel = args.kernel or "quartic"
         return adjoint_main(args)
+    kernel = args.kernel or "quartic_ref"
+    if kernel != "quartic_ref" and args.mode == "reference":
+        log(f"[bench] the {kernel} kernel has no reference search: exact mode")
+        args.mode = "exact"
     if args.workload == "v1":
         return volumetric_main(args)
     if args.adjoint:
@@ -492,7 +516,7 @@ def main():
     t0 = time.time()
     snap, views, desc = workload(args.workload)
     log(f"[bench] {args.workload}: {snap.n} cells, {len(views)} views ({time.time() - t0:.1f}s)")
-    r = B.BatchRenderer(mode=args.mode)
+    r = B.BatchRenderer(mode=args.mode, kernel=kernel)
     r.set_particles(snap.pos, snap.mass, snap.hsml, snap.rho, snap.temp, snap.xe)
     tab, x0, dx = synth.gaunt_table_1d()
     r.set_gaunt_table(tab, x0, dx)
@@ -567,8 +591,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (f64 per-pixel accumulation)",
         "data": "synthetic (seeded: uniform + NFW halos, 32-NN density, lognormal masses)",
-        "config": dict(desc, mode=args.mode, kernel="quartic_ref (the reference's projected "
-                       "quartic)", tile=list(tile), autotune=tuned,
+        "config": dict(desc, mode=args.mode,
+                       kernel="quartic_ref (the reference's projected quartic)"
+                       if kernel == "quartic_ref" else kernel, tile=list(tile), autotune=tuned,
                        evals_per_step=total_evals, pixels_per_step=npix,
                        parallelism="single GPU"),
         "pixels_per_s": npix / (ms_per_step * 1e-3),
@@ -597,11 +622,12 @@ def main():
     if not args.no_parity:
         log("[bench] sampled parity vs oracle ...")
         img = r.image_f64(big).cpu().numpy()
-        line["parity"] = dict(view=big, **sampled_parity(snap, views[big], img))
+        line["parity"] = dict(view=big, **sampled_parity(snap, views[big], img, kernel=kernel,
+                                                         mode=args.mode))
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
-        _, per = single_view_counts(snap, views[big])
-        line["cpu_baseline"] = cpu_baseline(snap, views[big], per, args.cpu_seconds)
+        _, per = single_view_counts(snap, views[big], kernel)
+        line["cpu_baseline"] = cpu_baseline(snap, views[big], per, args.cpu_seconds, kernel)
     print(json.dumps(line), flush=True)
 
 

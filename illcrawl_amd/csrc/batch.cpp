@@ -1037,6 +1037,7 @@ ilc_status ilc_batch_autotune(ilc_batch* b, int32_t* tile_w, int32_t* tile_h, do
     device_guard dg(b->device);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     static const tile_shape cand[] = {{16, 1}, {32, 2}, {32, 4}, {64, 4}};
+    ensure_cubic(b);  // the cubic gather reads its table from the first launch
     hipEvent_t e0, e1;
     ILC_HIP(hipEventCreate(&e0));
     ILC_HIP(hipEventCreate(&e1));

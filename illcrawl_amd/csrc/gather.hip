@@ -1056,6 +1056,8 @@ int gather_batch_grid(int op, int kernel, const tile_shape& ts) {
 void launch_gather_batch(const view_desc* views, const uint2* items, const queue_segs& segs,
                          uint32_t* counters, int op, int kernel, const tile_shape& ts,
                          const float* cubic_table, int grid_blocks, hipStream_t s) {
+  if (kernel == ILC_KERNEL_CUBIC && !cubic_table)
+    fail(ILC_EINVAL, "cubic gather without its table");
   const float support = kernel == ILC_KERNEL_CUBIC ? 2.0f : 1.0f;
   // whole rounds of kQueues workgroups, so that every XCD gets the same number
   const int grid = std::max(kQueues, grid_blocks / kQueues * kQueues);
@@ -1072,6 +1074,8 @@ void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,
     plain_launcher<32, 4, OP_COUNT, 0>::run(g, p, nullptr, counts, nullptr, support, s);
     return;
   }
+  if (kernel == ILC_KERNEL_CUBIC && !cubic_table)
+    fail(ILC_EINVAL, "cubic gather without its table");
   dispatch<plain_launcher>(ts, op, kernel, g, p, acc, counts, cubic_table, support, s);
 }
 

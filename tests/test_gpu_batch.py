@@ -53,7 +53,7 @@ def assert_same_sum(got, ref, what):
     assert rel < GROUPING_TOL, (what, rel)
 
 
-def single_view(B, snap, v, mode, tab):
+def single_view(B, snap, v, mode, tab, kernel="quartic_ref"):
     if v["quantity"] == 1:
         w = B.quantity_xray_bolometric(snap.rho, snap.temp, snap.xe, *tab)
     else:
@@ -62,7 +62,7 @@ def single_view(B, snap, v, mode, tab):
     p4[:, :3] = torch.from_numpy(snap.pos).cuda()
     p4[:, 3] = w
     proj = B.project_particles(v, p4)
-    b = B.ProjectiveSmoothingGrid(mode=mode)
+    b = B.ProjectiveSmoothingGrid(mode=mode, kernel=kernel)
     b.init_backend(snap.n)
     b.setup_projected_particles(proj, snap.hsml)
     b.setup_evaluation_points(B.camera_pixel_points(v), layout_nx=v["nx"])
@@ -70,8 +70,8 @@ def single_view(B, snap, v, mode, tab):
     return b.retrieve_results_f64().cpu().numpy().reshape(v["ny"], v["nx"]), proj
 
 
-def make_batch(B, snap, views, mode, tab, tile=None):
-    r = B.BatchRenderer(mode=mode, tile=tile)
+def make_batch(B, snap, views, mode, tab, tile=None, kernel="quartic_ref"):
+    r = B.BatchRenderer(mode=mode, tile=tile, kernel=kernel)
     r.set_particles(snap.pos, snap.mass, snap.hsml, snap.rho, snap.temp, snap.xe)
     r.set_gaunt_table(*tab)
     r.set_views(views)
@@ -335,3 +335,30 @@ def test_batch_backward_equals_sum_of_single_views(B):
         e = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
         say(f"  dL/d{name}: rel-to-max diff {e:.2e}")
         assert e < 1e-5, name
+
+
+def test_cubic_batch_autotunes_before_its_first_render(B, scene, orc):
+    """The cubic spline's table is set up by every entry point that launches
+    the gather, the autotuner included (a regression test: the autotuner once
+    ran before the first render without it)."""
+    from illcrawl_amd import synth
+    snap, views = scene
+    tab = synth.gaunt_table_1d()
+    say("case: cubic batch, autotune first")
+    r = make_batch(B, snap, views[:3], "exact", tab, kernel="cubic")
+    (tw, th), ms = r.autotune()
+    assert ms > 0
+    r.render()
+    for k, v in enumerate(views[:3]):
+        ref, _ = single_view(B, snap, v, "exact", tab, kernel="cubic")
+        assert_same_sum(r.image_f64(k).cpu().numpy(), ref, f"cubic view {k}")
+    v = views[1]
+    cam = orc.Camera(v["position"], v["look_at"], v["roll"], v["width"], v["nx"], v["ny"])
+    w = orc.xray_bolometric(snap.rho, snap.temp, snap.xe, *tab) if v["quantity"] == 1 \
+        else snap.mass
+    pp = cam.project_particles(snap.particles4(w), v["depth"])
+    ref = orc.gather_exact(pp, snap.hsml, cam.pixel_points(), kind=2)
+    got = r.image_f64(1).cpu().numpy().reshape(-1)
+    m = ref > 1e-6 * ref.max()
+    assert np.max(np.abs(got[m] - ref[m]) / ref[m]) < 2e-5
+    assert abs(got.sum() - ref.sum()) / ref.sum() < 2e-6
