@@ -39,6 +39,8 @@ level_view grid_store::levels() const {
   lv.w = lv_w.p;
   lv.cell = lv_cell.p;
   lv.meta = lv_meta.p;
+  lv.pdir = lv_pdir.p;
+  for (int l = 0; l <= kLevels3; ++l) lv.poff[l] = lv_poff[l];
   for (int l = 0; l < kLevels3; ++l) lv.cs[l] = lv_cs[l];
   return lv;
 }
@@ -54,6 +56,18 @@ void level_cell_sizes(const grid_params& g, float max_h, float cs[kLevels3]) {
     cs[l] = std::max(hl, floor_cs);
     if (!(cs[l] > 0.0f) || !std::isfinite(cs[l])) cs[l] = std::max(max_h, 1e-30f);
   }
+}
+
+void level_planes(const grid_params& g, const float cs[kLevels3], uint32_t poff[kLevels3 + 1]) {
+  // a particle's fine z is at most floor(extent / cs) (one more by rounding)
+  const double ext = double(g.nc[2]) * double(g.cs[2]);
+  uint32_t off = 0;
+  for (int l = 0; l < kLevels3; ++l) {
+    poff[l] = off;
+    const double np = std::floor(ext / double(cs[l])) + 2.0;
+    off += static_cast<uint32_t>(std::min(np, double(kFineMax3) + 1.0)) + 1u;
+  }
+  poff[kLevels3] = off;
 }
 
 grid_view grid_store::view() const {
@@ -147,9 +161,11 @@ void enqueue_volumetric_grid(grid_store& gs, grid_scratch& sc, const float4* p4,
   sc.lv_tmp.ensure(3 * std::max<uint64_t>(n, 1));
   sc.temp.ensure(radix_temp_words(n));
   level_cell_sizes(g, max_h, gs.lv_cs);
+  level_planes(g, gs.lv_cs, gs.lv_poff);
+  gs.lv_pdir.ensure(gs.lv_poff[kLevels3]);
   launch_volumetric_index(p4, h, uint32_t(n), g, max_h, kernel_norm_3d(kernel), gs.levels(),
                           gs.lv_key.p, gs.lv_pos.p, gs.lv_w.p, gs.lv_cell.p, gs.lv_meta.p,
-                          gs.maxh_bits.p, err, sc.lv_tmp.p, sc.temp.p, s);
+                          gs.lv_pdir.p, gs.maxh_bits.p, err, sc.lv_tmp.p, sc.temp.p, s);
   gs.gp = g;
   gs.n = n;
   gs.max_h = max_h;

@@ -26,6 +26,8 @@ struct grid_store {
   dbuf<float> lv_w;
   dbuf<int2> lv_cell;
   dbuf<uint32_t> lv_meta;
+  dbuf<uint32_t> lv_pdir;
+  uint32_t lv_poff[kLevels3 + 1] = {};
   float lv_cs[kLevels3] = {};
   grid_params gp{};
   float max_h = 0.f, cell_radius = 0.f;

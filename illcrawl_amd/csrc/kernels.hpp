@@ -253,10 +253,17 @@ struct level_view {
   const float* w;                 // w K / h^3
   const int2* cell;               // reference cell (cx | cy << 16, cz)
   const uint32_t* meta;
+  // plane directory: pdir[poff[L] + z] = first index of level L whose fine z
+  // is >= z, for z = 0 .. poff[L + 1] - poff[L] - 1 (the last: the level's end)
+  const uint32_t* pdir;
+  uint32_t poff[kLevels3 + 1];
   float cs[kLevels3];
 };
 // Fine cell sizes of the levels for a grid of that extent and max h (host).
 void level_cell_sizes(const grid_params& g, float max_h, float cs[kLevels3]);
+// Offsets of the levels' plane directories (host): every fine z a particle
+// can have, plus one entry for the level's end.
+void level_planes(const grid_params& g, const float cs[kLevels3], uint32_t poff[kLevels3 + 1]);
 // Builds the level index of particles p4 (x, y, z, w) with smoothing lengths
 // h on grid g, and the grid's per-cell max h (maxh_bits, ncells words); err
 // counts particles outside the grid.  norm: the kernel's 3-D normalisation
@@ -264,8 +271,8 @@ void level_cell_sizes(const grid_params& g, float max_h, float cs[kLevels3]);
 void launch_volumetric_index(const float4* p4, const float* h, uint32_t n, const grid_params& g,
                              float max_h, float norm, const level_view& lv,
                              unsigned long long* key, float4* pos, float* w, int2* cell,
-                             uint32_t* meta, uint32_t* maxh_bits, uint32_t* err, uint32_t* tmp,
-                             uint32_t* radix_temp, hipStream_t s);
+                             uint32_t* meta, uint32_t* pdir, uint32_t* maxh_bits, uint32_t* err,
+                             uint32_t* tmp, uint32_t* radix_temp, hipStream_t s);
 // op OP_REFERENCE / OP_EXACT add to acc, OP_COUNT adds evaluation counts to cnt
 void launch_smooth3d(const grid_view& g, const level_view& lv, const point_view& p, int op,
                      double* acc, hipStream_t s, unsigned long long* cnt = nullptr);

@@ -106,6 +106,7 @@ struct smem3 {
   int lv_mode[kLevels3];
   uint32_t lv_row0[kLevels3 + 1];
   uint32_t lv_s[kLevels3 + 1];
+  uint32_t lv_p[kLevels3 + 1];  // plane directory offsets
 };
 
 // level of a particle: the smallest L with h <= max_h 2^(L - 23) (by the
@@ -360,7 +361,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
     sm.lv_mode[l] = mode;
     sm.lv_row0[l] = rows;
     sm.lv_s[l] = lv.meta[2 * kLevels3 + l];
-    if (l == kLevels3 - 1) sm.lv_s[kLevels3] = lv.meta[3 * kLevels3];
+    sm.lv_p[l] = lv.poff[l];
+    if (l == kLevels3 - 1) {
+      sm.lv_s[kLevels3] = lv.meta[3 * kLevels3];
+      sm.lv_p[kLevels3] = lv.poff[kLevels3];
+    }
   }
   __syncthreads();
   if (t == 0) {
@@ -395,8 +400,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
       }
       const unsigned long long klo = fine_key(l, uint32_t(f0[0]), uint32_t(y0), uint32_t(z0));
       const unsigned long long khi = fine_key(l, uint32_t(f1[0]), uint32_t(y1), uint32_t(z1));
-      const uint32_t e_ = sm.lv_s[l + 1];
-      const uint32_t a_ = lower_bound_u64(lv.key, sm.lv_s[l], e_, klo);
+      // the keys of planes z0 .. z1 from the plane directory (a plane
+      // beyond the level's last holds no particle)
+      const uint32_t planes = sm.lv_p[l + 1] - sm.lv_p[l] - 1;
+      const uint32_t za = min(uint32_t(z0), planes - 1), zb = min(uint32_t(z1), planes - 1);
+      const uint32_t s_ = lv.pdir[sm.lv_p[l] + za];
+      const uint32_t e_ = lv.pdir[sm.lv_p[l] + zb + 1];
+      const uint32_t a_ = lower_bound_u64(lv.key, s_, e_, klo);
       rb = a_;
       rn = gallop_lower_bound_u64(lv.key, a_, e_, khi + 1ull) - a_;
     }
@@ -571,6 +581,29 @@ __global__ __launch_bounds__(kBlock) void k_vol_keys(const float4* __restrict__ 
   }
 }
 
+// pdir[poff[L] + z]: first index of level L whose fine z is >= z; the last
+// entry of a level is its end
+__global__ __launch_bounds__(kBlock) void k_lv_planes(level_view lv, uint32_t* __restrict__ pdir) {
+  __shared__ uint32_t sp[kLevels3 + 1], ss[kLevels3 + 1];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int l = 0; l <= kLevels3; ++l) {
+      sp[l] = lv.poff[l];
+      ss[l] = lv.meta[2 * kLevels3 + l];
+    }
+  }
+  __syncthreads();
+  const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= sp[kLevels3]) return;
+  int l = 0;
+  while (sp[l + 1] <= e) ++l;  // e < sp[kLevels3] ends the walk
+  const uint32_t z = e - sp[l];
+  const uint32_t last = sp[l + 1] - sp[l] - 1;
+  pdir[e] = z == last ? ss[l + 1]
+                      : lower_bound_u64(lv.key, ss[l], ss[l + 1],
+                                        fine_key(l, 0u, 0u, min(z, kFineMax3)));
+}
+
 __global__ void k_lv_starts(uint32_t* __restrict__ meta) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t s = 0;
@@ -621,8 +654,8 @@ __global__ __launch_bounds__(kBlock) void k_vol_soa(const float4* __restrict__ p
 void launch_volumetric_index(const float4* p4, const float* h, uint32_t n, const grid_params& g,
                              float max_h, float norm, const level_view& lv,
                              unsigned long long* key, float4* pos, float* w, int2* cell,
-                             uint32_t* meta, uint32_t* maxh_bits, uint32_t* err, uint32_t* tmp,
-                             uint32_t* radix_temp, hipStream_t s) {
+                             uint32_t* meta, uint32_t* pdir, uint32_t* maxh_bits, uint32_t* err,
+                             uint32_t* tmp, uint32_t* radix_temp, hipStream_t s) {
   launch_fill_u32(meta, 0u, kLvMetaWords, s);
   launch_fill_u32(maxh_bits, 0u, g.ncells(), s);
   if (n) {
@@ -646,6 +679,13 @@ void launch_volumetric_index(const float4* p4, const float* h, uint32_t n, const
   }
   hipLaunchKernelGGL(k_lv_starts, dim3(1), dim3(64), 0, s, meta);
   ILC_HIP(hipGetLastError());
+  // the directory searches the sorted keys (lv.key is key, lv.meta is meta)
+  const uint32_t nd = lv.poff[kLevels3];
+  ILC_REQUIRE(lv.key == key && lv.meta == meta && lv.pdir == pdir, "level index buffers");
+  if (nd) {
+    hipLaunchKernelGGL(k_lv_planes, dim3((nd + kBlock - 1) / kBlock), dim3(kBlock), 0, s, lv, pdir);
+    ILC_HIP(hipGetLastError());
+  }
 }
 
 void launch_smooth3d(const grid_view& g, const level_view& lv, const point_view& p, int op,
