@@ -53,7 +53,9 @@ def agree(got, ref, what):
     same = (got.view(np.uint32) == ref.view(np.uint32)) if got.dtype == np.float32 \
         else got == ref
     frac = float(same.mean())
-    rel = np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-300)
+    # equal infinities (same bits) give inf - inf; only unequal values are read
+    with np.errstate(invalid="ignore"):
+        rel = np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-300)
     worst = float(rel[~same].max()) if (~same).any() else 0.0
     say(f"  {what}: bit-identical {frac:.6f}, worst other rel diff {worst:.2e}")
     assert frac >= 0.999
