@@ -178,11 +178,12 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
   return off + incl - v;
 }
 
-// 6 waves per SIMD: the row searches and candidate loads are latency-bound,
-// and more waves hide them (5 -> 6 -> 7 waves: 3.31 -> 3.08 -> 3.10 ms on 8
-// slices of 2048^2 through 16M particles; 7 spills twice as many registers)
+// 7 waves per SIMD (the LDS allows 7 workgroups per CU): the row searches
+// and candidate loads are latency-bound, and more waves hide them (6 -> 7
+// waves: 2.85 -> 2.64 ms on 8 slices of 2048^2 through 16M particles, with 4
+// registers spilled at 7)
 template <bool REF, bool COUNT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_smooth3d(grid_view g, level_view lv, point_view p,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_smooth3d(grid_view g, level_view lv, point_view p,
                                                       uint32_t tiles_x, double* __restrict__ acc,
                                                       unsigned long long* __restrict__ cnt) {
   static_assert(!(REF && COUNT), "evaluations are counted with the exact search");
@@ -257,6 +258,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
       ihi[a] = min(ihi[a], sm.redi[w][9 + a]);
     }
     any |= sm.redi[w][12];
+  }
+  // the boxes are the workgroup's: scalar registers, not vector ones
+  for (int a = 0; a < 3; ++a) {
+    bmin[a] = __int_as_float(uniform(__float_as_int(bmin[a])));
+    bmax[a] = __int_as_float(uniform(__float_as_int(bmax[a])));
+    ulo[a] = uniform(ulo[a]);
+    uhi[a] = uniform(uhi[a]);
+    ilo[a] = uniform(ilo[a]);
+    ihi[a] = uniform(ihi[a]);
   }
   __syncthreads();
   if (!uniform(any)) return;
