@@ -52,7 +52,7 @@ FLOP_PER_EVAL_3D = 13
 # keyed by (workload, mode, tile)
 MEASURED_TRAFFIC = {("c3", "reference", (16, 16)): dict(read=19.30e9, write=1.781e9),
                     # k_smooth3d, profiles/k_smooth3d_slices_16M.md
-                    ("v1", "reference", (16, 16)): dict(read=1.559e9, write=1.597e9)}
+                    ("v1", "reference", (16, 16)): dict(read=1.244e9, write=0.461e9)}
 
 
 def log(msg):
