@@ -492,7 +492,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              f"(views would shard across GPUs with no exchange; see DESIGN.md)",
+              f"(views would shard across GPUs with no exchange between them)",
               file=sys.stderr)
         sys.exit(2)
 
