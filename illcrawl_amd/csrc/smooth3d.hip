@@ -8,9 +8,9 @@
 // Candidates come from the level index of the grid, not from the reference
 // cells: particles binned by h into levels (level L holds h <= max_h 2^(L-23)),
 // each level sorted by a fine cell key (z, y, x) whose cell size is about the
-// level's h.  The reference's cells are coarse (about 1000 particles each on
-// average, far more in clustered regions), and a tile inside a cell would
-// stream all of them; a level row holds only the particles near the tile.
+// level's h.  The reference's cells are coarse (64 particles each on average,
+// far more in clustered regions), and a tile inside a cell would stream all
+// of them; a level row holds only the particles near the tile.
 // The result is the reference's, since the search box test is made on each
 // particle's reference cell.
 //
@@ -183,9 +183,9 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
 // waves: 2.85 -> 2.64 ms on 8 slices of 2048^2 through 16M particles, with 4
 // registers spilled at 7)
 template <bool REF, bool COUNT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_smooth3d(grid_view g, level_view lv, point_view p,
-                                                      uint32_t tiles_x, double* __restrict__ acc,
-                                                      unsigned long long* __restrict__ cnt) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_smooth3d(
+    grid_view g, level_view lv, point_view p, uint32_t tiles_x, double* __restrict__ acc,
+    unsigned long long* __restrict__ cnt) {
   static_assert(!(REF && COUNT), "evaluations are counted with the exact search");
   __shared__ smem3 sm;
   const int t = threadIdx.x;
