@@ -103,6 +103,9 @@ grid_params plan_grid(const bbox_result& bb, uint64_t n, uint64_t target, int mo
   if (mode == ILC_MODE_REFERENCE && std::floor(zlo) != std::floor(zhi))
     fail(ILC_EINVAL, "reference mode needs projected particles (one z layer); z spans [%g, %g]",
          double(bb.mn[2]), double(bb.mx[2]));
+  // the gather's masked list packs a cell's x and y into 16 bits each
+  if (mode == ILC_MODE_REFERENCE && (g.nc[0] > 65536 || g.nc[1] > 65536))
+    fail(ILC_ERANGE, "grid of %d x %d cells in x, y (at most 65536 each)", g.nc[0], g.nc[1]);
   return g;
 }
 

@@ -362,3 +362,26 @@ def test_quantity_scaled(B):
     assert np.array_equal(got, (1e10 * x.astype(np.float64)).astype(np.float32))
     c = B.quantity_scaled(None, 0.25, n=17).cpu().numpy()
     assert np.all(c == np.float32(0.25))
+
+
+def test_reference_grid_beyond_16_bit_cell_indices_is_refused(B):
+    """The masked list packs a cell's x and y into 16 bits each: a reference
+    grid with more than 65536 cells along an axis is refused, not rendered
+    wrongly.  A thin strip of particles gives about 290,000 cells along y
+    (exact mode, which needs no packing, still renders it)."""
+    from illcrawl_amd import _native as N
+    n = 1 << 20
+    rng = np.random.default_rng(3)
+    p4 = np.zeros((n, 4), np.float32)
+    p4[:, 1] = rng.uniform(0.0, 1.0e6, n)
+    p4[:, 3] = 1.0
+    h = np.ones(n, np.float32)
+    b = B.ProjectiveSmoothingGrid(mode="reference")
+    b.init_backend(n)
+    with pytest.raises(N.IllcrawlError) as e:
+        b.setup_projected_particles(p4, h)
+    assert "65536" in str(e.value)
+    x = B.ProjectiveSmoothingGrid(mode="exact", kernel="quartic")
+    x.init_backend(n)
+    x.setup_projected_particles(p4, h)
+    assert x.grid_info()["num_cells"][1] > 65536
