@@ -161,7 +161,7 @@ void enqueue_volumetric_grid(grid_store& gs, grid_scratch& sc, const float4* p4,
   gs.lv_w.ensure(n);
   gs.lv_cell.ensure(n);
   gs.lv_meta.ensure(kLvMetaWords);
-  sc.lv_tmp.ensure(3 * std::max<uint64_t>(n, 1));
+  sc.lv_tmp.ensure(11 * std::max<uint64_t>(n, 1));
   sc.temp.ensure(radix_temp_words(n));
   level_cell_sizes(g, max_h, gs.lv_cs);
   level_planes(g, gs.lv_cs, gs.lv_poff);

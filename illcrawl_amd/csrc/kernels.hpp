@@ -267,7 +267,8 @@ void level_planes(const grid_params& g, const float cs[kLevels3], uint32_t poff[
 // Builds the level index of particles p4 (x, y, z, w) with smoothing lengths
 // h on grid g, and the grid's per-cell max h (maxh_bits, ncells words); err
 // counts particles outside the grid.  norm: the kernel's 3-D normalisation
-// (coef = w norm / h^3).  tmp: 3 n words; radix_temp: radix_temp_words(n).
+// (coef = w norm / h^3).  tmp: 11 n words, 16-byte aligned; radix_temp:
+// radix_temp_words(n).
 void launch_volumetric_index(const float4* p4, const float* h, uint32_t n, const grid_params& g,
                              float max_h, float norm, const level_view& lv,
                              unsigned long long* key, float4* pos, float* w, int2* cell,

@@ -536,10 +536,13 @@ __device__ inline bool cell_index_3d(const float4& q, const grid_params& g, int3
 }
 
 // per particle: the reference cell's max h (atomic max), the level and its
-// fine key (lo, hi words), the level counts and max h
+// fine key (lo, hi words), the level counts and max h, and a 32-byte record
+// (x, y, z, h | w, cell xy, cell z, 0) in input order, so that the sorted
+// gather reads one line per particle instead of one from p4 and one from h
 __global__ __launch_bounds__(kBlock) void k_vol_keys(const float4* __restrict__ p4,
                                                       const float* __restrict__ hsml, uint32_t n,
                                                       grid_params g, float max_h, level_view lv,
+                                                      float4* __restrict__ rec,
                                                       uint32_t* __restrict__ lo,
                                                       uint32_t* __restrict__ hi,
                                                       uint32_t* __restrict__ idx,
@@ -548,6 +551,7 @@ __global__ __launch_bounds__(kBlock) void k_vol_keys(const float4* __restrict__ 
                                                       uint32_t* __restrict__ err) {
   __shared__ uint32_t cnt[kLevels3], mh[kLevels3];
   __shared__ float scs[kLevels3];
+  __shared__ float4 srec[2 * kBlock];
   const int t = threadIdx.x;
   if (t == 0) {
 #pragma unroll
@@ -559,32 +563,46 @@ __global__ __launch_bounds__(kBlock) void k_vol_keys(const float4* __restrict__ 
   }
   __syncthreads();
   const uint32_t nx = uint32_t(g.nc[0]), nxy = nx * uint32_t(g.nc[1]);
-  for (uint32_t i = blockIdx.x * kBlock + uint32_t(t); i < n; i += gridDim.x * kBlock) {
-    const float4 q = p4[i];
-    const float h = hsml[i];
-    int32_t c[3];
-    if (cell_index_3d(q, g, c)) {
-      // clustered cells hold millions of particles: read the running max
-      // first and go to the atomic unit only with a larger h (a stale read
-      // is never above the true max, so nothing is skipped wrongly)
-      uint32_t* m = &maxh_bits[uint32_t(c[0]) + uint32_t(c[1]) * nx + uint32_t(c[2]) * nxy];
-      const uint32_t hb = __float_as_uint(h);
-      if (hb > __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicMax(m, hb);
-    } else {
-      atomicAdd(err, 1u);
+  // the whole block runs every round (the records go out through LDS)
+  for (uint64_t base = uint64_t(blockIdx.x) * kBlock; base < n;
+       base += uint64_t(gridDim.x) * kBlock) {
+    const uint32_t i = uint32_t(base) + uint32_t(t);
+    if (i < n) {
+      const float4 q = p4[i];
+      const float h = hsml[i];
+      int32_t c[3];
+      if (cell_index_3d(q, g, c)) {
+        // clustered cells hold millions of particles: read the running max
+        // first and go to the atomic unit only with a larger h (a stale read
+        // is never above the true max, so nothing is skipped wrongly)
+        uint32_t* m = &maxh_bits[uint32_t(c[0]) + uint32_t(c[1]) * nx + uint32_t(c[2]) * nxy];
+        const uint32_t hb = __float_as_uint(h);
+        if (hb > __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+          atomicMax(m, hb);
+      } else {
+        atomicAdd(err, 1u);
+        c[0] = c[1] = c[2] = 0;
+      }
+      srec[2 * t] = make_float4(q.x, q.y, q.z, h);
+      srec[2 * t + 1] = make_float4(q.w, __uint_as_float(uint32_t(c[0]) | (uint32_t(c[1]) << 16)),
+                                    __uint_as_float(uint32_t(c[2])), 0.0f);
+      const int l = level_of(h, max_h);
+      const float cs = scs[l];
+      const unsigned long long k = fine_key(l, fine_of(q.x, g.minc[0], cs),
+                                            fine_of(q.y, g.minc[1], cs),
+                                            fine_of(q.z, g.minc[2], cs));
+      lo[i] = uint32_t(k);
+      hi[i] = uint32_t(k >> 32);
+      idx[i] = i;
+      atomicAdd(&cnt[l], 1u);
+      if (__float_as_uint(h) > mh[l]) atomicMax(&mh[l], __float_as_uint(h));
     }
-    const int l = level_of(h, max_h);
-    const float cs = scs[l];
-    const unsigned long long k = fine_key(l, fine_of(q.x, g.minc[0], cs),
-                                          fine_of(q.y, g.minc[1], cs), fine_of(q.z, g.minc[2], cs));
-    lo[i] = uint32_t(k);
-    hi[i] = uint32_t(k >> 32);
-    idx[i] = i;
-    atomicAdd(&cnt[l], 1u);
-    if (__float_as_uint(h) > mh[l]) atomicMax(&mh[l], __float_as_uint(h));
+    __syncthreads();
+    // the round's records are contiguous: whole 16-byte words per lane
+    const uint32_t m2 = 2u * uint32_t(min(uint64_t(kBlock), uint64_t(n) - base));
+    for (uint32_t k = uint32_t(t); k < m2; k += kBlock) rec[2 * base + k] = srec[k];
+    __syncthreads();
   }
-  __syncthreads();
   if (t < kLevels3 && cnt[t]) {
     atomicAdd(&meta[t], cnt[t]);
     atomicMax(&meta[kLevels3 + t], mh[t]);
@@ -626,8 +644,7 @@ __global__ void k_lv_starts(uint32_t* __restrict__ meta) {
 
 // the level-ordered arrays: key, (x, y, z, 1/h^2), coef = w norm / h^3 and
 // the reference cell
-__global__ __launch_bounds__(kBlock) void k_vol_soa(const float4* __restrict__ p4,
-                                                     const float* __restrict__ hsml, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k_vol_soa(const float4* __restrict__ rec, uint32_t n,
                                                      grid_params g, float max_h, float norm,
                                                      level_view lv,
                                                      const uint32_t* __restrict__ order,
@@ -645,8 +662,10 @@ __global__ __launch_bounds__(kBlock) void k_vol_soa(const float4* __restrict__ p
   const uint32_t j = blockIdx.x * kBlock + uint32_t(t);
   if (j >= n) return;
   const uint32_t i = order[j];
-  const float4 q = p4[i];
-  const float h = hsml[i];
+  const float4 r0 = rec[2 * uint64_t(i)];
+  const float4 r1 = rec[2 * uint64_t(i) + 1];
+  const float4 q = make_float4(r0.x, r0.y, r0.z, r1.x);
+  const float h = r0.w;
   const int l = level_of(h, max_h);
   const float cs = scs[l];
   key[j] = fine_key(l, fine_of(q.x, g.minc[0], cs), fine_of(q.y, g.minc[1], cs),
@@ -654,9 +673,7 @@ __global__ __launch_bounds__(kBlock) void k_vol_soa(const float4* __restrict__ p
   const float ih2 = 1.0f / (h * h);
   pos[j] = make_float4(q.x, q.y, q.z, ih2);
   w[j] = q.w * (norm * (ih2 / h));
-  int32_t c[3];
-  if (!cell_index_3d(q, g, c)) c[0] = c[1] = c[2] = 0;  // reported by k_vol_keys
-  cell[j] = make_int2(int(uint32_t(c[0]) | (uint32_t(c[1]) << 16)), c[2]);
+  cell[j] = make_int2(int(__float_as_uint(r1.y)), int(__float_as_uint(r1.z)));
 }
 
 }  // namespace
@@ -669,22 +686,23 @@ void launch_volumetric_index(const float4* p4, const float* h, uint32_t n, const
   launch_fill_u32(meta, 0u, kLvMetaWords, s);
   launch_fill_u32(maxh_bits, 0u, g.ncells(), s);
   if (n) {
-    uint32_t* lo = tmp;
-    uint32_t* hi = tmp + n;
-    uint32_t* idx = tmp + 2 * uint64_t(n);
+    float4* rec = reinterpret_cast<float4*>(tmp);
+    uint32_t* lo = tmp + 8 * uint64_t(n);
+    uint32_t* hi = lo + n;
+    uint32_t* idx = hi + n;
     // a capped grid: each workgroup adds its level counts to the same few
     // global words once
     const uint32_t nb = std::min<uint32_t>((n + kBlock - 1) / kBlock, 2048u);
-    hipLaunchKernelGGL(k_vol_keys, dim3(nb), dim3(kBlock), 0, s, p4, h, n, g, max_h, lv, lo, hi,
-                       idx, meta, maxh_bits, err);
+    hipLaunchKernelGGL(k_vol_keys, dim3(nb), dim3(kBlock), 0, s, p4, h, n, g, max_h, lv, rec, lo,
+                       hi, idx, meta, maxh_bits, err);
     ILC_HIP(hipGetLastError());
     // (hi, lo) order by two stable sorts: low word, then high word
     launch_radix_sort(lo, idx, n, 32, radix_temp, s);
     launch_gather_u32(hi, idx, n, lo, s);
     launch_radix_sort(lo, idx, n, 30, radix_temp, s);
     const dim3 grid{(n + kBlock - 1) / kBlock}, block{kBlock};
-    hipLaunchKernelGGL(k_vol_soa, grid, block, 0, s, p4, h, n, g, max_h, norm, lv, idx, key, pos,
-                       w, cell);
+    hipLaunchKernelGGL(k_vol_soa, grid, block, 0, s, rec, n, g, max_h, norm, lv, idx, key, pos, w,
+                       cell);
     ILC_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_lv_starts, dim3(1), dim3(64), 0, s, meta);
