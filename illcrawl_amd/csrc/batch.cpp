@@ -150,9 +150,8 @@ struct ilc_batch {
   // SoA pass; a view's grid is a slice
   bool shared = false;
   uint64_t sh_ntot = 0, sh_ncells = 0;
-  int sh_cellbits = 0;
-  dbuf<uint32_t> sh_offs, sh_ckey, sh_orig, sh_vid, sh_key, sh_perm, sh_sk, sh_counts, sh_start,
-      sh_maxh, sh_temp, sh_cbase;
+  dbuf<uint32_t> sh_offs, sh_gkey, sh_orig, sh_key, sh_perm, sh_sk, sh_counts, sh_start, sh_maxh,
+      sh_temp, sh_cbase;
   pinned<uint32_t> sh_cbase_host;
   dbuf<float> sh_wx, sh_hc, sh_hs;
   dbuf<float4> sh_p4c, sh_part;
@@ -420,7 +419,6 @@ void plan(ilc_batch* b, hipStream_t s) {
     ILC_HIP(hipStreamSynchronize(s));
   }
   uint64_t ntot = 0, ncells = 0;
-  int cellbits = 1;
   b->sh_cbase_host.ensure(nv);
   for (size_t v = 0; v < nv; ++v) {
     view_state& vs = *b->views[v];
@@ -435,13 +433,11 @@ void plan(ilc_batch* b, hipStream_t s) {
     b->sh_cbase_host.p[v] = static_cast<uint32_t>(ncells);
     ntot += b->wtotal_host.p[v];
     ncells += vs.gp.ncells();
-    cellbits = std::max(cellbits, key_bits_for(vs.gp.ncells() - 1));
   }
   // shared build when every index fits in 31 bits
   b->shared = ntot > 0 && ntot < (uint64_t(1) << 31) && ncells < (uint64_t(1) << 31);
   b->sh_ntot = ntot;
   b->sh_ncells = ncells;
-  b->sh_cellbits = cellbits;
   sig.push_back(b->shared ? ntot : 0);
   sig.push_back(uint64_t(b->ts.tw) << 32 | uint64_t(b->ts.ppt));
   // work items: largest views first, tiles in image order
@@ -563,7 +559,7 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   b->sh_offs.ensure(nv * size_t(nt) + 1);
   b->sh_temp.ensure(std::max({radix_temp_words(ntot), scan_temp_words(nv * uint64_t(nt) + 1),
                               scan_temp_words(nc + 1)}));
-  for (auto* d : {&b->sh_ckey, &b->sh_orig, &b->sh_vid, &b->sh_key, &b->sh_perm, &b->sh_sk})
+  for (auto* d : {&b->sh_gkey, &b->sh_orig, &b->sh_key, &b->sh_perm, &b->sh_sk})
     d->ensure(ntot);
   b->sh_hc.ensure(ntot);
   b->sh_hs.ensure(ntot);
@@ -588,22 +584,19 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   launch_fill_u32(b->sh_maxh.p, 0u, nc, s);
   launch_multi_window_scatter(b->ps, any_xray ? b->sh_wx.p : nullptr, b->bdesc_dev.p,
                               b->fviews_dev.p, static_cast<int>(nv), b->sh_offs.p, b->sh_p4c.p,
-                              b->sh_hc.p, b->sh_ckey.p, b->sh_orig.p, b->sh_vid.p,
-                              b->counter.p + 1, b->sh_cbase.p, b->sh_maxh.p, s);
-  // stable sorts by descending h, cell, view: a view's rows end up in one
-  // range, by cell, biggest h first, ties in input order
+                              b->sh_hc.p, b->sh_gkey.p, b->sh_orig.p, b->counter.p + 1,
+                              b->sh_cbase.p, b->sh_maxh.p, s);
+  // stable sorts by descending h, then by global cell key (view, then cell):
+  // a view's rows end up in one range, by cell, biggest h first, ties in
+  // input order
   launch_hq_keys(b->sh_hc.p, n32, b->sh_sk.p, b->sh_perm.p, s);
   launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, 16, b->sh_temp.p, s);
-  launch_gather_u32(b->sh_ckey.p, b->sh_perm.p, n32, b->sh_sk.p, s);
-  launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, b->sh_cellbits, b->sh_temp.p, s);
-  if (nv > 1) {
-    launch_gather_u32(b->sh_vid.p, b->sh_perm.p, n32, b->sh_sk.p, s);
-    launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, key_bits_for(nv - 1), b->sh_temp.p, s);
-  }
-  // view-local cell keys for the gather, global ones (view's first cell +
-  // local) for the cell ranges
-  launch_final_keys(b->sh_perm.p, b->sh_ckey.p, b->sh_vid.p, b->sh_cbase.p, n32, b->sh_key.p,
-                    b->sh_sk.p, s);
+  launch_gather_u32(b->sh_gkey.p, b->sh_perm.p, n32, b->sh_sk.p, s);
+  launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, key_bits_for(b->sh_ncells - 1), b->sh_temp.p,
+                    s);
+  // view-local cell keys for the gather; the global ones (sh_sk) give the
+  // cell ranges
+  launch_local_keys(b->sh_sk.p, b->sh_cbase.p, static_cast<int>(nv), n32, b->sh_key.p, s);
   launch_fill_u32(b->sh_counts.p, 0u, nc + 1, s);
   launch_build_soa(b->sh_p4c.p, b->sh_hc.p, b->sh_perm.p, b->sh_sk.p, n32, kernel_norm(b->kernel),
                    b->sh_part.p, b->sh_hs.p, b->sh_counts.p, b->sh_maxh.p, s);

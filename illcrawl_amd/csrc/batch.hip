@@ -477,9 +477,8 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     particle_soa ps, const float* __restrict__ wx, const view_build_desc* __restrict__ vd,
     const int32_t* __restrict__ views, int nviews, uint32_t ntiles,
     const uint32_t* __restrict__ offs, float4* __restrict__ p4c, float* __restrict__ hc,
-    uint32_t* __restrict__ ckey, uint32_t* __restrict__ orig, uint32_t* __restrict__ vid,
-    uint32_t* __restrict__ err, const uint32_t* __restrict__ cbase,
-    uint32_t* __restrict__ maxh_bits) {
+    uint32_t* __restrict__ gkey, uint32_t* __restrict__ orig, uint32_t* __restrict__ err,
+    const uint32_t* __restrict__ cbase, uint32_t* __restrict__ maxh_bits) {
 #pragma clang fp contract(off)
   __shared__ __align__(16) uint32_t wc[2][kWinItems][4];
   // the tile's positions, read once (SoA)
@@ -551,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     const int v = views[k];
     const view_proj vp = vd[v].vp;
     const window_f win = vd[v].win;
-    const uint32_t cb = win.use_screen ? cbase[v] : 0u;
+    const uint32_t cb = cbase[v];
     const int slot = k & 1;
     uint32_t bits = 0;  // this thread's kept rows
 #pragma unroll
@@ -592,9 +591,8 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
         bool in_grid;
         const uint32_t key = cell_key(g, ax, ay, &in_grid);
         if (!in_grid) atomicAdd(err, 1u);
-        ckey[j] = key;
+        gkey[j] = cb + key;
         orig[j] = uint32_t(i);
-        vid[j] = uint32_t(v);
       }
       out += tot;
     }
@@ -617,18 +615,23 @@ __global__ __launch_bounds__(kBlock) void k_gather_u32(const uint32_t* __restric
   if (i < n) out[i] = src[idx[i]];
 }
 
-__global__ __launch_bounds__(kBlock) void k_final_keys(const uint32_t* __restrict__ order,
-                                                       const uint32_t* __restrict__ ckey,
-                                                       const uint32_t* __restrict__ vid,
-                                                       const uint32_t* __restrict__ cbase,
-                                                       uint32_t n, uint32_t* __restrict__ lkey,
-                                                       uint32_t* __restrict__ gkey) {
+// a row's view is the last v with cbase[v] <= its global key (every view has
+// at least one cell, so the bases rise strictly)
+__global__ __launch_bounds__(kBlock) void k_local_keys(const uint32_t* __restrict__ gkey,
+                                                       const uint32_t* __restrict__ cbase, int nv,
+                                                       uint32_t n, uint32_t* __restrict__ lkey) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const uint32_t j = order[i];
-  const uint32_t k = ckey[j];
-  lkey[i] = k;
-  gkey[i] = cbase[vid[j]] + k;
+  const uint32_t k = gkey[i];
+  int a = 0, b = nv;  // cbase[a] <= k < cbase[b] (cbase[nv] taken as infinite)
+  while (b - a > 1) {
+    const int m = (a + b) >> 1;
+    if (cbase[m] <= k)
+      a = m;
+    else
+      b = m;
+  }
+  lkey[i] = k - cbase[a];
 }
 
 // Kept particles in input order: offsets = exclusive scan of the counts;
@@ -755,12 +758,13 @@ void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* 
 void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  const view_build_desc* vd_dev, const int32_t* views_dev,
                                  int nviews, const uint32_t* offs, float4* p4c, float* hc,
-                                 uint32_t* ckey, uint32_t* orig, uint32_t* vid, uint32_t* err,
+                                 uint32_t* gkey, uint32_t* orig, uint32_t* err,
                                  const uint32_t* cbase, uint32_t* maxh_bits, hipStream_t s) {
   const uint32_t nb = window_tiles(ps.n);
   if (!nb || nviews <= 0) return;
+  ILC_REQUIRE(cbase != nullptr, "the scatter needs the views' first cells");
   hipLaunchKernelGGL(k_multi_window_scatter, dim3(nb), dim3(kBlock), 0, s, ps, wx, vd_dev,
-                     views_dev, nviews, nb, offs, p4c, hc, ckey, orig, vid, err, cbase, maxh_bits);
+                     views_dev, nviews, nb, offs, p4c, hc, gkey, orig, err, cbase, maxh_bits);
   ILC_HIP(hipGetLastError());
 }
 
@@ -779,12 +783,12 @@ void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uin
   ILC_HIP(hipGetLastError());
 }
 
-void launch_final_keys(const uint32_t* order, const uint32_t* ckey, const uint32_t* vid,
-                       const uint32_t* cbase, uint32_t n, uint32_t* lkey, uint32_t* gkey,
-                       hipStream_t s) {
+void launch_local_keys(const uint32_t* gkey, const uint32_t* cbase, int nv, uint32_t n,
+                       uint32_t* lkey, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_final_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, order,
-                     ckey, vid, cbase, n, lkey, gkey);
+  ILC_REQUIRE(nv >= 1, "no views");
+  hipLaunchKernelGGL(k_local_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, gkey,
+                     cbase, nv, n, lkey);
   ILC_HIP(hipGetLastError());
 }
 

@@ -116,24 +116,25 @@ void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* 
                           hipStream_t s);
 // The kept particles of every listed view, in input order, at the global
 // positions offs[v * ntiles + t] (exclusive scan of the multi-window counts):
-// projected float4 (x', y', 0, w with the depth mask), h, the cell key in the
-// view's grid, the snapshot index and the view.  wx: launch_particle_xray.
-// cbase / maxh_bits: every view's first cell in, and the per-cell max h of,
-// the shared cell arrays (rows kept out by the screen test go there only)
+// projected float4 (x', y', 0, w with the depth mask), h, the global cell key
+// (the view's first cell cbase[v] + the cell key in the view's grid, so that
+// it orders rows by view, then cell) and the snapshot index.
+// wx: launch_particle_xray.  maxh_bits: the per-cell max h of the shared cell
+// arrays (rows kept out by the screen test go there only)
 void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  const view_build_desc* vd_dev, const int32_t* views_dev,
                                  int nviews, const uint32_t* offs, float4* p4c, float* hc,
-                                 uint32_t* ckey, uint32_t* orig, uint32_t* vid, uint32_t* err,
+                                 uint32_t* gkey, uint32_t* orig, uint32_t* err,
                                  const uint32_t* cbase, uint32_t* maxh_bits, hipStream_t s);
 // keys[j] = descending-h key of h[j] (as launch_h_order), vals[j] = j
 void launch_hq_keys(const float* h, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
 // out[i] = src[idx[i]]
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* out,
                        hipStream_t s);
-// lkey[i] = ckey[order[i]], gkey[i] = cbase[vid[order[i]]] + lkey[i]
-void launch_final_keys(const uint32_t* order, const uint32_t* ckey, const uint32_t* vid,
-                       const uint32_t* cbase, uint32_t n, uint32_t* lkey, uint32_t* gkey,
-                       hipStream_t s);
+// lkey[i] = gkey[i] - cbase[v], v the view whose cells hold gkey[i] (cbase:
+// the nv views' first cells, rising strictly)
+void launch_local_keys(const uint32_t* gkey, const uint32_t* cbase, int nv, uint32_t n,
+                       uint32_t* lkey, hipStream_t s);
 void launch_window_count(const particle_soa& ps, const view_proj& vp, const window_f& win,
                          uint32_t* counts, uint32_t* total, hipStream_t s);
 void launch_window_scatter(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
