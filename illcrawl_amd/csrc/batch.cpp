@@ -598,7 +598,7 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   // cell ranges
   launch_local_keys(b->sh_sk.p, b->sh_cbase.p, static_cast<int>(nv), n32, b->sh_key.p, s);
   launch_fill_u32(b->sh_counts.p, 0u, nc + 1, s);
-  launch_build_soa(b->sh_p4c.p, b->sh_hc.p, b->sh_perm.p, b->sh_sk.p, n32, kernel_norm(b->kernel),
+  launch_build_soa(b->sh_p4c.p, nullptr, b->sh_perm.p, b->sh_sk.p, n32, kernel_norm(b->kernel),
                    b->sh_part.p, b->sh_hs.p, b->sh_counts.p, b->sh_maxh.p, s);
   launch_exclusive_scan(b->sh_counts.p, b->sh_start.p, nc + 1, b->sh_temp.p, s);
   for (size_t v = 0; v < nv; ++v) {

@@ -586,8 +586,10 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
                                                      __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
         float w = vp.quantity == 1 ? wx[i] : static_cast<float>(vp.scale * double(ps.mass[i]));
         if (az < 0.0f || az > vp.depth) w = 0.0f;
-        p4c[j] = make_float4(ax, ay, 0.0f, w);
-        hc[j] = ps.hsml[i];
+        // h rides in the z slot too, so the SoA pass reads one line per row
+        const float hh = ps.hsml[i];
+        p4c[j] = make_float4(ax, ay, hh, w);
+        hc[j] = hh;
         bool in_grid;
         const uint32_t key = cell_key(g, ax, ay, &in_grid);
         if (!in_grid) atomicAdd(err, 1u);

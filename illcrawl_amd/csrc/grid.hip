@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__
   if (valid) {
     const uint32_t i = perm[s];
     const float4 q = p4[i];
-    hh = h[i];
+    hh = h ? h[i] : q.z;
     key = skeys[s];
     const float ih2 = 1.0f / (hh * hh);
     part[s] = make_float4(q.x, q.y, q.w * (norm * ih2), ih2);

@@ -21,7 +21,7 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32
 uint64_t radix_temp_words(uint64_t n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
                        hipStream_t s);
-// coef = w norm / h^2
+// coef = w norm / h^2; h NULL: each particle's h is the z of its p4
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
                       const uint32_t* skeys, uint32_t n, float norm, float4* part, float* hs,
                       uint32_t* count, uint32_t* maxh_bits, hipStream_t s);
@@ -116,7 +116,7 @@ void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* 
                           hipStream_t s);
 // The kept particles of every listed view, in input order, at the global
 // positions offs[v * ntiles + t] (exclusive scan of the multi-window counts):
-// projected float4 (x', y', 0, w with the depth mask), h, the global cell key
+// projected float4 (x', y', h, w with the depth mask), h, the global cell key
 // (the view's first cell cbase[v] + the cell key in the view's grid, so that
 // it orders rows by view, then cell) and the snapshot index.
 // wx: launch_particle_xray.  maxh_bits: the per-cell max h of the shared cell
