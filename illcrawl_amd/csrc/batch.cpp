@@ -153,7 +153,7 @@ struct ilc_batch {
   dbuf<uint32_t> sh_offs, sh_gkey, sh_orig, sh_key, sh_perm, sh_sk, sh_counts, sh_start, sh_maxh,
       sh_temp, sh_cbase;
   pinned<uint32_t> sh_cbase_host;
-  dbuf<float> sh_wx, sh_hc, sh_hs;
+  dbuf<float> sh_wx, sh_hs;
   dbuf<float4> sh_p4c, sh_part;
   dbuf<uint2> items_dev;
   pinned<uint2> items_host;
@@ -561,7 +561,6 @@ void build_shared(ilc_batch* b, hipStream_t s) {
                               scan_temp_words(nc + 1)}));
   for (auto* d : {&b->sh_gkey, &b->sh_orig, &b->sh_key, &b->sh_perm, &b->sh_sk})
     d->ensure(ntot);
-  b->sh_hc.ensure(ntot);
   b->sh_hs.ensure(ntot);
   b->sh_p4c.ensure(ntot);
   b->sh_part.ensure(ntot);
@@ -584,12 +583,11 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   launch_fill_u32(b->sh_maxh.p, 0u, nc, s);
   launch_multi_window_scatter(b->ps, any_xray ? b->sh_wx.p : nullptr, b->bdesc_dev.p,
                               b->fviews_dev.p, static_cast<int>(nv), b->sh_offs.p, b->sh_p4c.p,
-                              b->sh_hc.p, b->sh_gkey.p, b->sh_orig.p, b->counter.p + 1,
-                              b->sh_cbase.p, b->sh_maxh.p, s);
+                              b->sh_sk.p, b->sh_perm.p, b->sh_gkey.p, b->sh_orig.p,
+                              b->counter.p + 1, b->sh_cbase.p, b->sh_maxh.p, s);
   // stable sorts by descending h, then by global cell key (view, then cell):
   // a view's rows end up in one range, by cell, biggest h first, ties in
   // input order
-  launch_hq_keys(b->sh_hc.p, n32, b->sh_sk.p, b->sh_perm.p, s);
   launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, 16, b->sh_temp.p, s);
   launch_gather_u32(b->sh_gkey.p, b->sh_perm.p, n32, b->sh_sk.p, s);
   launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, key_bits_for(b->sh_ncells - 1), b->sh_temp.p,

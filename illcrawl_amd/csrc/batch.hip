@@ -468,6 +468,9 @@ __global__ __launch_bounds__(kBlock) void k_particle_xray(particle_soa ps, gaunt
   }
 }
 
+// descending-h sort key: the top 16 bits of h's FP32 pattern, reversed
+__device__ inline uint32_t hq_key(float h) { return 0x7fffu - (__float_as_uint(h) >> 16); }
+
 // k_window_scatter for every listed view from one read of the tile.  Per
 // view, the kept counts of the kWinItems rows (rows of kBlock particles) are
 // taken per wave first, so one barrier per view gives every row's offset;
@@ -476,8 +479,9 @@ __global__ __launch_bounds__(kBlock) void k_particle_xray(particle_soa ps, gaunt
 __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     particle_soa ps, const float* __restrict__ wx, const view_build_desc* __restrict__ vd,
     const int32_t* __restrict__ views, int nviews, uint32_t ntiles,
-    const uint32_t* __restrict__ offs, float4* __restrict__ p4c, float* __restrict__ hc,
-    uint32_t* __restrict__ gkey, uint32_t* __restrict__ orig, uint32_t* __restrict__ err,
+    const uint32_t* __restrict__ offs, float4* __restrict__ p4c, uint32_t* __restrict__ hqkey,
+    uint32_t* __restrict__ hqval, uint32_t* __restrict__ gkey, uint32_t* __restrict__ orig,
+    uint32_t* __restrict__ err,
     const uint32_t* __restrict__ cbase, uint32_t* __restrict__ maxh_bits) {
 #pragma clang fp contract(off)
   __shared__ __align__(16) uint32_t wc[2][kWinItems][4];
@@ -586,10 +590,12 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
                                                      __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
         float w = vp.quantity == 1 ? wx[i] : static_cast<float>(vp.scale * double(ps.mass[i]));
         if (az < 0.0f || az > vp.depth) w = 0.0f;
-        // h rides in the z slot too, so the SoA pass reads one line per row
+        // h rides in the z slot, so the SoA pass reads one line per row; the
+        // descending-h sort starts from (hq key, row)
         const float hh = ps.hsml[i];
         p4c[j] = make_float4(ax, ay, hh, w);
-        hc[j] = hh;
+        hqkey[j] = hq_key(hh);
+        hqval[j] = j;
         bool in_grid;
         const uint32_t key = cell_key(g, ax, ay, &in_grid);
         if (!in_grid) atomicAdd(err, 1u);
@@ -601,14 +607,6 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_hq_keys(const float* __restrict__ h, uint32_t n,
-                                                    uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals) {
-  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= n) return;
-  keys[j] = 0x7fffu - (__float_as_uint(h[j]) >> 16);
-  vals[j] = j;
-}
 
 __global__ __launch_bounds__(kBlock) void k_gather_u32(const uint32_t* __restrict__ src,
                                                        const uint32_t* __restrict__ idx,
@@ -759,21 +757,16 @@ void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* 
 
 void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  const view_build_desc* vd_dev, const int32_t* views_dev,
-                                 int nviews, const uint32_t* offs, float4* p4c, float* hc,
+                                 int nviews, const uint32_t* offs, float4* p4c, uint32_t* hqkey,
+                                 uint32_t* hqval,
                                  uint32_t* gkey, uint32_t* orig, uint32_t* err,
                                  const uint32_t* cbase, uint32_t* maxh_bits, hipStream_t s) {
   const uint32_t nb = window_tiles(ps.n);
   if (!nb || nviews <= 0) return;
   ILC_REQUIRE(cbase != nullptr, "the scatter needs the views' first cells");
   hipLaunchKernelGGL(k_multi_window_scatter, dim3(nb), dim3(kBlock), 0, s, ps, wx, vd_dev,
-                     views_dev, nviews, nb, offs, p4c, hc, gkey, orig, err, cbase, maxh_bits);
-  ILC_HIP(hipGetLastError());
-}
-
-void launch_hq_keys(const float* h, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_hq_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, h, n, keys,
-                     vals);
+                     views_dev, nviews, nb, offs, p4c, hqkey, hqval, gkey, orig, err, cbase,
+                     maxh_bits);
   ILC_HIP(hipGetLastError());
 }
 

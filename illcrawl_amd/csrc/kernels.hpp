@@ -116,18 +116,17 @@ void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* 
                           hipStream_t s);
 // The kept particles of every listed view, in input order, at the global
 // positions offs[v * ntiles + t] (exclusive scan of the multi-window counts):
-// projected float4 (x', y', h, w with the depth mask), h, the global cell key
+// projected float4 (x', y', h, w with the depth mask), the descending-h sort
+// key (as launch_h_order) and the row index as its value, the global cell key
 // (the view's first cell cbase[v] + the cell key in the view's grid, so that
 // it orders rows by view, then cell) and the snapshot index.
 // wx: launch_particle_xray.  maxh_bits: the per-cell max h of the shared cell
 // arrays (rows kept out by the screen test go there only)
 void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  const view_build_desc* vd_dev, const int32_t* views_dev,
-                                 int nviews, const uint32_t* offs, float4* p4c, float* hc,
-                                 uint32_t* gkey, uint32_t* orig, uint32_t* err,
+                                 int nviews, const uint32_t* offs, float4* p4c, uint32_t* hqkey,
+                                 uint32_t* hqval, uint32_t* gkey, uint32_t* orig, uint32_t* err,
                                  const uint32_t* cbase, uint32_t* maxh_bits, hipStream_t s);
-// keys[j] = descending-h key of h[j] (as launch_h_order), vals[j] = j
-void launch_hq_keys(const float* h, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
 // out[i] = src[idx[i]]
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* out,
                        hipStream_t s);
