@@ -386,8 +386,7 @@ __device__ inline bool reaches_screen(const window_f& win, float ax, float ay, f
 __global__ __launch_bounds__(kBlock) void k_multi_window_count(
     const float* __restrict__ pos3, const float* __restrict__ hsml, uint64_t n,
     const view_build_desc* __restrict__ vd, const int32_t* __restrict__ views, int nviews,
-    uint32_t ntiles, uint32_t* __restrict__ counts, uint32_t* __restrict__ totals,
-    uint32_t* __restrict__ counts_plain, uint32_t* __restrict__ totals_plain) {
+    uint32_t ntiles, uint32_t* __restrict__ counts, uint32_t* __restrict__ counts_plain) {
 #pragma clang fp contract(off)
   __shared__ uint32_t wc[2][8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -444,9 +443,45 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
       const uint32_t tp = wc[slot][4] + wc[slot][5] + wc[slot][6] + wc[slot][7];
       counts[uint64_t(v) * ntiles + blockIdx.x] = t;
       counts_plain[uint64_t(v) * ntiles + blockIdx.x] = tp;
-      if (t) atomicAdd(totals + v, t);
-      if (tp) atomicAdd(totals_plain + v, tp);
     }
+  }
+}
+
+// totals[v] += the sum of view v's tile counts, one workgroup per listed view
+// (per-tile atomics would all land on the few lines of the totals)
+__global__ __launch_bounds__(kBlock) void k_count_totals(const int32_t* __restrict__ views,
+                                                         uint32_t ntiles,
+                                                         const uint32_t* __restrict__ counts,
+                                                         uint32_t* __restrict__ totals,
+                                                         const uint32_t* __restrict__ counts_plain,
+                                                         uint32_t* __restrict__ totals_plain) {
+  __shared__ uint32_t red[2][kBlock / 64];
+  const int v = views[blockIdx.x];
+  const uint32_t* c = counts + uint64_t(v) * ntiles;
+  const uint32_t* cp = counts_plain + uint64_t(v) * ntiles;
+  uint32_t a = 0, b = 0;
+  for (uint32_t t = threadIdx.x; t < ntiles; t += kBlock) {
+    a += c[t];
+    b += cp[t];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += uint32_t(__shfl_xor(int(a), o));
+    b += uint32_t(__shfl_xor(int(b), o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s0 = 0, s1 = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      s0 += red[0][w];
+      s1 += red[1][w];
+    }
+    // atomic: a view listed twice adds twice, as per-tile adds would
+    if (s0) atomicAdd(totals + v, s0);
+    if (s1) atomicAdd(totals_plain + v, s1);
   }
 }
 
@@ -809,7 +844,10 @@ void launch_multi_window_count(const float* pos3, const float* hsml, uint64_t n,
   const uint32_t nb = window_tiles(n);
   if (!nb || nviews <= 0) return;
   hipLaunchKernelGGL(k_multi_window_count, dim3(nb), dim3(kBlock), 0, s, pos3, hsml, n, vd_dev,
-                     views_dev, nviews, nb, counts, totals, counts_plain, totals_plain);
+                     views_dev, nviews, nb, counts, counts_plain);
+  ILC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_count_totals, dim3(uint32_t(nviews)), dim3(kBlock), 0, s, views_dev, nb,
+                     counts, totals, counts_plain, totals_plain);
   ILC_HIP(hipGetLastError());
 }
 
