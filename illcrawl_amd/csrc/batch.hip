@@ -347,31 +347,6 @@ __device__ inline bool in_window(const float* pos3, const view_proj& vp, const w
   return keep;
 }
 
-// counts[b] = kept particles of workgroup b's tile; *total += the same
-__global__ __launch_bounds__(kBlock) void k_window_count(const float* __restrict__ pos3,
-                                                          uint64_t n, view_proj vp, window_f win,
-                                                          uint32_t* __restrict__ counts,
-                                                          uint32_t* __restrict__ total) {
-  __shared__ uint32_t wc[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
-  uint32_t c = 0;
-  for (int r = 0; r < kWinItems; ++r) {
-    const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
-    float px, py;
-    const bool keep = i < n && in_window(pos3, vp, win, i, &px, &py);
-    c += __popcll(__ballot(keep));
-  }
-  if (lane == 0) wc[wave] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t t = wc[0] + wc[1] + wc[2] + wc[3];
-    counts[blockIdx.x] = t;
-    if (total && t) atomicAdd(total, t);
-  }
-}
-
-// counts of k_window_count for every listed view, from one read of the tile
 // the support disk (radius^2 = cull2 h^2) of a particle at (ax, ay) meets
 // the screen rectangle: the gather's tile cull with the screen for the tile
 // box (every tile box lies inside the rectangle, so a particle any tile keeps
@@ -383,6 +358,8 @@ __device__ inline bool reaches_screen(const window_f& win, float ax, float ay, f
   return d2 * ih2 < win.cull2;
 }
 
+// every listed view's kept rows per tile, with and without the screen test,
+// from one read of the tile (k_count_totals sums them per view)
 __global__ __launch_bounds__(kBlock) void k_multi_window_count(
     const float* __restrict__ pos3, const float* __restrict__ hsml, uint64_t n,
     const view_build_desc* __restrict__ vd, const int32_t* __restrict__ views, int nviews,
@@ -848,15 +825,6 @@ void launch_multi_window_count(const float* pos3, const float* hsml, uint64_t n,
   ILC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_count_totals, dim3(uint32_t(nviews)), dim3(kBlock), 0, s, views_dev, nb,
                      counts, totals, counts_plain, totals_plain);
-  ILC_HIP(hipGetLastError());
-}
-
-void launch_window_count(const particle_soa& ps, const view_proj& vp, const window_f& win,
-                         uint32_t* counts, uint32_t* total, hipStream_t s) {
-  const uint32_t nb = window_tiles(ps.n);
-  if (!nb) return;
-  hipLaunchKernelGGL(k_window_count, dim3(nb), dim3(kBlock), 0, s, ps.pos3, ps.n, vp, win, counts,
-                     total);
   ILC_HIP(hipGetLastError());
 }
 

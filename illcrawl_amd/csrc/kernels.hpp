@@ -134,8 +134,6 @@ void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uin
 // the nv views' first cells, rising strictly)
 void launch_local_keys(const uint32_t* gkey, const uint32_t* cbase, int nv, uint32_t n,
                        uint32_t* lkey, hipStream_t s);
-void launch_window_count(const particle_soa& ps, const view_proj& vp, const window_f& win,
-                         uint32_t* counts, uint32_t* total, hipStream_t s);
 void launch_window_scatter(const particle_soa& ps, const view_proj& vp, const gaunt_table& gt,
                            const window_f& win, const uint32_t* offsets, const grid_params& g,
                            float4* p4c, float* hc, uint32_t* keys, uint32_t* vals, uint32_t* orig,
