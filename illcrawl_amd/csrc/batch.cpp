@@ -631,16 +631,21 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   }
 }
 
-void fill_desc(ilc_batch* b, size_t v) {
-  view_state& vs = *b->views[v];
+view_desc make_desc(const ilc_batch* b, size_t v) {
+  const view_state& vs = *b->views[v];
   const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
-  view_desc& d = b->desc_host.p[v];
+  view_desc d{};
   d.g = grid_of(b, vs);
   d.p = point_view{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
   d.acc = vs.acc.p;
   d.tiles_x = vs.tiles_x;
   d.ntiles = vs.ntiles;
+  return d;
 }
+
+// The persistent gather's descriptor of view v.  desc_host is the source of
+// the captured graph's descriptor copy, so only persistent renders write it.
+void fill_desc(ilc_batch* b, size_t v) { b->desc_host.p[v] = make_desc(b, v); }
 
 void ensure_lanes(ilc_batch* b) {
   if (!b->ev_fork) ILC_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
@@ -669,8 +674,7 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
     for (size_t v = 0; v < nv; ++v) {
       view_state& vs = *b->views[v];
       build_view(b, vs, main);
-      fill_desc(b, v);
-      const view_desc& d = b->desc_host.p[v];
+      const view_desc d = make_desc(b, v);
       if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
       launch_gather(d.g, d.p, op, b->kernel, b->ts, vs.acc.p, nullptr, b->cubic.p, s);
       if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
@@ -789,6 +793,9 @@ ilc_status ilc_batch_set_particles(ilc_batch* b, const float* pos3_dev, const fl
     b->have_particles = true;
     b->graph_valid = false;
     b->planned = false;
+    // a new snapshot may need larger buffers: the next graph render first
+    // runs eagerly, so that no allocation happens inside a capture
+    b->ran_eagerly = false;
   });
 }
 
@@ -850,6 +857,10 @@ ilc_status ilc_batch_render(ilc_batch* b, int32_t flags, void* stream) {
         fail(ILC_ENOTREADY, "X-ray view without a Gaunt table");
     ensure_cubic(b);
     plan(b, s);
+    // A plain render points every view at its own grid; a graph replay
+    // re-runs no host code, so the next graph render captures afresh and the
+    // views' grids (read by backward and view_info) are the shared ones again.
+    if (!persistent) b->graph_valid = false;
     if (!use_graph || !b->ran_eagerly) {
       // eager (also sizes every buffer before a later capture)
       device_work(b, persistent, s, true);
@@ -1029,9 +1040,27 @@ ilc_status ilc_batch_autotune(ilc_batch* b, int32_t* tile_w, int32_t* tile_h, do
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     static const tile_shape cand[] = {{16, 1}, {32, 2}, {32, 4}, {64, 4}};
     ensure_cubic(b);  // the cubic gather reads its table from the first launch
-    hipEvent_t e0, e1;
-    ILC_HIP(hipEventCreate(&e0));
-    ILC_HIP(hipEventCreate(&e1));
+    struct events {
+      hipEvent_t e[2] = {nullptr, nullptr};
+      ~events() {
+        for (hipEvent_t x : e)
+          if (x) (void)hipEventDestroy(x);
+      }
+    } ev;
+    ILC_HIP(hipEventCreate(&ev.e[0]));
+    ILC_HIP(hipEventCreate(&ev.e[1]));
+    // on an error the batch keeps its previous tile and renders eagerly next
+    struct restore {
+      ilc_batch* b;
+      tile_shape ts;
+      bool armed = true;
+      ~restore() {
+        if (!armed) return;
+        b->ts = ts;
+        b->graph_valid = false;
+        b->ran_eagerly = false;
+      }
+    } rs{b, b->ts};
     tile_shape best = b->ts;
     float best_t = INFINITY;
     for (const tile_shape& ts : cand) {
@@ -1039,19 +1068,18 @@ ilc_status ilc_batch_autotune(ilc_batch* b, int32_t* tile_w, int32_t* tile_h, do
       b->graph_valid = false;
       plan(b, s);
       device_work(b, true, s);  // warm (allocations, caches)
-      ILC_HIP(hipEventRecord(e0, s));
+      ILC_HIP(hipEventRecord(ev.e[0], s));
       device_work(b, true, s);
-      ILC_HIP(hipEventRecord(e1, s));
-      ILC_HIP(hipEventSynchronize(e1));
+      ILC_HIP(hipEventRecord(ev.e[1], s));
+      ILC_HIP(hipEventSynchronize(ev.e[1]));
       float ms = 0.f;
-      ILC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ILC_HIP(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
       if (ms < best_t) {
         best_t = ms;
         best = ts;
       }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    rs.armed = false;
     b->ts = best;
     b->graph_valid = false;
     b->ran_eagerly = false;

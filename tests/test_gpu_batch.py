@@ -162,6 +162,63 @@ def test_graph_replays_match_eager(B, scene):
             assert np.array_equal(r.image_f64(k).cpu().numpy(), eager[k])
 
 
+def test_graph_after_plain_render(B, scene):
+    """A plain render between graph renders does not leak its per-view grids
+    into the replayed persistent launch: graph, plain, graph all give the
+    eager batched images bit for bit (the plain ones, the single views')."""
+    from illcrawl_amd import synth
+    snap, views = scene
+    tab = synth.gaunt_table_1d()
+    say("case: graph / plain / graph")
+    r = make_batch(B, snap, views, "reference", tab)
+    r.render()
+    eager = [r.image_f64(k).cpu().numpy() for k in range(len(views))]
+    r.render(graph=True)
+    for k in range(len(views)):
+        assert np.array_equal(r.image_f64(k).cpu().numpy(), eager[k]), ("graph 1", k)
+    r.render(plain=True)
+    plain = [r.image_f64(k).cpu().numpy() for k in range(len(views))]
+    for k in range(len(views)):
+        assert_same_sum(plain[k], eager[k], f"plain view {k}")
+    for rep in range(2):
+        r.render(graph=True)
+        torch.cuda.synchronize()
+        for k in range(len(views)):
+            assert np.array_equal(r.image_f64(k).cpu().numpy(), eager[k]), ("graph 2", rep, k)
+        # the views' grids are the shared build's again
+        assert r.view_info(1)["num_particles"] <= snap.n
+
+
+def test_graph_after_larger_snapshot(B, scene):
+    """set_particles with a larger snapshot after graph renders: the next
+    graph render sizes its buffers eagerly before it captures, and its images
+    equal an eager render of the new snapshot on a fresh batch."""
+    from illcrawl_amd import synth
+    snap, views = scene
+    tab = synth.gaunt_table_1d()
+    say("case: graph render after a larger snapshot")
+    half = snap.n // 2
+    r = B.BatchRenderer(mode="reference")
+    r.set_particles(snap.pos[:half], snap.mass[:half], snap.hsml[:half], snap.rho[:half],
+                    snap.temp[:half], snap.xe[:half])
+    r.set_gaunt_table(*tab)
+    r.set_views(views)
+    r.render()
+    r.render(graph=True)
+    r.render(graph=True)
+    r.set_particles(snap.pos, snap.mass, snap.hsml, snap.rho, snap.temp, snap.xe)
+    r.render(graph=True)
+    got = [r.image_f64(k).cpu().numpy() for k in range(len(views))]
+    r.render(graph=True)
+    again = [r.image_f64(k).cpu().numpy() for k in range(len(views))]
+    fresh = make_batch(B, snap, views, "reference", tab)
+    fresh.render()
+    for k in range(len(views)):
+        ref = fresh.image_f64(k).cpu().numpy()
+        assert np.array_equal(got[k], ref), ("first graph render", k)
+        assert np.array_equal(again[k], ref), ("replay", k)
+
+
 def test_batch_counts_and_autotune(B, scene, orc):
     from illcrawl_amd import synth
     snap, views = scene
