@@ -57,10 +57,16 @@ PINNING = {
     "orc_backward_exact": "FP64 central differences of orc_gather_exact in w, h, x, y "
                           "(quartic_ref, quartic, cubic)",
     "orc_xray_flux": "constant Gaunt table: band integral of exp(-E/kT) in closed form "
-                     "(RKF within its tolerance); a table linear in its indices recovers "
-                     "the sampling coordinates ln(gamma^2), ln(u) exactly",
+                     "plus exactly the bias of the reference's 2197/4101 weight, "
+                     "h (2197/4101 - 2197/4104) f(E + 12h/13) per step, on one- and "
+                     "two-step bands; identical step sequences and integrals scaled by "
+                     "lambda when the integrand is scaled by lambda (relative tolerance); "
+                     "a table linear in its indices recovers the sampling coordinates "
+                     "ln(gamma^2), ln(u) exactly",
     "orc_xray_spectral_flux": "closed form of the bremsstrahlung prefactor chain at one "
-                              "energy with a constant table",
+                              "energy with a constant table; the reference's van Hoof "
+                              "table (gaunt_factor.dat): its entries at texel centres and "
+                              "the mean of the four entries around each grid node",
     "orc_crawl_block": "hand-built rows: periodic images moved in by one period (each "
                        "axis, both signs), border tolerance, inclusive volume faces, "
                        "input order kept; numpy brute force on random rows",

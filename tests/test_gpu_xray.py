@@ -78,6 +78,28 @@ def test_xray_flux_vs_oracle(B, orc, num_samples, band):
     assert st.max() < 65536
 
 
+def reference_table():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "gaunt_factor_vanhoof.npy"))
+
+
+@pytest.mark.parametrize("num_samples", [10, 1000])
+def test_xray_flux_with_the_reference_gaunt_table(B, orc, num_samples):
+    """The reference's own van Hoof table (gaunt_factor.dat, tests/golden)."""
+    say(f"case: xray_flux, reference Gaunt table, num_samples={num_samples}")
+    rho, u, xe = gas(50000, 100 + num_samples)
+    tab = reference_table()
+    got, steps = B.quantity_xray_flux(rho, u, xe, tab, 0.02, 1e5, 0.5, 7.0, num_samples,
+                                      with_steps=True)
+    ref, rsteps = orc.xray_flux(rho, u, xe, 0.02, 1e5, tab, 0.5, 7.0, num_samples)
+    agree(got.cpu().numpy(), ref, "flux")
+    assert float((steps.cpu().numpy() == rsteps).mean()) >= 0.999
+    for E, dE in ((0.7, 0.1), (5.0, 1.0)):
+        got = B.quantity_xray_spectral_flux(rho, u, xe, tab, 0.0, 1e5, E, dE).cpu().numpy()
+        ref = orc.xray_spectral_flux(rho, u, xe, 0.0, 1e5, tab, E, dE)
+        agree(got, ref, f"spectral E={E}")
+
+
 def test_xray_spectral_flux_vs_oracle(B, orc):
     say("case: xray_spectral_flux")
     rho, u, xe = gas(100000, 7)

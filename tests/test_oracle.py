@@ -594,3 +594,123 @@ def test_smooth3d_ref_drops_what_the_search_box_excludes(orc):
     big = orc.quartic3d(np.linalg.norm(p4[2, :3] - pts[0, :3]), h[2])
     assert big > 0                         # the big particle's support reaches the point
     assert ref == pytest.approx(near, rel=1e-6)   # but the search box drops it
+
+
+# ------------------------------------------- RKF band integral and Gaunt data pins
+
+_W_BIAS = 2197.0 * 3.0 / (4101.0 * 4104.0)  # 2197/4101 - 2197/4104
+
+
+def _brems_closed(rho, u, xe, g0):
+    """C and kT (keV) of the constant-table integrand f(E) = C exp(-E/kT)."""
+    T = _temperature64(np.float64(np.float32(u)), np.float64(np.float32(xe)))
+    ion = 4.046263 * np.float64(np.float32(rho)) * np.float64(np.float32(xe))
+    return 0.002535 * ion ** 2 * g0 / np.sqrt(T), K_B * T / 1000.0
+
+
+@pytest.mark.parametrize("ns,e0,e1", [(1, 2.0, 2.5), (2, 2.0, 2.5), (1, 2.0, 3.0), (4, 1.0, 2.5)])
+def test_xray_flux_rkf_weights_closed_form(orc, ns, e0, e1):
+    """With num_samples = ns the first step is h0 = (E1 - E0) / ns
+    (quantities.cl:201-202).  For an integrand of order one the step control
+    accepts it and grows the next step about sevenfold, past the band's end,
+    which trims it to the end (integration.cl:86-130): the band is two steps,
+    h0 and E1 - E0 - h0 (one when ns = 1).  For f = C exp(-E/kT) with steps
+    below kT/2, Fehlberg's 4th-order rule is exact to 1e-7, so the
+    reference's weight 2197/4101 (integration.cl:69, where Fehlberg has
+    2197/4104) adds exactly h * (2197/4101 - 2197/4104) * f(E + 12h/13) per
+    step to the closed form C kT (exp(-E0/kT) - exp(-E1/kT)).  A 4104
+    transcription misses by 3.9e-4."""
+    g0 = 1.21
+    tab = np.full((146, 81), g0, np.float32)
+    xe = 1.1
+    mu = 4.0 / (1.0 + 3 * 0.76 + 4 * 0.76 * xe)
+    u = (4000.0 / K_B) / ((2.0 / 3.0) * mu * 121.1475502)  # kT = 4 keV
+    rho = 450.0                                            # f(2 keV) ~ 1
+    got, steps = orc.xray_flux(np.array([rho]), np.array([u]), np.array([xe]), 0.0, 1.0, tab,
+                               e0, e1, ns)
+    C, kT = _brems_closed(rho, u, xe, g0)
+    f = lambda E: C * np.exp(-E / kT)  # noqa: E731
+    exact = C * kT * (np.exp(-e0 / kT) - np.exp(-e1 / kT))
+    h0 = (e1 - e0) / ns
+    hs = [h0] if ns == 1 else [h0, e1 - e0 - h0]
+    starts = [e0, e0 + h0]
+    bias = sum(_W_BIAS * h * f(a + 12.0 / 13.0 * h) for a, h in zip(starts, hs))
+    assert int(steps[0]) == len(hs)
+    assert float(got[0]) / (exact + bias) == pytest.approx(1.0, abs=1e-6)
+    assert float(got[0]) / exact - 1.0 > 0.8 * _W_BIAS
+
+
+def test_xray_flux_relative_tolerance_is_scale_free(orc):
+    """With a negligible absolute tolerance (num_samples = 1e6) the step
+    control compares each step's error with |running integral / position|
+    (integration.cl:88-96): scaling the integrand by lambda scales both, so the
+    step sequence is the same and the band integral scales by lambda.  Here
+    lambda = (rho / rho0)^2 through the ion density."""
+    tab = np.full((146, 81), 1.21, np.float32)
+    xe = 1.1
+    mu = 4.0 / (1.0 + 3 * 0.76 + 4 * 0.76 * xe)
+    u = (4000.0 / K_B) / ((2.0 / 3.0) * mu * 121.1475502)
+    rho = np.array([450.0, 900.0, 1800.0, 3600.0], np.float32)
+    got, steps = orc.xray_flux(rho, np.full(4, u), np.full(4, xe), 0.0, 1.0, tab, 0.5, 7.0,
+                               10 ** 6)
+    assert len(set(steps.tolist())) == 1 and steps[0] > 4
+    lam = (rho.astype(np.float64) / 450.0) ** 2
+    assert np.allclose(got / got[0], lam, rtol=1e-6)
+
+
+def _gaunt_fixture():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "gaunt_factor_vanhoof.npy"))
+
+
+def _g_at(orc, tab, cg, cu):
+    """g_ff (quantities.cl:69-83) read back from one bin of
+    xray_spectral_flux at the texel coordinates (cg, cu): ln(gamma^2) =
+    -6 + 0.2 cg, ln(u) = -16 + 0.2 cu (the table's first node at 0)."""
+    kT_eV = RYD / np.exp(-6.0 + 0.2 * cg)
+    T = kT_eV / K_B
+    xe = 1.1
+    mu = 4.0 / (1.0 + 3 * 0.76 + 4 * 0.76 * xe)
+    u_int = T / ((2.0 / 3.0) * mu * 121.1475502)
+    E = np.exp(-16.0 + 0.2 * cu) * kT_eV / 1000.0
+    rho = 1e3
+    got = float(orc.xray_spectral_flux(np.array([rho]), np.array([u_int]), np.array([xe]), 0.0,
+                                       1.0, tab, E, 1.0)[0])
+    Tf = _temperature64(np.float64(np.float32(u_int)), np.float64(np.float32(xe)))
+    ion = 4.046263 * np.float64(np.float32(rho)) * np.float64(np.float32(xe))
+    pref = 0.002535 * ion ** 2 / np.sqrt(Tf) * np.exp(-float(np.float32(E)) * 1000.0 / (K_B * Tf))
+    return got / pref
+
+
+def test_gaunt_table_fixture_shape_and_values():
+    """The reference's table (tests/golden, from gaunt_factor.dat): 146 rows of
+    ln u by 81 columns of ln gamma^2; g_ff is near 20.76 at the first entry,
+    falls towards large u, and is positive everywhere."""
+    tab = _gaunt_fixture()
+    assert tab.shape == (146, 81) and tab.dtype == np.float32
+    assert tab[0, 0] == np.float32(2.07575626e+01) and tab[0, 1] == np.float32(2.07575534e+01)
+    assert tab[1, 0] == np.float32(2.05036528e+01)
+    assert (tab > 0).all() and tab[-1].max() < tab[0].min()
+
+
+def test_gff_at_texel_centres_and_nodes_of_the_reference_table(orc):
+    """read_imagef with a linear filter on unnormalised coordinates
+    (tabulated_function.cl:46-59): at a texel centre (coordinate i + 0.5) it
+    returns that table entry; at a grid node (coordinate i) the mean of the
+    four entries around it.  Checked on the reference's own table at nodes
+    where the mean differs from every single entry by more than the
+    tolerance, so a missing half-texel shift, swapped axes or log10 axes
+    fail."""
+    tab = _gaunt_fixture()
+    checked = 0
+    for cg in (3, 11, 20, 33, 47, 60, 71, 79):
+        for cu in (4, 17, 30, 45, 58, 70, 84):
+            centre = _g_at(orc, tab, cg + 0.5, cu + 0.5)
+            assert centre == pytest.approx(float(tab[cu, cg]), rel=2e-5), (cg, cu)
+            quad = tab[cu - 1:cu + 1, cg - 1:cg + 1].astype(np.float64)
+            mean = float(quad.mean())
+            node = _g_at(orc, tab, cg, cu)
+            assert node == pytest.approx(mean, rel=2e-5), (cg, cu)
+            if np.min(np.abs(quad - mean)) > 1e-4 * mean:
+                checked += 1
+    assert checked >= 20
