@@ -106,6 +106,8 @@ SIGNATURES = {
     "ilc_particles_set_w": (_ST, [_P, _P, _U64, _P]),
     "ilc_crawl_block": (_ST, [ctypes.POINTER(VolumeC), _P, _U64, _P, ctypes.c_int32, _P, _P,
                               _P, ctypes.POINTER(_U64), _P]),
+    "ilc_crawl_block_static": (_ST, [ctypes.POINTER(VolumeC), _P, _U64, _P, ctypes.c_int32, _P,
+                                     _P, _P, ctypes.POINTER(_U64), _P]),
     "ilc_quantity_xray_flux": (_ST, [_P, _P, _P, _U64, _P, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_int32, _P, _P, _P]),

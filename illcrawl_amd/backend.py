@@ -429,10 +429,12 @@ def project_particles(cam, particles4, device=0, stream=None, out=None):
 
 
 def crawl_block(center, extent, periodic_wraparound, border, xyz, columns=(), device=0,
-                stream=None, with_index=False):
+                stream=None, with_index=False, static_scratch=False):
     """Block body of the data crawler (reconstructing_data_crawler.cpp:180-213):
     periodic correction towards the render volume, cutout, compaction in input
-    order.  Returns (particles4 (m, 4) with w = 0, [columns (m,)], [index (m,)])."""
+    order.  Returns (particles4 (m, 4) with w = 0, [columns (m,)], [index (m,)]).
+    static_scratch: use ilc_crawl_block_static, which allocates nothing (blocks
+    of at most 2^29 rows; calls overlapping on one GPU race)."""
     dev = torch.device("cuda", device)
     x = _dev_f32(xyz, dev, 3)
     n = x.shape[0]
@@ -448,7 +450,8 @@ def crawl_block(center, extent, periodic_wraparound, border, xyz, columns=(), de
     cin = (ctypes.c_void_p * k)(*[c.data_ptr() for c in cols])
     cout = (ctypes.c_void_p * k)(*[c.data_ptr() for c in outs])
     m = ctypes.c_uint64()
-    N.call("ilc_crawl_block", ctypes.byref(vol), _ptr(x), n, ctypes.cast(cin, ctypes.c_void_p),
+    fn = "ilc_crawl_block_static" if static_scratch else "ilc_crawl_block"
+    N.call(fn, ctypes.byref(vol), _ptr(x), n, ctypes.cast(cin, ctypes.c_void_p),
            len(cols), _ptr(out4), ctypes.cast(cout, ctypes.c_void_p), _ptr(idx), ctypes.byref(m),
            _stream(stream))
     m = m.value

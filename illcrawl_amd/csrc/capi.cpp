@@ -577,51 +577,89 @@ ilc_status ilc_particles_set_w(float* particles4_dev, const float* w_dev, uint64
   });
 }
 
+}  // extern "C"
+
+namespace {
+// Shared body of ilc_crawl_block and ilc_crawl_block_static: `get_scratch`
+// returns the counts / offsets / scan scratch of the given number of words.
+template <class Scratch>
+void crawl_block_body(const ilc_volume* vol, const float* xyz_dev, uint64_t n,
+                      const float* const* columns_dev, int32_t ncolumns, float* particles4_dev,
+                      float* const* columns_out_dev, uint64_t* index_out_dev, uint64_t* n_out,
+                      void* stream, Scratch&& get_scratch) {
+  ILC_REQUIRE(vol && n_out, "null argument");
+  ILC_REQUIRE(ncolumns >= 0 && ncolumns <= kMaxCrawlColumns, "ncolumns %d not in [0, %d]",
+              ncolumns, kMaxCrawlColumns);
+  ILC_REQUIRE(n == 0 || (xyz_dev && particles4_dev), "null argument");
+  ILC_REQUIRE(ncolumns == 0 || (columns_dev && columns_out_dev), "null column list");
+  *n_out = 0;
+  if (!n) return;
+  crawl_columns cols{};
+  cols.n = ncolumns;
+  for (int k = 0; k < ncolumns; ++k) {
+    ILC_REQUIRE(columns_dev[k] && columns_out_dev[k], "null column %d", k);
+    cols.in[k] = columns_dev[k];
+    cols.out[k] = columns_out_dev[k];
+  }
+  double vmin[3], vmax[3], gmin[3], gmax[3];
+  for (int j = 0; j < 3; ++j) {
+    vmin[j] = vol->center[j] - 0.5 * vol->extent[j];
+    vmax[j] = vol->center[j] + 0.5 * vol->extent[j];
+    gmin[j] = vmin[j] - vol->border;
+    gmax[j] = vmax[j] + vol->border;
+  }
+  hipStream_t s = as_stream(stream);
+  const uint64_t nt = crawl_tiles(n);
+  uint32_t* work = get_scratch(2 * (nt + 1) + scan_temp_words(nt + 1));
+  uint32_t* counts = work;
+  uint32_t* offsets = work + nt + 1;
+  // every tile's count is written by the count pass; the scan's extra
+  // element must read 0
+  launch_fill_u32(counts + nt, 0u, 1, s);
+  launch_crawl(vol->periodic_wraparound, gmin, gmax, vmin, vmax, xyz_dev, n, cols, counts,
+               offsets, work + 2 * (nt + 1), reinterpret_cast<float4*>(particles4_dev),
+               index_out_dev, s);
+  uint32_t m = 0;
+  ILC_HIP(hipMemcpyAsync(&m, offsets + nt, sizeof m, hipMemcpyDeviceToHost, s));
+  ILC_HIP(hipStreamSynchronize(s));
+  *n_out = m;
+}
+}  // namespace
+
+extern "C" {
+
 ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t n,
                            const float* const* columns_dev, int32_t ncolumns,
                            float* particles4_dev, float* const* columns_out_dev,
                            uint64_t* index_out_dev, uint64_t* n_out, void* stream) {
   return guarded([&] {
-    ILC_REQUIRE(vol && n_out, "null argument");
-    ILC_REQUIRE(ncolumns >= 0 && ncolumns <= kMaxCrawlColumns, "ncolumns %d not in [0, %d]",
-                ncolumns, kMaxCrawlColumns);
-    ILC_REQUIRE(n == 0 || (xyz_dev && particles4_dev), "null argument");
-    ILC_REQUIRE(ncolumns == 0 || (columns_dev && columns_out_dev), "null column list");
-    *n_out = 0;
-    if (!n) return;
-    crawl_columns cols{};
-    cols.n = ncolumns;
-    for (int k = 0; k < ncolumns; ++k) {
-      ILC_REQUIRE(columns_dev[k] && columns_out_dev[k], "null column %d", k);
-      cols.in[k] = columns_dev[k];
-      cols.out[k] = columns_out_dev[k];
-    }
-    double vmin[3], vmax[3], gmin[3], gmax[3];
-    for (int j = 0; j < 3; ++j) {
-      vmin[j] = vol->center[j] - 0.5 * vol->extent[j];
-      vmax[j] = vol->center[j] + 0.5 * vol->extent[j];
-      gmin[j] = vmin[j] - vol->border;
-      gmax[j] = vmax[j] + vol->border;
-    }
-    hipStream_t s = as_stream(stream);
-    const uint64_t nt = crawl_tiles(n);
-    const uint64_t words = 2 * (nt + 1) + scan_temp_words(nt + 1);
-    uint32_t* scratch = nullptr;
-    ILC_HIP(hipMalloc(&scratch, sizeof(uint32_t) * words));
+    uint32_t* work = nullptr;
     struct freer {
-      uint32_t* p;
-      ~freer() { (void)hipFree(p); }
-    } fr{scratch};
-    uint32_t* counts = scratch;
-    uint32_t* offsets = scratch + nt + 1;
-    launch_fill_u32(counts + nt, 0u, 1, s);
-    launch_crawl(vol->periodic_wraparound, gmin, gmax, vmin, vmax, xyz_dev, n, cols, counts,
-                 offsets, scratch + 2 * (nt + 1), reinterpret_cast<float4*>(particles4_dev),
-                 index_out_dev, s);
-    uint32_t m = 0;
-    ILC_HIP(hipMemcpyAsync(&m, offsets + nt, sizeof m, hipMemcpyDeviceToHost, s));
-    ILC_HIP(hipStreamSynchronize(s));
-    *n_out = m;
+      uint32_t** p;
+      ~freer() {
+        if (*p) (void)hipFree(*p);
+      }
+    } fr{&work};
+    crawl_block_body(vol, xyz_dev, n, columns_dev, ncolumns, particles4_dev, columns_out_dev,
+                     index_out_dev, n_out, stream, [&](uint64_t words) {
+                       ILC_HIP(hipMalloc(&work, sizeof(uint32_t) * words));
+                       return work;
+                     });
+  });
+}
+
+ilc_status ilc_crawl_block_static(const ilc_volume* vol, const float* xyz_dev, uint64_t n,
+                                  const float* const* columns_dev, int32_t ncolumns,
+                                  float* particles4_dev, float* const* columns_out_dev,
+                                  uint64_t* index_out_dev, uint64_t* n_out, void* stream) {
+  return guarded([&] {
+    if (n > kCrawlStaticMaxRows)
+      fail(ILC_ERANGE, "block of %llu rows exceeds the allocation-free maximum of %llu rows",
+           (unsigned long long)n, (unsigned long long)kCrawlStaticMaxRows);
+    static_assert(ILC_CRAWL_STATIC_MAX_ROWS == kCrawlStaticMaxRows, "header limit");
+    crawl_block_body(vol, xyz_dev, n, columns_dev, ncolumns, particles4_dev, columns_out_dev,
+                     index_out_dev, n_out, stream,
+                     [](uint64_t words) { return crawl_static_scratch(words); });
   });
 }
 

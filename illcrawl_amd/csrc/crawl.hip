@@ -10,6 +10,9 @@
 // particle block handed to the grid is the reference's.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -19,6 +22,16 @@ namespace {
 
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;
+
+// Scratch of the allocation-free block body: per-tile counts, their
+// exclusive scan and the scan's block sums, for blocks of up to
+// kCrawlStaticMaxRows rows (1 MiB; every GPU holds its own copy for the life
+// of the process).  Host code never passes `scratch` itself to a kernel: its
+// device address comes from hipGetSymbolAddress, once per device.
+constexpr uint64_t kStaticTiles = kCrawlStaticMaxRows / kTile;
+constexpr uint64_t kStaticWords = 2 * (kStaticTiles + 1) + 64;
+static_assert(kCrawlStaticMaxRows % kTile == 0, "whole tiles");
+static __device__ __align__(256) uint32_t scratch[kStaticWords];
 
 struct crawl_params {
   double period[3];
@@ -99,6 +112,24 @@ __global__ __launch_bounds__(kBlock) void k_crawl_scatter(const float* __restric
 }  // namespace
 
 uint64_t crawl_tiles(uint64_t n) { return (n + kTile - 1) / kTile; }
+
+uint32_t* crawl_static_scratch(uint64_t words) {
+  ILC_REQUIRE(words <= kStaticWords, "crawl scratch of %llu words exceeds the static %llu",
+              (unsigned long long)words, (unsigned long long)kStaticWords);
+  int dev = 0;
+  ILC_HIP(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::vector<uint32_t*> addr;  // per device
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr.size() <= size_t(dev)) addr.resize(size_t(dev) + 1, nullptr);
+  if (!addr[size_t(dev)]) {
+    void* p = nullptr;
+    ILC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ILC_REQUIRE(p != nullptr, "no device address for the crawl scratch on device %d", dev);
+    addr[size_t(dev)] = static_cast<uint32_t*>(p);
+  }
+  return addr[size_t(dev)];
+}
 
 void launch_crawl(const double period[3], const double gmin[3], const double gmax[3],
                   const double vmin[3], const double vmax[3], const float* xyz, uint64_t n,

@@ -71,6 +71,11 @@ struct crawl_columns {
   int n;
 };
 uint64_t crawl_tiles(uint64_t n);
+// Largest block of the allocation-free block body (ILC_CRAWL_STATIC_MAX_ROWS).
+constexpr uint64_t kCrawlStaticMaxRows = uint64_t(1) << 29;
+// Device address, on the current device, of the block body's static scratch
+// (`words` uint32 at most; fails above its size).  Calls on one device share it.
+uint32_t* crawl_static_scratch(uint64_t words);
 // counts/offsets: crawl_tiles(n) + 1 words each; scan_temp: scan_temp_words(tiles + 1)
 void launch_crawl(const double period[3], const double gmin[3], const double gmax[3],
                   const double vmin[3], const double vmax[3], const float* xyz, uint64_t n,

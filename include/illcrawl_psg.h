@@ -264,6 +264,21 @@ ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t
                            float* particles4_dev, float* const* columns_out_dev,
                            uint64_t* index_out_dev, uint64_t* n_out, void* stream);
 
+/* ilc_crawl_block without any allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and synchronisation as
+ * ilc_crawl_block; its per-tile counts and their scan live in a static
+ * device array of 1 MiB per GPU, held until the process ends, so blocks of
+ * more than ILC_CRAWL_STATIC_MAX_ROWS rows are refused (ILC_ERANGE).  Every
+ * call on one GPU uses that GPU's array: calls that overlap on one GPU (other
+ * host threads, other streams) race. */
+#define ILC_CRAWL_STATIC_MAX_ROWS (1ull << 29)
+ilc_status ilc_crawl_block_static(const ilc_volume* vol, const float* xyz_dev, uint64_t n,
+                                  const float* const* columns_dev, int32_t ncolumns,
+                                  float* particles4_dev, float* const* columns_out_dev,
+                                  uint64_t* index_out_dev, uint64_t* n_out, void* stream);
+
 /* ------------------------------------------- volumetric smoothing (dm::grid) */
 
 /* reconstruction_backends::dm::grid (dm_reconstruction_backend_grid.hpp:32-60,

@@ -118,3 +118,15 @@ def test_gaunt_table_loader(tmp_path):
     f.write_text("1 2 3\n4 5\n")
     with pytest.raises(ValueError):
         backend.load_gaunt_table(str(f))
+
+
+def test_static_crawl_block_refuses_oversized_blocks_without_a_gpu():
+    """ilc_crawl_block_static checks its size limit before any HIP call."""
+    vol = N.VolumeC((ctypes.c_double * 3)(0, 0, 0), (ctypes.c_double * 3)(1, 1, 1),
+                    (ctypes.c_double * 3)(10, 10, 10), 0.0)
+    m = ctypes.c_uint64(7)
+    st = N.lib().ilc_crawl_block_static(ctypes.byref(vol), None, (1 << 29) + 1, None, 0, None,
+                                        None, None, ctypes.byref(m), None)
+    assert st == N.ILC_ERANGE
+    assert b"allocation-free maximum" in N.lib().ilc_last_error()
+    assert m.value == 7
