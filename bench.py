@@ -469,6 +469,54 @@ def volumetric_main(args):
     print(json.dumps(line), flush=True)
 
 
+def breakdown_main(args, snap, views, kernel, tile):
+    """Where the batched-vs-plain gap comes from: every resolution group of
+    the workload rendered alone, batched (one persistent launch) and plain
+    (one view per launch sequence), with the gather share of each step and
+    the group's exact evals; then the whole batch."""
+    import numpy as np
+    import torch
+    from illcrawl_amd import backend as B, synth
+    tab, x0, dx = synth.gaunt_table_1d()
+    groups = {}
+    for k, v in enumerate(views):
+        groups.setdefault((v["nx"], v["ny"]), []).append(k)
+    groups["all"] = list(range(len(views)))
+
+    def timed(r, plain, steps):
+        r.render(plain=plain)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gs = []
+        for _ in range(steps):
+            r.render(plain=plain)
+            gs.append(r.last_gather_ms())
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, float(np.mean(gs))
+
+    out = {}
+    for key, idx in groups.items():
+        r = B.BatchRenderer(mode=args.mode, kernel=kernel, tile=tile)
+        r.set_particles(snap.pos, snap.mass, snap.hsml, snap.rho, snap.temp, snap.xe)
+        r.set_gaunt_table(tab, x0, dx)
+        r.set_views([views[k] for k in idx])
+        r.render()
+        evals = int(sum(r.count_evals()))
+        b_ms, b_g = timed(r, False, args.steps)
+        p_ms, p_g = timed(r, True, max(1, args.plain_steps))
+        name = key if key == "all" else f"{key[0]}x{key[1]}"
+        out[name] = dict(views=len(idx), evals=evals, batched_ms=b_ms, batched_gather_ms=b_g,
+                         plain_ms=p_ms, plain_gather_ms=p_g, ratio=p_ms / b_ms,
+                         batched_gather_evals_per_s=evals / (b_g * 1e-3),
+                         plain_gather_evals_per_s=evals / (p_g * 1e-3))
+        log(f"[bench] {name}: batched {b_ms:.2f} ms (gather {b_g:.2f}), plain {p_ms:.2f} ms "
+            f"(gathers {p_g:.2f}), evals {evals:.4g}")
+        del r
+        torch.cuda.empty_cache()
+    print(json.dumps({"workload": args.workload, "tile": list(tile), "breakdown": out}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -486,6 +534,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--breakdown", action="store_true",
+                    help="time every resolution group alone, batched and plain (diagnostic)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (a known-traffic kernel for "
                          "scaling hardware byte counters)")
@@ -529,6 +579,9 @@ def main():
         tile, best_ms = r.autotune()
         tuned = dict(tile=list(tile), best_ms=best_ms)
         log(f"[bench] autotuned tile {tile} ({best_ms:.1f} ms)")
+    if args.breakdown:
+        del r
+        return breakdown_main(args, snap, views, kernel, tile)
     r.render()
     counts = r.count_evals()
     total_evals = int(sum(counts))

@@ -11,8 +11,9 @@
 //      host derives each view's reference grid (particle_grid.cpp:43-100);
 //   2. device work: per view, the projected particles and the grid build
 //      (keys, stable radix sort, sorted SoA, cell offsets), all resident;
-//      then ONE persistent gather over every (view, tile) work item, ordered
-//      largest views first, pulled from a device counter; then per-view
+//      then ONE persistent gather over every (view, tile) work item, the
+//      views with the coarsest pixels first, pulled from per-XCD device
+//      counters; then per-view
 //      retrieval (FP64 -> FP32 times the view's dA scale).
 // Step 2 can be captured in a hipGraph and replayed while the plan is
 // unchanged.
@@ -440,13 +441,18 @@ void plan(ilc_batch* b, hipStream_t s) {
   b->sh_ncells = ncells;
   sig.push_back(b->shared ? ntot : 0);
   sig.push_back(uint64_t(b->ts.tw) << 32 | uint64_t(b->ts.ppt));
-  // work items: largest views first, tiles in image order
+  // work items: views with the coarsest pixels first (their tiles cover the
+  // most area, so over dense regions they stream the most particles and run
+  // longest: started first, they finish while the finer views' many light
+  // tiles keep the GPU busy), then larger views first; tiles in image order
   std::vector<uint32_t> order(nv);
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
-    const uint64_t pa = uint64_t(b->views[a]->v.cam.nx) * b->views[a]->v.cam.ny;
-    const uint64_t pc = uint64_t(b->views[c]->v.cam.nx) * b->views[c]->v.cam.ny;
-    return pa > pc;
+    const ilc_camera& ca = b->views[a]->v.cam;
+    const ilc_camera& cc = b->views[c]->v.cam;
+    const double psa = ca.width / double(ca.nx), psc = cc.width / double(cc.nx);
+    if (psa != psc) return psa > psc;
+    return uint64_t(ca.nx) * ca.ny > uint64_t(cc.nx) * cc.ny;
   });
   uint64_t nitems = 0;
   for (size_t v = 0; v < nv; ++v) {
@@ -597,6 +603,7 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   launch_local_keys(b->sh_sk.p, b->sh_cbase.p, static_cast<int>(nv), n32, b->sh_key.p, s);
   launch_fill_u32(b->sh_counts.p, 0u, nc + 1, s);
   launch_build_soa(b->sh_p4c.p, nullptr, b->sh_perm.p, b->sh_sk.p, n32, kernel_norm(b->kernel),
+                   b->kernel != ILC_KERNEL_CUBIC,
                    b->sh_part.p, b->sh_hs.p, b->sh_counts.p, b->sh_maxh.p, s);
   launch_exclusive_scan(b->sh_counts.p, b->sh_start.p, nc + 1, b->sh_temp.p, s);
   for (size_t v = 0; v < nv; ++v) {

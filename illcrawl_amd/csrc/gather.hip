@@ -152,14 +152,26 @@ struct adjoint_io {
   float* gy;
 };
 
+// the small and masked lists share one array from its two ends: before a
+// batch of kBlock candidates each holds at most kFlush - 1 entries
+constexpr int kPool = kCap + kFlush;
+// a flush compacts each wave's small / masked entries only for lists of at
+// least this length (shorter lists cost less to evaluate whole)
+constexpr int kCompactMin = kFlush;
+static_assert(2 * (kFlush - 1) + kBlock <= kPool, "pool capacity");
+
 template <bool REF, int KERNEL, bool ADJ = false>
 struct gather_smem {
   float4 Lb[KERNEL == 0 && !ADJ ? kCap : 1];  // big: expansion coefficients
-  float4 Ls[kCap];                    // small: position form
-  float4 Lm[REF ? kCap : 1];          // masked: position form
+  // small (position form) from the front; reference mode: masked (position
+  // form) from the back, the m-th at Ls[kPool - 1 - m]
+  float4 Ls[REF ? kPool : kCap];
   alignas(16) int Lmc[REF ? kCap : 1];  // masked: packed cell (cx | cy << 16)
+  // per wave, at a flush: the positions of the small (then masked) entries
+  // whose support reaches the wave's points
+  alignas(8) uint16_t wl[ADJ ? 1 : 4][kCap];
   uint32_t wcnt[2][16];
-  float redf[4][4];
+  alignas(16) float redf[4][4];  // per wave: box of its valid points (x0, y0, x1, y1)
   int redi[4][10];
   float tab[KERNEL == 2 ? kCubicTable : 1];
   float dtab[KERNEL == 2 && ADJ ? kCubicTable : 1];
@@ -391,6 +403,36 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   cz1 = uniform(cz1);
   const bool allz_u = uniform(r_allz) != 0;
   const float cull2 = COUNT ? support * support * 1.0001f : support * support;
+  // At a flush each wave evaluates only the small or masked entries whose
+  // support disk (with a slack far beyond FP32 rounding) meets the box of its
+  // own valid points: at its points the others add exactly 0.  compact()
+  // writes the list positions of those entries, in list order, to wl[wave]
+  // (small entries: position j at Ls[j]; masked: at Ls[kPool - 1 - j]).
+  const float cullw = support * support * 1.0001f;
+  constexpr int kPoolN = REF ? kPool : kCap;
+  auto compact = [&](bool masked, int cnt) -> int {
+    const float4 bx = *reinterpret_cast<const float4*>(sm.redf[wave]);
+    int wn = 0;
+    for (int j0 = 0; j0 < cnt; j0 += 64) {
+      const int j = j0 + lane;
+      bool mine = false;
+      if (j < cnt) {
+        // (x, y, c/h^2, c), or the cubic's (x, y, coef, 1/h^2)
+        const float4 e = sm.Ls[masked ? kPoolN - 1 - j : j];
+        const float dx = fmaxf(fmaxf(bx.x - e.x, e.x - bx.z), 0.0f);
+        const float dy = fmaxf(fmaxf(bx.y - e.y, e.y - bx.w), 0.0f);
+        const float d2 = fmaf(dy, dy, dx * dx);
+        mine = KERNEL == 2 ? d2 * e.w < cullw : d2 * e.z < cullw * e.w;
+      }
+      const uint64_t bal = __ballot(mine);
+      if (mine) sm.wl[wave][wn + int(prefix_count(bal))] = uint16_t(j);
+      wn += __popcll(bal);
+    }
+    // the wave reads what its lanes wrote
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    return uniform(wn);
+  };
 
   // list sizes (identical in every thread).  The big list holds the
   // particles whose support covers the whole tile (argument positive at every
@@ -437,8 +479,46 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       run(kCap - cntp, cntp, std::true_type{});
     }
   };
+  auto small_pair = [&](float* a, const float4& q) {
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const float dx = q.x - ex[k], dy = q.y - ey[k];
+      const float r2 = fmaf(dy, dy, dx * dx);
+      if constexpr (KERNEL == 2)
+        a[k] = fmaf(q.z, cubic_shape(r2 * q.w, sm.tab), a[k]);
+      else
+        a[k] += pow25(fmaf(-r2, q.z, q.w));
+    }
+  };
   auto eval_small = [&](int cnt) {
     cnt = uniform(cnt);
+    if constexpr (!COUNT && !ADJ) {
+      if (cnt >= kCompactMin) {
+      const int wn = compact(false, cnt);
+      const uint16_t* my = sm.wl[wave];
+      for (int i0 = 0; i0 < wn; i0 += 32) {
+        const int in = min(32, wn - i0);
+        float a[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
+        int i = 0;
+        for (; i + 4 <= in; i += 4) {
+          const uint2 ix = *reinterpret_cast<const uint2*>(my + i0 + i);
+          float4 q[4];
+          q[0] = sm.Ls[ix.x & 0xffffu];
+          q[1] = sm.Ls[ix.x >> 16];
+          q[2] = sm.Ls[ix.y & 0xffffu];
+          q[3] = sm.Ls[ix.y >> 16];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) small_pair(a, q[u]);
+        }
+        for (; i < in; ++i) small_pair(a, sm.Ls[my[i0 + i]]);
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
+      }
+      return;
+      }
+    }
     for (int j0 = 0; j0 < cnt; j0 += 32) {
       const int jn = min(32, cnt - j0);
       float a[PPT];
@@ -525,23 +605,54 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
           a[k] = fmaf(in ? 1.0f : 0.0f, pow25(fmaf(-r2, q.z, q.w)), a[k]);
         }
       };
-      for (int j0 = 0; j0 < cnt; j0 += 32) {
-        const int jn = min(32, cnt - j0);
+      if (cnt < kCompactMin) {
+        for (int j0 = 0; j0 < cnt; j0 += 32) {
+          const int jn = min(32, cnt - j0);
+          float a[PPT];
+#pragma unroll
+          for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
+          int j = 0;
+          for (; j + 4 <= jn; j += 4) {
+            const int4 cc = *reinterpret_cast<const int4*>(&sm.Lmc[j0 + j]);
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = sm.Ls[kPoolN - 1 - (j0 + j + u)];
+            add(a, q[0], uniform(cc.x));
+            add(a, q[1], uniform(cc.y));
+            add(a, q[2], uniform(cc.z));
+            add(a, q[3], uniform(cc.w));
+          }
+          for (; j < jn; ++j) add(a, sm.Ls[kPoolN - 1 - (j0 + j)], uniform(sm.Lmc[j0 + j]));
+#pragma unroll
+          for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
+        }
+        return;
+      }
+      const int wn = compact(true, cnt);
+      const uint16_t* my = sm.wl[wave];
+      for (int i0 = 0; i0 < wn; i0 += 32) {
+        const int in = min(32, wn - i0);
         float a[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
-        int j = 0;
-        for (; j + 4 <= jn; j += 4) {
-          const int4 cc = *reinterpret_cast<const int4*>(&sm.Lmc[j0 + j]);
+        int i = 0;
+        for (; i + 4 <= in; i += 4) {
+          const uint2 ix = *reinterpret_cast<const uint2*>(my + i0 + i);
+          const uint32_t j[4] = {ix.x & 0xffffu, ix.x >> 16, ix.y & 0xffffu, ix.y >> 16};
           float4 q[4];
+          int c[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) q[u] = sm.Lm[j0 + j + u];
-          add(a, q[0], uniform(cc.x));
-          add(a, q[1], uniform(cc.y));
-          add(a, q[2], uniform(cc.z));
-          add(a, q[3], uniform(cc.w));
+          for (int u = 0; u < 4; ++u) {
+            q[u] = sm.Ls[kPoolN - 1 - int(j[u])];
+            c[u] = sm.Lmc[j[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) add(a, q[u], uniform(c[u]));
         }
-        for (; j < jn; ++j) add(a, sm.Lm[j0 + j], uniform(sm.Lmc[j0 + j]));
+        for (; i < in; ++i) {
+          const uint32_t j = my[i0 + i];
+          add(a, sm.Ls[kPoolN - 1 - int(j)], uniform(sm.Lmc[j]));
+        }
 #pragma unroll
         for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
       }
@@ -636,30 +747,36 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   const int ncols = cx1 >= cx0 ? cx1 - cx0 + 1 : 0;
   const int nrows = cy1 >= cy0 ? cy1 - cy0 + 1 : 0;
   const int nlays = cz1 >= cz0 ? cz1 - cz0 + 1 : 0;
-  const int64_t ncand = int64_t(ncols) * nrows * nlays;
-  const int64_t nrowz = ncols > 0 ? int64_t(nrows) * nlays : 0;
+  // the box lies inside the grid, whose cells have 32-bit keys: 32-bit
+  // counts and indices (one layer, as in reference mode, skips a division)
+  const uint32_t ncand = uint32_t(ncols) * uint32_t(nrows) * uint32_t(nlays);
+  const uint32_t nrowz = ncols > 0 ? uint32_t(nrows) * uint32_t(nlays) : 0u;
   const uint32_t nxny = uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
   const float sup_h2 = support * support * 1.00002f;
-  const bool rows_mode = g.sparse != 0 && ncand > int64_t(kBlock);
-  const int64_t nitems = rows_mode ? nrowz : ncand;
-  for (int64_t c0 = 0; c0 < nitems; c0 += kBlock) {
-    const int64_t ci = c0 + t;
+  const bool rows_mode = g.sparse != 0 && ncand > uint32_t(kBlock);
+  const bool one_layer = nlays == 1;
+  const uint64_t nitems = rows_mode ? nrowz : ncand;
+  for (uint64_t c0 = 0; c0 < nitems; c0 += kBlock) {
+    const bool in_box = c0 + uint64_t(t) < nitems;
+    const uint32_t ci = uint32_t(c0) + uint32_t(t);
     uint32_t cb = 0, cn = 0;
     int ccx = 0, ccy = 0, cfull = 0;
     if (rows_mode) {
-      if (ci < nrowz) {
-        ccy = cy0 + int(ci % nrows);
-        const uint32_t kb = uint32_t(ccy) * uint32_t(g.nc[0]) + uint32_t(cz0 + int(ci / nrows)) * nxny;
+      if (in_box) {
+        const uint32_t lay = one_layer ? 0u : ci / uint32_t(nrows);
+        ccy = cy0 + int(ci - lay * uint32_t(nrows));
+        const uint32_t kb = uint32_t(ccy) * uint32_t(g.nc[0]) + uint32_t(cz0 + int(lay)) * nxny;
         cb = g.start[kb + uint32_t(cx0)];
         cn = g.start[kb + uint32_t(cx1) + 1] - cb;
         ccx = int(kb);  // stored as the row's base key
       }
-    } else if (ci < ncand) {
-      const int col_c = int(ci % ncols);
-      const int64_t rest = ci / ncols;
+    } else if (in_box) {
+      const uint32_t rest = ci / uint32_t(ncols);
+      const int col_c = int(ci - rest * uint32_t(ncols));
+      const uint32_t lay = one_layer ? 0u : rest / uint32_t(nrows);
       ccx = cx0 + col_c;
-      ccy = cy0 + int(rest % nrows);
-      const int ccz = cz0 + int(rest / nrows);
+      ccy = cy0 + int(rest - lay * uint32_t(nrows));
+      const int ccz = cz0 + int(lay);
       const uint32_t key = uint32_t(ccx) + uint32_t(ccy) * uint32_t(g.nc[0]) + uint32_t(ccz) * nxny;
       cb = g.start[key];
       const uint32_t ce = g.start[key + 1];
@@ -732,7 +849,8 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
             cf = pf;
           }
         }
-        const float4 q = g.part[sidx];  // (x, y, coef, 1/h^2)
+        // (x, y, coef, 1/h^2); the quartic's z is c = coef^0.4
+        const float4 q = g.part[sidx];
         if (ADJ || q.z != 0.0f) {
           const float dx = fmaxf(fmaxf(tbx0 - q.x, q.x - tbx1), 0.0f);
           const float dy = fmaxf(fmaxf(tby0 - q.y, q.y - tby1), 0.0f);
@@ -744,13 +862,14 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
           if constexpr (COUNT) {
             e = make_float4(q.x, q.y, g.hs[sidx], 0.0f);
           } else if constexpr (ADJ) {
-            e = make_float4(q.x, q.y, q.w, q.z);  // (x, y, 1/h^2, coef)
+            // (x, y, 1/h^2, coef)
+            e = make_float4(q.x, q.y, q.w, KERNEL == 2 ? q.z : pow25_pos(q.z));
             adj_idx = sidx;
           } else if constexpr (KERNEL == 2) {
             e = q;
           } else {
-            const float c = powf(q.z, 0.4f);  // coef^0.4
-            const float ci2 = c * q.w;        // c / h^2
+            const float c = q.z;        // coef^0.4
+            const float ci2 = c * q.w;  // c / h^2
             big = full && big_r2 * q.w <= 1.0f;
             if (big) {
               const float xr = q.x - tcx, yr = q.y - tcy;
@@ -806,7 +925,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       if constexpr (REF) {
         if (keep && !full) {
           const int pos = basem + prefix_count(bm);
-          sm.Lm[pos] = e;
+          sm.Ls[kPool - 1 - pos] = e;
           sm.Lmc[pos] = cinf;
         }
       }

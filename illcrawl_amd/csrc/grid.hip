@@ -297,7 +297,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
 
 // Sorted SoA: part[s] = (x, y, coef, 1/h^2) of particle perm[s], hs[s] = h.
 // coef = w * C / h^2 with C the kernel's normalisation, so that the kernel
-// value is coef * shape(r^2 / h^2).
+// value is coef * shape(r^2 / h^2).  For the projected quartic (shape
+// s^2.5, s = 1 - r^2/h^2) the z slot holds c = coef^0.4 instead: the gather
+// evaluates coef s^2.5 as (c s)^2.5, and the power belongs to the particle,
+// not to each tile that lists it.
 // The same pass derives the per-cell particle counts and max h
 // (grid3d_determine_cells_begin/_end and grid3d_determine_max_per_cell,
 // particle_grid.cl:114-216): keys are sorted, so the particles of a cell form
@@ -308,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__
                                                        const float* __restrict__ h,
                                                        const uint32_t* __restrict__ perm,
                                                        const uint32_t* __restrict__ skeys,
-                                                       uint32_t n, float norm,
+                                                       uint32_t n, float norm, bool pow04,
                                                        float4* __restrict__ part,
                                                        float* __restrict__ hs,
                                                        uint32_t* __restrict__ count,
@@ -324,7 +327,8 @@ __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__
     hh = h ? h[i] : q.z;
     key = skeys[s];
     const float ih2 = 1.0f / (hh * hh);
-    part[s] = make_float4(q.x, q.y, q.w * (norm * ih2), ih2);
+    const float coef = q.w * (norm * ih2);
+    part[s] = make_float4(q.x, q.y, pow04 ? powf(coef, 0.4f) : coef, ih2);
     hs[s] = hh;
   }
   // run heads within the wave
@@ -485,11 +489,11 @@ void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits,
 }
 
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
-                      const uint32_t* skeys, uint32_t n, float norm, float4* part, float* hs,
+                      const uint32_t* skeys, uint32_t n, float norm, bool pow04, float4* part, float* hs,
                       uint32_t* count, uint32_t* maxh_bits, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_build_soa, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, perm,
-                     skeys, n, norm, part, hs, count, maxh_bits);
+                     skeys, n, norm, pow04, part, hs, count, maxh_bits);
   ILC_HIP(hipGetLastError());
 }
 

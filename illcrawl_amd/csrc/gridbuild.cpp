@@ -137,7 +137,8 @@ void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, 
   launch_fill_u32(sc.counts.p, 0u, ncells + 1, s);
   launch_fill_u32(gs.maxh_bits.p, 0u, ncells, s);
   gs.volumetric = false;
-  launch_build_soa(p4, h, gs.perm.p, gs.key.p, uint32_t(n), kernel_norm(kernel), gs.part.p,
+  launch_build_soa(p4, h, gs.perm.p, gs.key.p, uint32_t(n), kernel_norm(kernel),
+                   kernel != ILC_KERNEL_CUBIC, gs.part.p,
                    gs.hs.p, sc.counts.p, gs.maxh_bits.p, s);
   launch_exclusive_scan(sc.counts.p, gs.start.p, ncells + 1, sc.temp.p, s);
   gs.gp = g;

@@ -21,10 +21,11 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32
 uint64_t radix_temp_words(uint64_t n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
                        hipStream_t s);
-// coef = w norm / h^2; h NULL: each particle's h is the z of its p4
+// coef = w norm / h^2 (pow04: coef^0.4, the projected quartic's form); h
+// NULL: each particle's h is the z of its p4
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
-                      const uint32_t* skeys, uint32_t n, float norm, float4* part, float* hs,
-                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s);
+                      const uint32_t* skeys, uint32_t n, float norm, bool pow04, float4* part,
+                      float* hs, uint32_t* count, uint32_t* maxh_bits, hipStream_t s);
 // Reorder (key, perm) pairs, n of them, by descending h[perm] (top 16 bits,
 // stable); horder holds 2n words, temp radix_temp_words(n).
 void launch_h_order(const float* h, uint32_t* key, uint32_t* perm, uint32_t n, uint32_t* horder,
