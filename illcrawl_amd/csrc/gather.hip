@@ -28,6 +28,8 @@
 //   5. when a list is full, every thread evaluates the listed particles for
 //      its points from LDS (broadcast reads), FP32 over 32 particles, then
 //      into a per-point FP64 accumulator.
+// The small and masked lists share one LDS array from its two ends (as the
+// big list's two kinds do), which keeps the workgroup at 17 KB of LDS.
 //
 // Kernel arithmetic.  With c = coef^0.4, the quartic term coef*s^2.5,
 // s = 1 - r^2/h^2, equals (c*s)^2.5, and c*s is affine in the point's
@@ -155,9 +157,6 @@ struct adjoint_io {
 // the small and masked lists share one array from its two ends: before a
 // batch of kBlock candidates each holds at most kFlush - 1 entries
 constexpr int kPool = kCap + kFlush;
-// a flush compacts each wave's small / masked entries only for lists of at
-// least this length (shorter lists cost less to evaluate whole)
-constexpr int kCompactMin = kFlush;
 static_assert(2 * (kFlush - 1) + kBlock <= kPool, "pool capacity");
 
 template <bool REF, int KERNEL, bool ADJ = false>
@@ -167,11 +166,8 @@ struct gather_smem {
   // form) from the back, the m-th at Ls[kPool - 1 - m]
   float4 Ls[REF ? kPool : kCap];
   alignas(16) int Lmc[REF ? kCap : 1];  // masked: packed cell (cx | cy << 16)
-  // per wave, at a flush: the positions of the small (then masked) entries
-  // whose support reaches the wave's points
-  alignas(8) uint16_t wl[ADJ ? 1 : 4][kCap];
   uint32_t wcnt[2][16];
-  alignas(16) float redf[4][4];  // per wave: box of its valid points (x0, y0, x1, y1)
+  float redf[4][4];
   int redi[4][10];
   float tab[KERNEL == 2 ? kCubicTable : 1];
   float dtab[KERNEL == 2 && ADJ ? kCubicTable : 1];
@@ -403,36 +399,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   cz1 = uniform(cz1);
   const bool allz_u = uniform(r_allz) != 0;
   const float cull2 = COUNT ? support * support * 1.0001f : support * support;
-  // At a flush each wave evaluates only the small or masked entries whose
-  // support disk (with a slack far beyond FP32 rounding) meets the box of its
-  // own valid points: at its points the others add exactly 0.  compact()
-  // writes the list positions of those entries, in list order, to wl[wave]
-  // (small entries: position j at Ls[j]; masked: at Ls[kPool - 1 - j]).
-  const float cullw = support * support * 1.0001f;
   constexpr int kPoolN = REF ? kPool : kCap;
-  auto compact = [&](bool masked, int cnt) -> int {
-    const float4 bx = *reinterpret_cast<const float4*>(sm.redf[wave]);
-    int wn = 0;
-    for (int j0 = 0; j0 < cnt; j0 += 64) {
-      const int j = j0 + lane;
-      bool mine = false;
-      if (j < cnt) {
-        // (x, y, c/h^2, c), or the cubic's (x, y, coef, 1/h^2)
-        const float4 e = sm.Ls[masked ? kPoolN - 1 - j : j];
-        const float dx = fmaxf(fmaxf(bx.x - e.x, e.x - bx.z), 0.0f);
-        const float dy = fmaxf(fmaxf(bx.y - e.y, e.y - bx.w), 0.0f);
-        const float d2 = fmaf(dy, dy, dx * dx);
-        mine = KERNEL == 2 ? d2 * e.w < cullw : d2 * e.z < cullw * e.w;
-      }
-      const uint64_t bal = __ballot(mine);
-      if (mine) sm.wl[wave][wn + int(prefix_count(bal))] = uint16_t(j);
-      wn += __popcll(bal);
-    }
-    // the wave reads what its lanes wrote
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    return uniform(wn);
-  };
 
   // list sizes (identical in every thread).  The big list holds the
   // particles whose support covers the whole tile (argument positive at every
@@ -479,46 +446,8 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       run(kCap - cntp, cntp, std::true_type{});
     }
   };
-  auto small_pair = [&](float* a, const float4& q) {
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-      const float dx = q.x - ex[k], dy = q.y - ey[k];
-      const float r2 = fmaf(dy, dy, dx * dx);
-      if constexpr (KERNEL == 2)
-        a[k] = fmaf(q.z, cubic_shape(r2 * q.w, sm.tab), a[k]);
-      else
-        a[k] += pow25(fmaf(-r2, q.z, q.w));
-    }
-  };
   auto eval_small = [&](int cnt) {
     cnt = uniform(cnt);
-    if constexpr (!COUNT && !ADJ) {
-      if (cnt >= kCompactMin) {
-      const int wn = compact(false, cnt);
-      const uint16_t* my = sm.wl[wave];
-      for (int i0 = 0; i0 < wn; i0 += 32) {
-        const int in = min(32, wn - i0);
-        float a[PPT];
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
-        int i = 0;
-        for (; i + 4 <= in; i += 4) {
-          const uint2 ix = *reinterpret_cast<const uint2*>(my + i0 + i);
-          float4 q[4];
-          q[0] = sm.Ls[ix.x & 0xffffu];
-          q[1] = sm.Ls[ix.x >> 16];
-          q[2] = sm.Ls[ix.y & 0xffffu];
-          q[3] = sm.Ls[ix.y >> 16];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) small_pair(a, q[u]);
-        }
-        for (; i < in; ++i) small_pair(a, sm.Ls[my[i0 + i]]);
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
-      }
-      return;
-      }
-    }
     for (int j0 = 0; j0 < cnt; j0 += 32) {
       const int jn = min(32, cnt - j0);
       float a[PPT];
@@ -605,54 +534,23 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
           a[k] = fmaf(in ? 1.0f : 0.0f, pow25(fmaf(-r2, q.z, q.w)), a[k]);
         }
       };
-      if (cnt < kCompactMin) {
-        for (int j0 = 0; j0 < cnt; j0 += 32) {
-          const int jn = min(32, cnt - j0);
-          float a[PPT];
-#pragma unroll
-          for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
-          int j = 0;
-          for (; j + 4 <= jn; j += 4) {
-            const int4 cc = *reinterpret_cast<const int4*>(&sm.Lmc[j0 + j]);
-            float4 q[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = sm.Ls[kPoolN - 1 - (j0 + j + u)];
-            add(a, q[0], uniform(cc.x));
-            add(a, q[1], uniform(cc.y));
-            add(a, q[2], uniform(cc.z));
-            add(a, q[3], uniform(cc.w));
-          }
-          for (; j < jn; ++j) add(a, sm.Ls[kPoolN - 1 - (j0 + j)], uniform(sm.Lmc[j0 + j]));
-#pragma unroll
-          for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
-        }
-        return;
-      }
-      const int wn = compact(true, cnt);
-      const uint16_t* my = sm.wl[wave];
-      for (int i0 = 0; i0 < wn; i0 += 32) {
-        const int in = min(32, wn - i0);
+      for (int j0 = 0; j0 < cnt; j0 += 32) {
+        const int jn = min(32, cnt - j0);
         float a[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) a[k] = 0.0f;
-        int i = 0;
-        for (; i + 4 <= in; i += 4) {
-          const uint2 ix = *reinterpret_cast<const uint2*>(my + i0 + i);
-          const uint32_t j[4] = {ix.x & 0xffffu, ix.x >> 16, ix.y & 0xffffu, ix.y >> 16};
+        int j = 0;
+        for (; j + 4 <= jn; j += 4) {
+          const int4 cc = *reinterpret_cast<const int4*>(&sm.Lmc[j0 + j]);
           float4 q[4];
-          int c[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            q[u] = sm.Ls[kPoolN - 1 - int(j[u])];
-            c[u] = sm.Lmc[j[u]];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) add(a, q[u], uniform(c[u]));
+          for (int u = 0; u < 4; ++u) q[u] = sm.Ls[kPoolN - 1 - (j0 + j + u)];
+          add(a, q[0], uniform(cc.x));
+          add(a, q[1], uniform(cc.y));
+          add(a, q[2], uniform(cc.z));
+          add(a, q[3], uniform(cc.w));
         }
-        for (; i < in; ++i) {
-          const uint32_t j = my[i0 + i];
-          add(a, sm.Ls[kPoolN - 1 - int(j)], uniform(sm.Lmc[j]));
-        }
+        for (; j < jn; ++j) add(a, sm.Ls[kPoolN - 1 - (j0 + j)], uniform(sm.Lmc[j0 + j]));
 #pragma unroll
         for (int k = 0; k < PPT; ++k) a64[k] += double(a[k]);
       }
