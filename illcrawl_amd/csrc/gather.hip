@@ -871,7 +871,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
 }
 
 template <int TW, int PPT, int OP, int KERNEL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1 ? 7 : (PPT <= 2 ? 5 : 4), 8))) void k_gather(grid_view g, point_view p, uint32_t tiles_x,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1 ? 8 : (PPT <= 2 ? 5 : 4), 8))) void k_gather(grid_view g, point_view p, uint32_t tiles_x,
                                                    uint32_t ntiles, double* __restrict__ acc_out,
                                                    unsigned long long* __restrict__ counts,
                                                    const float* __restrict__ cubic_table,
@@ -897,7 +897,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1
 // workgroup drains its XCD's queue, then helps with the others; every queue
 // counter only grows, so every workgroup leaves once all queues are empty.
 template <int TW, int PPT, int OP, int KERNEL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1 ? 7 : (PPT <= 2 ? 5 : 4), 8))) void k_gather_batch(const view_desc* __restrict__ views,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1 ? 8 : (PPT <= 2 ? 5 : 4), 8))) void k_gather_batch(const view_desc* __restrict__ views,
                                                          const uint2* __restrict__ items,
                                                          queue_segs segs,
                                                          uint32_t* __restrict__ counters,
