@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -435,8 +436,13 @@ void plan(ilc_batch* b, hipStream_t s) {
     ntot += b->wtotal_host.p[v];
     ncells += vs.gp.ncells();
   }
-  // shared build when every index fits in 31 bits
-  b->shared = ntot > 0 && ntot < (uint64_t(1) << 31) && ncells < (uint64_t(1) << 31);
+  // shared build when every index fits in 31 bits.  ILC_BATCH_SHARED_MAX_ROWS
+  // lowers the row limit, so that tests reach the per-view builds (taken
+  // above 2^31 rows) with small snapshots.
+  uint64_t max_rows = uint64_t(1) << 31;
+  if (const char* e = std::getenv("ILC_BATCH_SHARED_MAX_ROWS"))
+    max_rows = std::min<uint64_t>(max_rows, std::strtoull(e, nullptr, 10));
+  b->shared = ntot > 0 && ntot < max_rows && ncells < (uint64_t(1) << 31);
   b->sh_ntot = ntot;
   b->sh_ncells = ncells;
   sig.push_back(b->shared ? ntot : 0);
