@@ -147,6 +147,51 @@ def cpu_baseline(snap, view, per_point_counts, seconds=15.0, kernel="quartic_ref
                        f"build ({t_grid:.2f} s, one thread) not included")
 
 
+def clip_parity(snap, view, img_f64, n=240, seed=7):
+    """Line-of-sight clip: the oracle's FP64 quadrature of the clipped 3-D
+    quartic over every particle (orc_gather_clip) at a random pixel sample
+    plus the maximum; its timing is the CPU baseline (no reference
+    algorithm exists for the clip)."""
+    import numpy as np
+    import oracle
+    from illcrawl_amd import synth
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count())
+    oracle.set_num_threads(threads)
+    cam = oracle.Camera(view["position"], view["look_at"], view["roll"], view["width"],
+                        view["nx"], view["ny"])
+    if view.get("quantity", 0) == 1:
+        tab, x0, dx = synth.gaunt_table_1d()
+        w = oracle.xray_bolometric(snap.rho, snap.temp, snap.xe, tab, x0, dx)
+    else:
+        w = snap.mass
+    pz = cam.project_particles_depth(snap.particles4(w))
+    pts = cam.pixel_points()
+    img = img_f64.reshape(-1)
+    rng = np.random.default_rng(seed)
+    idx = np.unique(np.concatenate([rng.choice(img.size, n, replace=False),
+                                    [int(np.argmax(img))]]))
+    t0 = time.perf_counter()
+    ref = oracle.gather_clip(pz, snap.hsml, pts[idx], view["depth"], 0)
+    elapsed = time.perf_counter() - t0
+    m = ref > 1e-6 * ref.max()
+    linf = float(np.max(np.abs(img[idx][m] - ref[m]) / ref[m]))
+    s = float(abs(img[idx].sum() - ref.sum()) / ref.sum())
+    # pairs inside the support at the sampled pixels with w != 0 after the
+    # depth cut (the eval unit; the clip keeps the particles whose support
+    # overlaps the depth range)
+    p4 = pz.copy()
+    over = (pz[:, 2] + snap.hsml >= 0) & (pz[:, 2] - snap.hsml <= np.float32(view["depth"]))
+    p4[~over, 3] = 0.0
+    p4[:, 2] = 0.0
+    _, cnt = oracle.gather_exact(p4, snap.hsml, pts[idx], with_counts=True)
+    evals = int(cnt.sum())
+    cpu = dict(value=evals / elapsed, unit="evals/s", cores=threads, kind="port",
+               sample=f"{idx.size} random pixels of one {view['nx']}x{view['ny']} slab "
+                      f"({elapsed:.1f} s of the oracle's FP64 clipped quadrature over every "
+                      f"particle, {threads} OpenMP threads)")
+    return dict(pixels=int(idx.size), linf_vs_oracle=linf, sum_vs_oracle=s), cpu
+
+
 def single_view_counts(snap, view, kernel="quartic_ref"):
     """Per-pixel exact eval counts of one view (single-view backend)."""
     import torch
@@ -534,6 +579,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--clip", action="store_true",
+                    help="line-of-sight kernel clip of each view's depth range instead of the "
+                         "reference's centre mask (exact mode; no plain leg)")
     ap.add_argument("--breakdown", action="store_true",
                     help="time every resolution group alone, batched and plain (diagnostic)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -558,6 +606,11 @@ def main():
     if kernel != "quartic_ref" and args.mode == "reference":
         log(f"[bench] the {kernel} kernel has no reference search: exact mode")
         args.mode = "exact"
+    if args.clip:
+        if args.mode == "reference":
+            log("[bench] the line-of-sight clip has no reference search: exact mode")
+        args.mode = "exact"
+        args.plain_steps = 0
     if args.workload == "v1":
         return volumetric_main(args)
     if args.adjoint:
@@ -570,6 +623,9 @@ def main():
     r.set_particles(snap.pos, snap.mass, snap.hsml, snap.rho, snap.temp, snap.xe)
     tab, x0, dx = synth.gaunt_table_1d()
     r.set_gaunt_table(tab, x0, dx)
+    if args.clip:
+        r.set_depth_clip(True)
+        desc = dict(desc, depth_cut="line-of-sight kernel clip")
     r.set_views(views)
     if args.tile:
         w, h = (int(x) for x in args.tile.split("x"))
@@ -672,6 +728,16 @@ def main():
         line["copy_calibration_bytes"] = nb
         del src, dst
     big = max(range(len(views)), key=lambda k: views[k]["nx"] * views[k]["ny"])
+    if args.clip:
+        big = len(views) // 2
+        if not args.no_parity:
+            log("[bench] sampled parity vs the oracle's clipped quadrature ...")
+            line["parity"], cpu = clip_parity(snap, views[big], r.image_f64(big).cpu().numpy())
+            line["parity"]["view"] = big
+            if not args.no_cpu_baseline:
+                line["cpu_baseline"] = cpu
+        print(json.dumps(line), flush=True)
+        return
     if not args.no_parity:
         log("[bench] sampled parity vs oracle ...")
         img = r.image_f64(big).cpu().numpy()

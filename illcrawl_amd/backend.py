@@ -692,6 +692,12 @@ class BatchRenderer:
     def set_tile(self, w, h):
         N.call("ilc_batch_set_tile", self._h, int(w), int(h))
 
+    def set_depth_clip(self, clip=True):
+        """Line-of-sight kernel clip of every view's depth range instead of the
+        reference's centre mask (exact mode, quartic kernels, persistent
+        renders)."""
+        N.call("ilc_batch_set_depth_clip", self._h, 1 if clip else 0)
+
     def last_gather_ms(self):
         ms = ctypes.c_float()
         N.call("ilc_batch_last_gather_ms", self._h, ctypes.byref(ms))

@@ -152,6 +152,11 @@ struct ilc_batch {
   // SoA pass; a view's grid is a slice
   bool shared = false;
   uint64_t sh_ntot = 0, sh_ncells = 0;
+  // line-of-sight kernel clip of every view's depth range (exact mode,
+  // quartic kernels, persistent renders): per kept row and per sorted row,
+  // the range relative to the particle in units of h
+  int clip = 0;
+  dbuf<float2> sh_zc, sh_zcs;
   dbuf<uint32_t> sh_offs, sh_gkey, sh_orig, sh_key, sh_perm, sh_sk, sh_counts, sh_start, sh_maxh,
       sh_temp, sh_cbase;
   pinned<uint32_t> sh_cbase_host;
@@ -392,6 +397,7 @@ void plan(ilc_batch* b, hipStream_t s) {
     } else {
       vs.win = window_f{0.f, 0.f, 0.f, 0.f, 0, 0};  // keeps every particle
     }
+    vs.win.clip = b->clip;
   }
   {
     // kept counts per window tile of every view (unfiltered views keep
@@ -446,6 +452,7 @@ void plan(ilc_batch* b, hipStream_t s) {
   b->sh_ntot = ntot;
   b->sh_ncells = ncells;
   sig.push_back(b->shared ? ntot : 0);
+  sig.push_back(uint64_t(b->clip));
   sig.push_back(uint64_t(b->ts.tw) << 32 | uint64_t(b->ts.ppt));
   // work items: views with the coarsest pixels first (their tiles cover the
   // most area, so over dense regions they stream the most particles and run
@@ -490,8 +497,9 @@ void plan(ilc_batch* b, hipStream_t s) {
   if (!same) b->graph_valid = false;
   b->plan_sig = sig;
   b->planned = true;
-  b->grid_blocks = gather_batch_grid(b->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT,
-                                     b->kernel, b->ts);
+  b->grid_blocks = gather_batch_grid(
+      b->clip ? OP_CLIP : (b->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT), b->kernel,
+      b->ts);
 }
 
 // An event for timing the gather launch(es) of an eager render.
@@ -593,10 +601,15 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   // per-cell max h starts at zero: the scatter raises it for the rows its
   // screen test leaves out, the SoA pass for the kept rows
   launch_fill_u32(b->sh_maxh.p, 0u, nc, s);
+  if (b->clip) {
+    b->sh_zc.ensure(ntot);
+    b->sh_zcs.ensure(ntot);
+  }
   launch_multi_window_scatter(b->ps, any_xray ? b->sh_wx.p : nullptr, b->bdesc_dev.p,
                               b->fviews_dev.p, static_cast<int>(nv), b->sh_offs.p, b->sh_p4c.p,
                               b->sh_sk.p, b->sh_perm.p, b->sh_gkey.p, b->sh_orig.p,
-                              b->counter.p + 1, b->sh_cbase.p, b->sh_maxh.p, s);
+                              b->counter.p + 1, b->sh_cbase.p, b->sh_maxh.p,
+                              b->clip ? b->sh_zc.p : nullptr, s);
   // stable sorts by descending h, then by global cell key (view, then cell):
   // a view's rows end up in one range, by cell, biggest h first, ties in
   // input order
@@ -610,12 +623,14 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   launch_fill_u32(b->sh_counts.p, 0u, nc + 1, s);
   launch_build_soa(b->sh_p4c.p, nullptr, b->sh_perm.p, b->sh_sk.p, n32, kernel_norm(b->kernel),
                    b->kernel != ILC_KERNEL_CUBIC,
-                   b->sh_part.p, b->sh_hs.p, b->sh_counts.p, b->sh_maxh.p, s);
+                   b->sh_part.p, b->sh_hs.p, b->sh_counts.p, b->sh_maxh.p, s,
+                   b->clip ? b->sh_zc.p : nullptr, b->clip ? b->sh_zcs.p : nullptr);
   launch_exclusive_scan(b->sh_counts.p, b->sh_start.p, nc + 1, b->sh_temp.p, s);
   for (size_t v = 0; v < nv; ++v) {
     view_state& vs = *b->views[v];
     grid_view g{};
     g.part = b->sh_part.p;
+    g.clip = b->clip ? b->sh_zcs.p : nullptr;
     g.key = b->sh_key.p;
     g.hs = b->sh_hs.p;
     g.start = b->sh_start.p + vs.cbase;
@@ -677,7 +692,9 @@ void ensure_lanes(ilc_batch* b) {
 void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = false) {
   const size_t nv = b->views.size();
   b->tev_used = 0;
-  const int op = b->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT;
+  const int op = b->clip ? OP_CLIP : (b->mode == ILC_MODE_REFERENCE ? OP_REFERENCE : OP_EXACT);
+  if (b->clip && (!persistent || !b->shared))
+    fail(ILC_EINVAL, "the line-of-sight clip needs a persistent render with the shared build");
   launch_fill_u32(b->counter.p, 0u, 2, s);
   b->qcounters.ensure(size_t(kQueues) * kQueueStride);
   launch_fill_u32(b->qcounters.p, 0u, uint64_t(kQueues) * kQueueStride, s);
@@ -977,6 +994,7 @@ ilc_status ilc_batch_backward(ilc_batch* b, const float* const* dl_dimage_dev, f
     ILC_REQUIRE(dl_dimage_dev && dl_dw_dev && dl_dh_dev && dl_dpos3_dev, "null argument");
     if (b->mode != ILC_MODE_EXACT)
       fail(ILC_EINVAL, "backward needs exact mode (the reference-mode search box is not smooth)");
+    if (b->clip) fail(ILC_EINVAL, "backward of the line-of-sight clip is not provided");
     if (!b->planned) fail(ILC_ENOTREADY, "render before backward");
     device_guard dg(b->device);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -1029,6 +1047,20 @@ ilc_status ilc_batch_last_gather_ms(const ilc_batch* b, float* ms) {
   return guarded([&] {
     ILC_REQUIRE(b && ms, "null argument");
     *ms = b->last_gather_ms;
+  });
+}
+
+ilc_status ilc_batch_set_depth_clip(ilc_batch* b, int32_t clip) {
+  return guarded([&] {
+    ILC_REQUIRE(b, "null batch");
+    if (clip) {
+      ILC_REQUIRE(b->mode == ILC_MODE_EXACT, "the line-of-sight clip needs exact mode");
+      ILC_REQUIRE(b->kernel != ILC_KERNEL_CUBIC, "the line-of-sight clip needs a quartic kernel");
+    }
+    b->clip = clip ? 1 : 0;
+    b->graph_valid = false;
+    b->planned = false;
+    b->ran_eagerly = false;
   });
 }
 

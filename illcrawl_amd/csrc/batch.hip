@@ -347,6 +347,13 @@ __device__ inline bool in_window(const float* pos3, const view_proj& vp, const w
   return keep;
 }
 
+// the depth cut of a row at camera depth az: the reference's centre mask
+// (projection.cl:80-91), or with the line-of-sight clip, its support
+// [az - h, az + h] overlapping [0, depth]
+__device__ inline bool depth_keep(float az, float h, float depth, int clip) {
+  return clip ? !(az + h < 0.0f || az - h > depth) : !(az < 0.0f || az > depth);
+}
+
 // the support disk (radius^2 = cull2 h^2) of a particle at (ax, ay) meets
 // the screen rectangle: the gather's tile cull with the screen for the tile
 // box (every tile box lies inside the rectangle, so a particle any tile keeps
@@ -368,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
   __shared__ uint32_t wc[2][8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
-  float px[kWinItems], py[kWinItems], pz[kWinItems], ih2[kWinItems];
+  float px[kWinItems], py[kWinItems], pz[kWinItems], ih2[kWinItems], hr[kWinItems];
   bool in[kWinItems];
 #pragma unroll
   for (int r = 0; r < kWinItems; ++r) {
@@ -378,6 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
     py[r] = in[r] ? pos3[3 * i + 1] : 0.0f;
     pz[r] = in[r] ? pos3[3 * i + 2] : 0.0f;
     const float h = in[r] ? hsml[i] : 1.0f;
+    hr[r] = h;
     ih2[r] = 1.0f / (h * h);  // as the grid's SoA holds it
   }
   for (int k = 0; k < nviews; ++k) {
@@ -402,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
         }
         if (win.use_depth) {
           const float az = dot3(vp.m + 6, vx, vy, vz);
-          keep = keep && !(az < 0.0f || az > vp.depth);
+          keep = keep && depth_keep(az, hr[r], vp.depth, win.clip);
         }
         cp += __popcll(__ballot(keep));
         if (win.use_screen) keep = keep && reaches_screen(win, ax, ay, ih2[r]);
@@ -494,7 +502,8 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     const uint32_t* __restrict__ offs, float4* __restrict__ p4c, uint32_t* __restrict__ hqkey,
     uint32_t* __restrict__ hqval, uint32_t* __restrict__ gkey, uint32_t* __restrict__ orig,
     uint32_t* __restrict__ err,
-    const uint32_t* __restrict__ cbase, uint32_t* __restrict__ maxh_bits) {
+    const uint32_t* __restrict__ cbase, uint32_t* __restrict__ maxh_bits,
+    float2* __restrict__ zc) {
 #pragma clang fp contract(off)
   __shared__ __align__(16) uint32_t wc[2][kWinItems][4];
   // the tile's positions, read once (SoA)
@@ -542,7 +551,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     }
     if (win.use_depth) {
       const float az = dot3(vp.m + 6, vx, vy, vz);
-      keep = keep && !(az < 0.0f || az > vp.depth);
+      keep = keep && depth_keep(az, ps.hsml[i], vp.depth, win.clip);
     }
     if (win.use_screen && keep) {
       const float h = ps.hsml[i];
@@ -601,10 +610,11 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
                            __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
                                                      __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
         float w = vp.quantity == 1 ? wx[i] : static_cast<float>(vp.scale * double(ps.mass[i]));
-        if (az < 0.0f || az > vp.depth) w = 0.0f;
         // h rides in the z slot, so the SoA pass reads one line per row; the
         // descending-h sort starts from (hq key, row)
         const float hh = ps.hsml[i];
+        if (!depth_keep(az, hh, vp.depth, win.clip)) w = 0.0f;
+        if (zc) zc[j] = make_float2((0.0f - az) / hh, (vp.depth - az) / hh);
         p4c[j] = make_float4(ax, ay, hh, w);
         hqkey[j] = hq_key(hh);
         hqval[j] = j;
@@ -772,13 +782,14 @@ void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  int nviews, const uint32_t* offs, float4* p4c, uint32_t* hqkey,
                                  uint32_t* hqval,
                                  uint32_t* gkey, uint32_t* orig, uint32_t* err,
-                                 const uint32_t* cbase, uint32_t* maxh_bits, hipStream_t s) {
+                                 const uint32_t* cbase, uint32_t* maxh_bits, float2* zc,
+                                 hipStream_t s) {
   const uint32_t nb = window_tiles(ps.n);
   if (!nb || nviews <= 0) return;
   ILC_REQUIRE(cbase != nullptr, "the scatter needs the views' first cells");
   hipLaunchKernelGGL(k_multi_window_scatter, dim3(nb), dim3(kBlock), 0, s, ps, wx, vd_dev,
                      views_dev, nviews, nb, offs, p4c, hqkey, hqval, gkey, orig, err, cbase,
-                     maxh_bits);
+                     maxh_bits, zc);
   ILC_HIP(hipGetLastError());
 }
 

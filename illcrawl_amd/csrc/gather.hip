@@ -132,6 +132,25 @@ __device__ inline float pow25(float cs) {
 // The same for an argument known to be positive.
 __device__ inline float pow25_pos(float cs) { return (cs * cs) * __builtin_amdgcn_sqrtf(cs); }
 
+// The projected quartic clipped along the line of sight.  The 3-D kernel
+// along a line at projected distance r from the particle is proportional to
+// (a^2 - z^2)^2 on |z| < a = h s^(1/2), s = 1 - r^2/h^2; its integral over
+// the depth range [z0, z1] relative to the particle is the full column's
+// times (15/16) (G(u1) - G(u0)), G(u) = u - 2u^3/3 + u^5/5, u = z / a clamped
+// to [-1, 1].  With v = c s (the argument of pow25) and t = (z / h) c^(1/2),
+// u = t / v^(1/2).  tc: (t0, t1) of the range's two ends.
+__device__ inline float clip_pow(float v, float2 tc) {
+  v = fmaxf(v, 0.0f);
+  const float sv = __builtin_amdgcn_sqrtf(v);
+  const float iv = v > 0.0f ? __builtin_amdgcn_rcpf(sv) : 0.0f;
+  const float u0 = __builtin_amdgcn_fmed3f(tc.x * iv, -1.0f, 1.0f);
+  const float u1 = __builtin_amdgcn_fmed3f(tc.y * iv, -1.0f, 1.0f);
+  const float q0 = u0 * u0, q1 = u1 * u1;
+  const float g0 = u0 * fmaf(q0, fmaf(q0, 0.2f, -2.0f / 3.0f), 1.0f);
+  const float g1 = u1 * fmaf(q1, fmaf(q1, 0.2f, -2.0f / 3.0f), 1.0f);
+  return ((v * v) * sv) * ((g1 - g0) * 0.9375f);
+}
+
 // Tabulated projected cubic spline on u = r^2/h^2 in [0, 4), linear.
 __device__ inline float cubic_shape(float u, const float* tab) {
   const float x = fminf(u * (float(kCubicTable - 1) / 4.0f), float(kCubicTable - 1));
@@ -159,13 +178,17 @@ struct adjoint_io {
 constexpr int kPool = kCap + kFlush;
 static_assert(2 * (kFlush - 1) + kBlock <= kPool, "pool capacity");
 
-template <bool REF, int KERNEL, bool ADJ = false>
+template <bool REF, int KERNEL, bool ADJ = false, bool CLIP = false>
 struct gather_smem {
   float4 Lb[KERNEL == 0 && !ADJ ? kCap : 1];  // big: expansion coefficients
   // small (position form) from the front; reference mode: masked (position
   // form) from the back, the m-th at Ls[kPool - 1 - m]
   float4 Ls[REF ? kPool : kCap];
   alignas(16) int Lmc[REF ? kCap : 1];  // masked: packed cell (cx | cy << 16)
+  // line-of-sight clip: per big / small entry, the depth range relative to
+  // the particle in units of h, times c^(1/2) (see clip_pow)
+  float2 Lbz[CLIP ? kCap : 1];
+  float2 Lsz[CLIP ? kCap : 1];
   uint32_t wcnt[2][16];
   float redf[4][4];
   int redi[4][10];
@@ -188,14 +211,16 @@ template <int TW, int PPT, int OP, int KERNEL>
 __device__ __attribute__((always_inline)) inline void gather_tile(
     const grid_view& g, const point_view& p, uint32_t tx, uint32_t ty, double* __restrict__ acc_out,
     unsigned long long* __restrict__ counts, float support,
-    gather_smem<OP == OP_REFERENCE, KERNEL, OP == OP_ADJOINT>& sm,
+    gather_smem<OP == OP_REFERENCE, KERNEL, OP == OP_ADJOINT, OP == OP_CLIP>& sm,
     const adjoint_io* adj = nullptr) {
   constexpr int ROWS = kBlock / TW;
   constexpr int TH = ROWS * PPT;
   constexpr bool REF = OP == OP_REFERENCE;
   constexpr bool COUNT = OP == OP_COUNT;
   constexpr bool ADJ = OP == OP_ADJOINT;
+  constexpr bool CLIP = OP == OP_CLIP;
   constexpr bool QUARTIC = KERNEL == 0 && !COUNT && !ADJ;
+  static_assert(!CLIP || KERNEL == 0, "the line-of-sight clip is the quartic's");
   static_assert(!ADJ || PPT * kBlock <= kBlock * 4, "adjoint tile table");
 
   const int t = threadIdx.x;
@@ -420,14 +445,21 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
           int j = 0;
           for (; j + 4 <= jn; j += 4) {
             float4 q[4];
+            float2 z[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = sm.Lb[base + j0 + j + u];
+            for (int u = 0; u < 4; ++u) {
+              q[u] = sm.Lb[base + j0 + j + u];
+              if constexpr (CLIP) z[u] = sm.Lbz[base + j0 + j + u];
+            }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
               for (int k = 0; k < PPT; ++k) {
                 const float v = fmaf(q[u].y, lx[k], fmaf(q[u].z, ly[k], fmaf(q[u].w, l2[k], q[u].x)));
-                a[k] += decltype(clamp)::value ? pow25(v) : pow25_pos(v);
+                if constexpr (CLIP)
+                  a[k] += clip_pow(v, z[u]);
+                else
+                  a[k] += decltype(clamp)::value ? pow25(v) : pow25_pos(v);
               }
           }
           for (; j < jn; ++j) {
@@ -435,7 +467,10 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
               const float v = fmaf(q.y, lx[k], fmaf(q.z, ly[k], fmaf(q.w, l2[k], q.x)));
-              a[k] += decltype(clamp)::value ? pow25(v) : pow25_pos(v);
+              if constexpr (CLIP)
+                a[k] += clip_pow(v, sm.Lbz[base + j0 + j]);
+              else
+                a[k] += decltype(clamp)::value ? pow25(v) : pow25_pos(v);
             }
           }
 #pragma unroll
@@ -459,6 +494,11 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
           float4 q[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) q[u] = sm.Ls[j0 + j + u];
+          float2 z[4];
+          if constexpr (CLIP) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) z[u] = sm.Lsz[j0 + j + u];
+          }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
 #pragma unroll
@@ -467,6 +507,8 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
               const float r2 = fmaf(dy, dy, dx * dx);
               if constexpr (KERNEL == 2)
                 a[k] = fmaf(q[u].z, cubic_shape(r2 * q[u].w, sm.tab), a[k]);
+              else if constexpr (CLIP)
+                a[k] += clip_pow(fmaf(-r2, q[u].z, q[u].w), z[u]);
               else
                 a[k] += pow25(fmaf(-r2, q[u].z, q[u].w));
             }
@@ -493,6 +535,8 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
             const float r2 = fmaf(dy, dy, dx * dx);
             if constexpr (KERNEL == 2)
               a[k] = fmaf(q.z, cubic_shape(r2 * q.w, sm.tab), a[k]);
+            else if constexpr (CLIP)
+              a[k] += clip_pow(fmaf(-r2, q.z, q.w), sm.Lsz[j0 + j]);
             else
               a[k] += pow25(fmaf(-r2, q.z, q.w));
           }
@@ -727,6 +771,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       bool keep = false, full = false, big = false, cover = false;
       int cinf = 0, cf = 0;
       uint32_t adj_idx = 0;
+      float2 tcl = make_float2(0.0f, 0.0f);
       if (v < tot) {
         // last cell whose offset is <= v (it has a nonzero count)
         int lo = 0;
@@ -754,6 +799,14 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
           const float dy = fmaxf(fmaxf(tby0 - q.y, q.y - tby1), 0.0f);
           const float d2 = fmaf(dy, dy, dx * dx);
           keep = d2 * q.w < cull2;
+        }
+        if constexpr (CLIP) {
+          // the depth range relative to the particle, in units of h: its
+          // support must overlap it
+          const float2 zc = g.clip[sidx];
+          keep = keep && zc.x < 1.0f && zc.y > -1.0f;
+          const float rc = __builtin_amdgcn_sqrtf(q.z);
+          tcl = make_float2(zc.x * rc, zc.y * rc);
         }
         full = keep && (!REF || cf);
         if (keep) {
@@ -812,13 +865,22 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       }
       if constexpr (QUARTIC) {
         // nb + nbp <= kFlush - 1 + kBlock <= kCap: the two ends never meet
-        if (big && cover) sm.Lb[baseb + prefix_count(bb)] = e;
-        if (big && !cover) sm.Lb[kCap - 1 - (basebp + int(prefix_count(bp)))] = e;
+        if (big && cover) {
+          const int pb = baseb + int(prefix_count(bb));
+          sm.Lb[pb] = e;
+          if constexpr (CLIP) sm.Lbz[pb] = tcl;
+        }
+        if (big && !cover) {
+          const int pc = kCap - 1 - (basebp + int(prefix_count(bp)));
+          sm.Lb[pc] = e;
+          if constexpr (CLIP) sm.Lbz[pc] = tcl;
+        }
       }
       if (full && !big) {
         const int pos = bases + prefix_count(bs);
         sm.Ls[pos] = e;
         if constexpr (ADJ) sm.Lidx[pos] = adj_idx;
+        if constexpr (CLIP) sm.Lsz[pos] = tcl;
       }
       if constexpr (REF) {
         if (keep && !full) {
@@ -876,7 +938,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1
                                                    unsigned long long* __restrict__ counts,
                                                    const float* __restrict__ cubic_table,
                                                    float support) {
-  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false> sm;
+  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false, OP == OP_CLIP> sm;
   if constexpr (KERNEL == 2) {
     for (int i = threadIdx.x; i < kCubicTable; i += kBlock) sm.tab[i] = cubic_table[i];
   }
@@ -903,7 +965,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1
                                                          uint32_t* __restrict__ counters,
                                                          const float* __restrict__ cubic_table,
                                                          float support) {
-  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false> sm;
+  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false, OP == OP_CLIP> sm;
   __shared__ uint32_t next;
   if constexpr (KERNEL == 2) {
     for (int i = threadIdx.x; i < kCubicTable; i += kBlock) sm.tab[i] = cubic_table[i];
@@ -1005,6 +1067,10 @@ template <template <int, int, int, int> class F, class... A>
 int dispatch(const tile_shape& ts, int op, int kernel, A&&... a) {
   auto by_op = [&](auto tw, auto ppt) -> int {
     constexpr int TW = decltype(tw)::value, PPT = decltype(ppt)::value;
+    if (op == OP_CLIP) {
+      if (kernel == ILC_KERNEL_CUBIC) fail(ILC_EINVAL, "the line-of-sight clip needs a quartic kernel");
+      return F<TW, PPT, OP_CLIP, 0>::run(a...);
+    }
     if (op == OP_REFERENCE) {
       if (kernel == ILC_KERNEL_CUBIC) fail(ILC_EINVAL, "reference mode uses the quartic kernel");
       return F<TW, PPT, OP_REFERENCE, 0>::run(a...);
@@ -1093,6 +1159,7 @@ void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,
   }
   if (kernel == ILC_KERNEL_CUBIC && !cubic_table)
     fail(ILC_EINVAL, "cubic gather without its table");
+  if (op == OP_CLIP) ILC_REQUIRE(g.clip != nullptr, "clip gather without clip parameters");
   dispatch<plain_launcher>(ts, op, kernel, g, p, acc, counts, cubic_table, support, s);
 }
 

@@ -315,7 +315,9 @@ __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__
                                                        float4* __restrict__ part,
                                                        float* __restrict__ hs,
                                                        uint32_t* __restrict__ count,
-                                                       uint32_t* __restrict__ maxh_bits) {
+                                                       uint32_t* __restrict__ maxh_bits,
+                                                       const float2* __restrict__ zc_in,
+                                                       float2* __restrict__ zc_out) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   const bool valid = s < n;
   const uint32_t lane = threadIdx.x & 63u;
@@ -330,6 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__
     const float coef = q.w * (norm * ih2);
     part[s] = make_float4(q.x, q.y, pow04 ? powf(coef, 0.4f) : coef, ih2);
     hs[s] = hh;
+    if (zc_out) zc_out[s] = zc_in[i];
   }
   // run heads within the wave
   const uint32_t prev = __shfl_up(key, 1, 64);
@@ -490,10 +493,12 @@ void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits,
 
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
                       const uint32_t* skeys, uint32_t n, float norm, bool pow04, float4* part, float* hs,
-                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s) {
+                      uint32_t* count, uint32_t* maxh_bits, hipStream_t s, const float2* zc_in,
+                      float2* zc_out) {
   if (!n) return;
+  ILC_REQUIRE((zc_in == nullptr) == (zc_out == nullptr), "clip parameters need both arrays");
   hipLaunchKernelGGL(k_build_soa, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, perm,
-                     skeys, n, norm, pow04, part, hs, count, maxh_bits);
+                     skeys, n, norm, pow04, part, hs, count, maxh_bits, zc_in, zc_out);
   ILC_HIP(hipGetLastError());
 }
 

@@ -71,7 +71,7 @@ void level_planes(const grid_params& g, const float cs[kLevels3], uint32_t poff[
 }
 
 grid_view grid_store::view() const {
-  grid_view g;
+  grid_view g{};
   g.part = part.p;
   g.key = key.p;
   g.hs = hs.p;

@@ -25,7 +25,8 @@ void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits,
 // NULL: each particle's h is the z of its p4
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
                       const uint32_t* skeys, uint32_t n, float norm, bool pow04, float4* part,
-                      float* hs, uint32_t* count, uint32_t* maxh_bits, hipStream_t s);
+                      float* hs, uint32_t* count, uint32_t* maxh_bits, hipStream_t s,
+                      const float2* zc_in = nullptr, float2* zc_out = nullptr);
 // Reorder (key, perm) pairs, n of them, by descending h[perm] (top 16 bits,
 // stable); horder holds 2n words, temp radix_temp_words(n).
 void launch_h_order(const float* h, uint32_t* key, uint32_t* perm, uint32_t n, uint32_t* horder,
@@ -93,6 +94,10 @@ struct window_f {
   // cull arithmetic) only raise their cell's max h
   int use_screen;
   float sx0, sx1, sy0, sy1, cull2;
+  // line-of-sight kernel clip: a row counts for the depth cut when its
+  // support [z' - h, z' + h] overlaps [0, depth] (instead of its centre lying
+  // inside), and the scatter writes its clip parameters
+  int clip;
 };
 // One view of a multi-view pass over the snapshot.
 struct view_build_desc {
@@ -128,11 +133,13 @@ void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* 
 // it orders rows by view, then cell) and the snapshot index.
 // wx: launch_particle_xray.  maxh_bits: the per-cell max h of the shared cell
 // arrays (rows kept out by the screen test go there only)
+// zc (NULL unless clip mode): per kept row, (0 - z') / h and (depth - z') / h
 void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  const view_build_desc* vd_dev, const int32_t* views_dev,
                                  int nviews, const uint32_t* offs, float4* p4c, uint32_t* hqkey,
                                  uint32_t* hqval, uint32_t* gkey, uint32_t* orig, uint32_t* err,
-                                 const uint32_t* cbase, uint32_t* maxh_bits, hipStream_t s);
+                                 const uint32_t* cbase, uint32_t* maxh_bits, float2* zc,
+                                 hipStream_t s);
 // out[i] = src[idx[i]]
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* out,
                        hipStream_t s);
@@ -160,6 +167,9 @@ void launch_view_project(const particle_soa& ps, const view_proj& vp, const gaun
 // Device view of one built grid (sorted SoA + cells).
 struct grid_view {
   const float4* part;         // (x, y, coef, 1/h^2) sorted by cell
+  // line-of-sight clip (OP_CLIP): per sorted particle, (0 - z') / h and
+  // (depth - z') / h, the view's depth range relative to the particle
+  const float2* clip;
   const uint32_t* key;        // sorted cell keys
   int sparse;                 // fewer than 0.4 particles per cell: stream row ranges
   const float* hs;            // sorted h
@@ -181,7 +191,8 @@ struct point_view {
   uint32_t lnx, lny;
 };
 
-enum gather_op { OP_REFERENCE = 0, OP_EXACT = 1, OP_COUNT = 2, OP_ADJOINT = 3 };
+// OP_CLIP: exact mode with the line-of-sight kernel clip (grid_view::clip)
+enum gather_op { OP_REFERENCE = 0, OP_EXACT = 1, OP_COUNT = 2, OP_ADJOINT = 3, OP_CLIP = 4 };
 
 struct tile_shape {
   int tw;   // tile width (points)

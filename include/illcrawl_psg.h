@@ -449,6 +449,15 @@ ilc_status ilc_batch_backward(ilc_batch* b, const float* const* dl_dimage_dev, f
 ilc_status ilc_batch_last_gather_ms(const ilc_batch* b, float* ms);
 /* Gather tile (points); tile_w * tile_h must be a multiple of 256. */
 ilc_status ilc_batch_set_tile(ilc_batch* b, int32_t tile_w, int32_t tile_h);
+/* Depth cut of every view.  clip = 0 (default): the reference's centre
+ * mask, a particle counts whole when its camera depth lies in [0, depth] and
+ * not at all otherwise (projection.cl:80-91).  clip = 1: the line-of-sight
+ * kernel clip, which the reference does not have: each particle adds the
+ * part of its 3-D quartic kernel's column that lies in [0, depth] (slabs of
+ * a tomography stack share the particles their supports straddle).  Needs
+ * exact mode and a quartic kernel; renders must be persistent (not plain),
+ * and backward is not provided for it. */
+ilc_status ilc_batch_set_depth_clip(ilc_batch* b, int32_t clip);
 /* Times the batched render for each supported tile shape, keeps the
  * fastest and returns it (no reference counterpart: measurement-driven
  * tuning). */

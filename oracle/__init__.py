@@ -67,6 +67,12 @@ PINNING = {
                               "energy with a constant table; the reference's van Hoof "
                               "table (gaunt_factor.dat): its entries at texel centres and "
                               "the mean of the four entries around each grid node",
+    "orc_gather_clip": "a depth range holding the whole support equals orc_gather_exact; "
+                       "complementary ranges add up to the whole; a cut through a "
+                       "particle's centre gives half its column; a thin slab gives its "
+                       "thickness times the 3-D quartic (1 - q^2)^2 K (15/16) / h^3",
+    "orc_project_particles_depth": "equals orc_project_particles in x', y' and z' = the "
+                                   "depth it masks on",
     "orc_crawl_block": "hand-built rows: periodic images moved in by one period (each "
                        "axis, both signs), border tolerance, inclusive volume faces, "
                        "input order kept; numpy brute force on random rows",
@@ -110,6 +116,11 @@ def _declare(L):
     L.orc_projection_matrix.restype = None
     L.orc_project_particles.argtypes = [_f32p, _u64, _f32p, _f32p, ctypes.c_float, _f32p]
     L.orc_project_particles.restype = None
+    L.orc_project_particles_depth.argtypes = [_f32p, _u64, _f32p, _f32p, _f32p]
+    L.orc_project_particles_depth.restype = None
+    L.orc_gather_clip.argtypes = [_f32p, _f32p, _u64, _f32p, _u64, ctypes.c_double,
+                                  ctypes.c_int, _f64p]
+    L.orc_gather_clip.restype = None
     L.orc_pixel_points.argtypes = [_u64, _u64, ctypes.c_float, _f32p, _f32p, _f32p, _f32p,
                                    _f32p, _f32p]
     L.orc_pixel_points.restype = None
@@ -221,6 +232,14 @@ class Camera:
                                     self.position.astype(np.float32), np.float32(depth), out)
         return out
 
+    def project_particles_depth(self, p4):
+        """(x', y', z', w): the projection keeping the camera depth, w unmasked."""
+        p4 = _c(p4, np.float32)
+        out = np.zeros_like(p4)
+        lib().orc_project_particles_depth(p4, p4.shape[0], self.projection_matrix(),
+                                          self.position.astype(np.float32), out)
+        return out
+
 
 # ---------------------------------------------------------------- grid
 
@@ -315,6 +334,18 @@ def gather_exact(p4, h, pts4, kind=K_QUARTIC_REF, with_counts=False):
     lib().orc_gather_exact(p4, h, p4.shape[0], pts4, pts4.shape[0], int(kind), out,
                            cnt.ctypes.data if cnt is not None else None)
     return (out, cnt) if with_counts else out
+
+
+def gather_clip(p4z, h, pts4, depth, kind=K_QUARTIC_REF):
+    """Line-of-sight clipped projection over [0, depth] (FP64 quadrature);
+    p4z from Camera.project_particles_depth."""
+    p4z = _c(p4z, np.float32)
+    h = _c(h, np.float32)
+    pts4 = _c(pts4, np.float32)
+    out = np.zeros(pts4.shape[0], np.float64)
+    lib().orc_gather_clip(p4z, h, p4z.shape[0], pts4, pts4.shape[0], float(depth), int(kind),
+                          out)
+    return out
 
 
 def count_evals_lattice(p4, h, pts4, nx, ny, support_factor=1.0):

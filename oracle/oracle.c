@@ -186,6 +186,22 @@ void orc_project_particles(const float* in4, uint64_t n, const float minv[9],
   }
 }
 
+/* The same projection keeping the camera depth z' in the z slot and w
+ * unmasked (input of orc_gather_clip). */
+void orc_project_particles_depth(const float* in4, uint64_t n, const float minv[9],
+                                 const float center[3], float* out4)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)n; ++k) {
+    float p[3];
+    f_project(minv, center, in4 + 4 * k, p);
+    out4[4 * k + 0] = p[0];
+    out4[4 * k + 1] = p[1];
+    out4[4 * k + 2] = p[2];
+    out4[4 * k + 3] = in4[4 * k + 3];
+  }
+}
+
 /*
  * Pixel corner coordinates, then their projection:
  * camera.cl:23-41 (camera_generate_pixel_coordinates, index iy*nx+ix),
@@ -709,6 +725,63 @@ void orc_gather_exact(const float* p4, const float* h, uint64_t n,
     }
     out[t] = acc;
     if (counts) counts[t] = cnt;
+  }
+}
+
+/*
+ * Line-of-sight clipped projection (the reference has none: it masks by
+ * the particle centre, projection.cl:80-91).  Each particle contributes the
+ * part of its 3-D quartic's column that lies in the view's depth range
+ * [0, depth]:
+ *   out[t] = sum_j w_j int_{0 - z_j}^{depth - z_j} W3(sqrt(r^2 + z^2), h_j) dz,
+ * with W3(q h) = K (15/16) (1 - q^2)^2 / h^3, K the projected kernel's
+ * normalisation of d_kernel2d (the reference's constants, with its factor 2,
+ * for ORC_K_QUARTIC_REF), so that the whole column equals d_kernel2d.  The
+ * integral runs over the part of the range inside the support (|z| < a,
+ * a^2 = h^2 - r^2) by 8-point Gauss-Legendre quadrature in FP64 (the
+ * integrand is a polynomial of degree 4 in z, so the rule is exact up to
+ * rounding).  p4: (x', y', z', w) camera frame (orc_project_particles_depth).
+ */
+static const double orc_gl8_x[8] = {-0.9602898564975363, -0.7966664774136267,
+                                    -0.5255324099163290, -0.1834346424956498,
+                                    0.1834346424956498,  0.5255324099163290,
+                                    0.7966664774136267,  0.9602898564975363};
+static const double orc_gl8_w[8] = {0.1012285362903763, 0.2223810344533745,
+                                    0.3137066458778873, 0.3626837833783620,
+                                    0.3626837833783620, 0.3137066458778873,
+                                    0.2223810344533745, 0.1012285362903763};
+
+void orc_gather_clip(const float* p4, const float* h, uint64_t n, const float* pts4,
+                     uint64_t npts, double depth, int kind, double* out)
+{
+  const double K = kind == ORC_K_QUARTIC_REF ? 2.0 * (double)1.04445f * (double)1.066667f
+                                             : 3.5 / M_PI;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t t = 0; t < (int64_t)npts; ++t) {
+    const double ex = pts4[4 * t], ey = pts4[4 * t + 1];
+    double acc = 0.0;
+    for (uint64_t j = 0; j < n; ++j) {
+      const double w = p4[4 * j + 3];
+      if (w == 0.0) continue;
+      const double hj = h[j];
+      const double dx = (double)p4[4 * j] - ex, dy = (double)p4[4 * j + 1] - ey;
+      const double r2 = dx * dx + dy * dy;
+      if (r2 >= hj * hj) continue;
+      const double a = sqrt(hj * hj - r2);
+      const double zj = p4[4 * j + 2];
+      const double lo = fmax(0.0 - zj, -a), hi = fmin(depth - zj, a);
+      if (!(hi > lo)) continue;
+      double s = 0.0;
+      for (int k = 0; k < 8; ++k) {
+        const double z = 0.5 * (lo + hi) + 0.5 * (hi - lo) * orc_gl8_x[k];
+        const double q2 = (r2 + z * z) / (hj * hj);
+        const double f = q2 < 1.0 ? (1.0 - q2) * (1.0 - q2) : 0.0;
+        s += orc_gl8_w[k] * f;
+      }
+      s *= 0.5 * (hi - lo);
+      acc += w * K * (15.0 / 16.0) * s / (hj * hj * hj);
+    }
+    out[t] = acc;
   }
 }
 

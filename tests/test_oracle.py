@@ -714,3 +714,81 @@ def test_gff_at_texel_centres_and_nodes_of_the_reference_table(orc):
             if np.min(np.abs(quad - mean)) > 1e-4 * mean:
                 checked += 1
     assert checked >= 20
+
+
+# ------------------------------------------- line-of-sight clipped projection
+
+def _clip_scene(n=300, seed=9):
+    rng = np.random.default_rng(seed)
+    p = np.zeros((n, 4), np.float32)
+    p[:, 0] = rng.uniform(-50, 50, n)
+    p[:, 1] = rng.uniform(-50, 50, n)
+    p[:, 2] = rng.uniform(100, 400, n)     # camera depth z'
+    p[:, 3] = rng.lognormal(0.0, 0.3, n)
+    h = rng.uniform(5.0, 60.0, n).astype(np.float32)
+    g = np.stack(np.meshgrid(np.linspace(-60, 60, 17), np.linspace(-60, 60, 17)), -1)
+    pts = np.zeros((17 * 17, 4), np.float32)
+    pts[:, :2] = g.reshape(-1, 2)
+    return p, h, pts
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_clip_whole_support_equals_unclipped(orc, kind):
+    """A depth range holding every particle's whole support gives the plain
+    projected kernel (orc_gather_exact)."""
+    p, h, pts = _clip_scene()
+    got = orc.gather_clip(p, h, pts, 1000.0, kind)
+    flat = p.copy()
+    flat[:, 2] = 0.0
+    ref = orc.gather_exact(flat, h, pts, kind)
+    assert np.allclose(got, ref, rtol=1e-12, atol=0)
+
+
+def test_clip_ranges_add_and_halve(orc):
+    """Complementary depth ranges add up to the whole (the depth range of a
+    view starts at 0, so [0, d1] + [d1, d] is checked as the whole minus the
+    first part shifted); a range ending at a particle's centre takes half of
+    its column; a thin slab of thickness e at depth offset z takes e times
+    the 3-D quartic K (15/16) (1 - q^2)^2 / h^3 at q^2 = (r^2 + z^2) / h^2."""
+    p, h, pts = _clip_scene(n=80, seed=4)
+    whole = orc.gather_clip(p, h, pts, 1000.0)
+    first = orc.gather_clip(p, h, pts, 250.0)
+    q = p.copy()
+    q[:, 2] -= 250.0                        # the range [250, 1000] seen from 250
+    second = orc.gather_clip(q, h, pts, 750.0)
+    assert np.allclose(first + second, whole, rtol=1e-12, atol=1e-300)
+    # one particle: a range ending at its centre
+    one = np.array([[3.0, -4.0, 200.0, 1.7]], np.float32)
+    h1 = np.array([20.0], np.float32)
+    half = orc.gather_clip(one, h1, pts, 200.0)
+    full = orc.gather_clip(one, h1, pts, 1000.0)
+    m = full > 0
+    assert m.sum() > 4 and np.allclose(half[m], 0.5 * full[m], rtol=1e-13)
+    # a thin slab [z0, z0 + e]: the midpoint rule to O(e^2)
+    K = 2.0 * float(np.float32(1.04445)) * float(np.float32(1.066667))
+    e, z0 = 1e-3, 7.0
+    sl = one.copy()
+    sl[0, 2] = -z0                           # the particle z0 before the range's start
+    thin = orc.gather_clip(sl, h1, pts, e)
+    r2 = (pts[:, 0].astype(np.float64) - 3.0) ** 2 + (pts[:, 1].astype(np.float64) + 4.0) ** 2
+    zc = z0 + 0.5 * e
+    q2 = (r2 + zc * zc) / 400.0
+    w3 = np.where(q2 < 1.0, (1.0 - q2) ** 2, 0.0) * K * 15.0 / 16.0 / 8000.0
+    m = w3 > 1e-3 * w3.max()
+    assert np.allclose(thin[m], 1.7 * e * w3[m], rtol=1e-6)
+
+
+def test_project_particles_depth_matches_masked_projection(orc):
+    """x', y' equal the masked projection's; the masked one zeroes w exactly
+    where z' lies outside [0, depth]."""
+    from illcrawl_amd import synth
+    snap = synth.make_snapshot(1 << 12, 3, 10000.0)
+    v = synth.centred_view(snap.box, look_at=(0.3, -0.5, 0.8), width=2000.0, nx=16, roll=0.4)
+    cam = orc.Camera(v["position"], v["look_at"], v["roll"], v["width"], v["nx"], v["ny"])
+    depth = 0.6 * v["depth"]
+    pz = cam.project_particles_depth(snap.particles4())
+    pm = cam.project_particles(snap.particles4(), depth)
+    assert np.array_equal(pz[:, :2], pm[:, :2])
+    inside = (pz[:, 2] >= 0) & (pz[:, 2] <= np.float32(depth))
+    assert np.array_equal(pm[:, 3] != 0, inside & (pz[:, 3] != 0))
+    assert 0 < inside.sum() < snap.n
