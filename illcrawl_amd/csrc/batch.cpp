@@ -130,7 +130,7 @@ struct ilc_batch {
   // persistent render; the builds of different views are independent
   std::vector<std::unique_ptr<build_lane>> lanes;
   hipEvent_t ev_fork = nullptr;
-  std::vector<std::unique_ptr<grid_store>> group_grids;  // full grid per slab group
+  std::vector<std::unique_ptr<grid_store>> group_grids;  // per-cell max h per slab group
   std::vector<size_t> group_leader;
   dbuf<uint32_t> wcounts, woffs, wtotal;
   pinned<uint32_t> wtotal_host;
@@ -528,13 +528,14 @@ size_t view_index(const ilc_batch* b, const view_state& vs) {
   fail(ILC_EINVAL, "view not in batch");
 }
 
-// the full grid of slab group gi
+// the per-cell max h of slab group gi's full grid (its depth-compacted views
+// search their own cells with it; nothing else of that grid is read)
 void build_group(ilc_batch* b, size_t gi, const build_ctx& c) {
   view_state& ls = *b->views[b->group_leader[gi]];
   c.p4tmp->ensure(b->ps.n);
   launch_view_project(b->ps, proj_of(b, ls), gaunt_of(b), c.p4tmp->p, c.st);
-  enqueue_grid_build(*b->group_grids[gi], *c.sc, c.p4tmp->p, b->ps.hsml, b->ps.n, ls.gp, ls.kz0,
-                     ls.max_h, b->kernel, b->counter.p + 1, c.st);
+  enqueue_grid_maxh(*b->group_grids[gi], c.p4tmp->p, b->ps.hsml, b->ps.n, ls.gp, ls.kz0,
+                    ls.max_h, b->counter.p + 1, c.st);
 }
 
 // view v's grid and cleared accumulator

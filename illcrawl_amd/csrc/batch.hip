@@ -365,6 +365,10 @@ __device__ inline bool reaches_screen(const window_f& win, float ax, float ay, f
   return d2 * ih2 < win.cull2;
 }
 
+// views per workgroup barrier in the multi-view passes: a tile's per-wave
+// counts of kViewChunk views are exchanged at once
+constexpr int kViewChunk = 16;
+
 // every listed view's kept rows per tile, with and without the screen test,
 // from one read of the tile (k_count_totals sums them per view)
 __global__ __launch_bounds__(kBlock) void k_multi_window_count(
@@ -372,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
     const view_build_desc* __restrict__ vd, const int32_t* __restrict__ views, int nviews,
     uint32_t ntiles, uint32_t* __restrict__ counts, uint32_t* __restrict__ counts_plain) {
 #pragma clang fp contract(off)
-  __shared__ uint32_t wc[2][8];
+  __shared__ uint32_t wc[2][kViewChunk][8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
   float px[kWinItems], py[kWinItems], pz[kWinItems], ih2[kWinItems], hr[kWinItems];
@@ -388,46 +392,51 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
     hr[r] = h;
     ih2[r] = 1.0f / (h * h);  // as the grid's SoA holds it
   }
-  for (int k = 0; k < nviews; ++k) {
-    const int v = views[k];
-    const view_proj vp = vd[v].vp;
-    const window_f win = vd[v].win;
-    uint32_t c = 0, cp = 0;
-    if (!win.use_window && !win.use_depth) {
+  int slot = 0;
+  for (int k0 = 0; k0 < nviews; k0 += kViewChunk, slot ^= 1) {
+    const int nk = min(kViewChunk, nviews - k0);
+    for (int kk = 0; kk < nk; ++kk) {
+      const int v = views[k0 + kk];
+      const view_proj vp = vd[v].vp;
+      const window_f win = vd[v].win;
+      uint32_t c = 0, cp = 0;
+      if (!win.use_window && !win.use_depth) {
 #pragma unroll
-      for (int r = 0; r < kWinItems; ++r) c += __popcll(__ballot(in[r]));
-      cp = c;
-    } else {
+        for (int r = 0; r < kWinItems; ++r) c += __popcll(__ballot(in[r]));
+        cp = c;
+      } else {
 #pragma unroll
-      for (int r = 0; r < kWinItems; ++r) {
-        const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
-        bool keep = in[r];
-        float ax = 0.0f, ay = 0.0f;
-        if (win.use_window) {
-          ax = dot3(vp.m + 0, vx, vy, vz);
-          ay = dot3(vp.m + 3, vx, vy, vz);
-          keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
+        for (int r = 0; r < kWinItems; ++r) {
+          const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
+          bool keep = in[r];
+          float ax = 0.0f, ay = 0.0f;
+          if (win.use_window) {
+            ax = dot3(vp.m + 0, vx, vy, vz);
+            ay = dot3(vp.m + 3, vx, vy, vz);
+            keep = keep && ax >= win.x0 && ax <= win.x1 && ay >= win.y0 && ay <= win.y1;
+          }
+          if (win.use_depth) {
+            const float az = dot3(vp.m + 6, vx, vy, vz);
+            keep = keep && depth_keep(az, hr[r], vp.depth, win.clip);
+          }
+          cp += __popcll(__ballot(keep));
+          if (win.use_screen) keep = keep && reaches_screen(win, ax, ay, ih2[r]);
+          c += __popcll(__ballot(keep));
         }
-        if (win.use_depth) {
-          const float az = dot3(vp.m + 6, vx, vy, vz);
-          keep = keep && depth_keep(az, hr[r], vp.depth, win.clip);
-        }
-        cp += __popcll(__ballot(keep));
-        if (win.use_screen) keep = keep && reaches_screen(win, ax, ay, ih2[r]);
-        c += __popcll(__ballot(keep));
+      }
+      if (lane == 0) {
+        wc[slot][kk][wave] = c;
+        wc[slot][kk][4 + wave] = cp;
       }
     }
-    const int slot = k & 1;
-    if (lane == 0) {
-      wc[slot][wave] = c;
-      wc[slot][4 + wave] = cp;
-    }
+    // chunk j writes wc[j & 1]; its readers are done before barrier j + 1,
+    // which every writer of chunk j + 2 passes first
     __syncthreads();
-    if (threadIdx.x == 0) {
-      const uint32_t t = wc[slot][0] + wc[slot][1] + wc[slot][2] + wc[slot][3];
-      const uint32_t tp = wc[slot][4] + wc[slot][5] + wc[slot][6] + wc[slot][7];
-      counts[uint64_t(v) * ntiles + blockIdx.x] = t;
-      counts_plain[uint64_t(v) * ntiles + blockIdx.x] = tp;
+    if (int(threadIdx.x) < nk) {
+      const uint32_t* w = wc[slot][threadIdx.x];
+      const uint64_t o = uint64_t(views[k0 + int(threadIdx.x)]) * ntiles + blockIdx.x;
+      counts[o] = w[0] + w[1] + w[2] + w[3];
+      counts_plain[o] = w[4] + w[5] + w[6] + w[7];
     }
   }
 }
@@ -491,11 +500,12 @@ __global__ __launch_bounds__(kBlock) void k_particle_xray(particle_soa ps, gaunt
 // descending-h sort key: the top 16 bits of h's FP32 pattern, reversed
 __device__ inline uint32_t hq_key(float h) { return 0x7fffu - (__float_as_uint(h) >> 16); }
 
-// k_window_scatter for every listed view from one read of the tile.  Per
-// view, the kept counts of the kWinItems rows (rows of kBlock particles) are
-// taken per wave first, so one barrier per view gives every row's offset;
-// the rows are then written one at a time, skipping those with nothing kept,
-// in the order of k_window_scatter (rows, then threads).
+// k_window_scatter for every listed view from one read of the tile.  For
+// kViewChunk views at a time, the kept counts of the kWinItems rows (rows of
+// kBlock particles) are taken per wave first (a thread's keep bits in two
+// registers), so one barrier gives every row's offset in all of them; each
+// view's rows are then written one at a time, skipping those with nothing
+// kept, in the order of k_window_scatter (rows, then threads).
 __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     particle_soa ps, const float* __restrict__ wx, const view_build_desc* __restrict__ vd,
     const int32_t* __restrict__ views, int nviews, uint32_t ntiles,
@@ -505,11 +515,13 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     const uint32_t* __restrict__ cbase, uint32_t* __restrict__ maxh_bits,
     float2* __restrict__ zc) {
 #pragma clang fp contract(off)
-  __shared__ __align__(16) uint32_t wc[2][kWinItems][4];
-  // the tile's positions, read once (SoA)
+  static_assert(kViewChunk <= 16 && kWinItems == 8, "keep bits: 8 views per 64-bit word");
+  __shared__ __align__(16) uint32_t wc[2][kViewChunk][kWinItems][4];
+  // the tile's positions, read once (SoA); h and 1/h^2 in registers
   __shared__ float sx[kWinTile], sy[kWinTile], sz[kWinTile];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
+  float hr[kWinItems], ih2[kWinItems];
 #pragma unroll
   for (int r = 0; r < kWinItems; ++r) {
     const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
@@ -517,6 +529,9 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     sx[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i] : 0.0f;
     sy[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i + 1] : 0.0f;
     sz[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i + 2] : 0.0f;
+    const float h = ok ? ps.hsml[i] : 1.0f;
+    hr[r] = h;
+    ih2[r] = 1.0f / (h * h);
   }
   __syncthreads();
   // cell key of (x', y', 0) in a view's grid: grid3d_get_cell_indices,
@@ -540,7 +555,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
   // and depth cut whose support cannot reach the screen is not kept; it
   // raises its cell's max h, which the search box of the reference reads.
   auto kept = [&](const view_proj& vp, const window_f& win, const grid_params& g, uint32_t cb,
-                  float x, float y, float z, bool ok, uint64_t i) {
+                  float x, float y, float z, float h, float ih2r, bool ok) {
     const float vx = x - vp.c[0], vy = y - vp.c[1], vz = z - vp.c[2];
     bool keep = ok;
     float ax = 0.0f, ay = 0.0f;
@@ -551,11 +566,10 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     }
     if (win.use_depth) {
       const float az = dot3(vp.m + 6, vx, vy, vz);
-      keep = keep && depth_keep(az, ps.hsml[i], vp.depth, win.clip);
+      keep = keep && depth_keep(az, h, vp.depth, win.clip);
     }
     if (win.use_screen && keep) {
-      const float h = ps.hsml[i];
-      if (!reaches_screen(win, ax, ay, 1.0f / (h * h))) {
+      if (!reaches_screen(win, ax, ay, ih2r)) {
         keep = false;
         bool in_grid;
         const uint32_t key = cell_key(g, ax, ay, &in_grid);
@@ -571,60 +585,81 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     }
     return keep;
   };
-  for (int k = 0; k < nviews; ++k) {
-    const int v = views[k];
-    const view_proj vp = vd[v].vp;
-    const window_f win = vd[v].win;
-    const uint32_t cb = cbase[v];
-    const int slot = k & 1;
-    uint32_t bits = 0;  // this thread's kept rows
+  int slot = 0;
+  for (int k0 = 0; k0 < nviews; k0 += kViewChunk, slot ^= 1) {
+    const int nk = min(kViewChunk, nviews - k0);
+    // this thread's kept rows: view kk of the chunk in byte kk % 8 of
+    // bits[kk / 8]
+    uint64_t bits_lo = 0, bits_hi = 0;
+    for (int kk = 0; kk < nk; ++kk) {
+      const int v = views[k0 + kk];
+      const view_proj vp = vd[v].vp;
+      const window_f win = vd[v].win;
+      const uint32_t cb = cbase[v];
+      uint32_t rb = 0;
 #pragma unroll
-    for (int r = 0; r < kWinItems; ++r) {
-      const int t = r * kBlock + threadIdx.x;
-      const bool ok = base + uint64_t(t) < ps.n;
-      const bool keep = kept(vp, win, vd[v].g, cb, sx[t], sy[t], sz[t], ok, base + uint64_t(t));
-      bits |= keep ? 1u << r : 0u;
-      const uint32_t c = __popcll(__ballot(keep));
-      if (lane == 0) wc[slot][r][wave] = c;
-    }
-    __syncthreads();
-    uint32_t out = offs[uint64_t(v) * ntiles + blockIdx.x];
-#pragma unroll 1
-    for (int r = 0; r < kWinItems; ++r) {
-      const uint4 cw = *reinterpret_cast<const uint4*>(wc[slot][r]);
-      const uint32_t tot = cw.x + cw.y + cw.z + cw.w;
-      if (tot == 0) continue;
-      const uint32_t before = (wave > 0 ? cw.x : 0u) + (wave > 1 ? cw.y : 0u) +
-                              (wave > 2 ? cw.z : 0u);
-      const int t = r * kBlock + threadIdx.x;
-      const uint64_t i = base + uint64_t(t);
-      const bool keep = (bits >> r) & 1u;
-      const uint64_t bal = __ballot(keep);
-      if (keep) {
-        const float vx = sx[t] - vp.c[0], vy = sy[t] - vp.c[1], vz = sz[t] - vp.c[2];
-        const float ax = dot3(vp.m + 0, vx, vy, vz);
-        const float ay = dot3(vp.m + 3, vx, vy, vz);
-        const float az = dot3(vp.m + 6, vx, vy, vz);
-        const grid_params& g = vd[v].g;
-        const uint32_t j = out + before +
-                           __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
-        float w = vp.quantity == 1 ? wx[i] : static_cast<float>(vp.scale * double(ps.mass[i]));
-        // h rides in the z slot, so the SoA pass reads one line per row; the
-        // descending-h sort starts from (hq key, row)
-        const float hh = ps.hsml[i];
-        if (!depth_keep(az, hh, vp.depth, win.clip)) w = 0.0f;
-        if (zc) zc[j] = make_float2((0.0f - az) / hh, (vp.depth - az) / hh);
-        p4c[j] = make_float4(ax, ay, hh, w);
-        hqkey[j] = hq_key(hh);
-        hqval[j] = j;
-        bool in_grid;
-        const uint32_t key = cell_key(g, ax, ay, &in_grid);
-        if (!in_grid) atomicAdd(err, 1u);
-        gkey[j] = cb + key;
-        orig[j] = uint32_t(i);
+      for (int r = 0; r < kWinItems; ++r) {
+        const int t = r * kBlock + threadIdx.x;
+        const bool ok = base + uint64_t(t) < ps.n;
+        const bool keep = kept(vp, win, vd[v].g, cb, sx[t], sy[t], sz[t], hr[r], ih2[r], ok);
+        rb |= keep ? 1u << r : 0u;
+        const uint32_t c = __popcll(__ballot(keep));
+        if (lane == 0) wc[slot][kk][r][wave] = c;
       }
-      out += tot;
+      const uint64_t sh = uint64_t(rb) << (8 * (kk & 7));
+      if (kk < 8)
+        bits_lo |= sh;
+      else
+        bits_hi |= sh;
+    }
+    // chunk j writes wc[j & 1]; every read of it comes before barrier j + 1,
+    // which every writer of chunk j + 2 passes first
+    __syncthreads();
+    for (int kk = 0; kk < nk; ++kk) {
+      const int v = views[k0 + kk];
+      const view_proj vp = vd[v].vp;
+      const window_f win = vd[v].win;
+      const uint32_t cb = cbase[v];
+      const uint32_t bits = uint32_t((kk < 8 ? bits_lo : bits_hi) >> (8 * (kk & 7))) & 0xffu;
+      uint32_t out = offs[uint64_t(v) * ntiles + blockIdx.x];
+#pragma unroll 1
+      for (int r = 0; r < kWinItems; ++r) {
+        const uint4 cw = *reinterpret_cast<const uint4*>(wc[slot][kk][r]);
+        const uint32_t tot = cw.x + cw.y + cw.z + cw.w;
+        if (tot == 0) continue;
+        const uint32_t before = (wave > 0 ? cw.x : 0u) + (wave > 1 ? cw.y : 0u) +
+                                (wave > 2 ? cw.z : 0u);
+        const int t = r * kBlock + threadIdx.x;
+        const uint64_t i = base + uint64_t(t);
+        const bool keep = (bits >> r) & 1u;
+        const uint64_t bal = __ballot(keep);
+        if (keep) {
+          const float vx = sx[t] - vp.c[0], vy = sy[t] - vp.c[1], vz = sz[t] - vp.c[2];
+          const float ax = dot3(vp.m + 0, vx, vy, vz);
+          const float ay = dot3(vp.m + 3, vx, vy, vz);
+          const float az = dot3(vp.m + 6, vx, vy, vz);
+          const grid_params& g = vd[v].g;
+          const uint32_t j =
+              out + before +
+              __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
+                                        __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
+          float w = vp.quantity == 1 ? wx[i] : static_cast<float>(vp.scale * double(ps.mass[i]));
+          // h rides in the z slot, so the SoA pass reads one line per row;
+          // the descending-h sort starts from (hq key, row)
+          const float hh = ps.hsml[i];
+          if (!depth_keep(az, hh, vp.depth, win.clip)) w = 0.0f;
+          if (zc) zc[j] = make_float2((0.0f - az) / hh, (vp.depth - az) / hh);
+          p4c[j] = make_float4(ax, ay, hh, w);
+          hqkey[j] = hq_key(hh);
+          hqval[j] = j;
+          bool in_grid;
+          const uint32_t key = cell_key(g, ax, ay, &in_grid);
+          if (!in_grid) atomicAdd(err, 1u);
+          gkey[j] = cb + key;
+          orig[j] = uint32_t(i);
+        }
+        out += tot;
+      }
     }
   }
 }

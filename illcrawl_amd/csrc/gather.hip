@@ -57,6 +57,9 @@ namespace {
 
 constexpr int kCap = 320;    // list capacity (entries)
 constexpr int kFlush = 64;   // evaluate when a list reaches this size
+// a tile of a sparse grid leaves the row streaming for the cell enumeration
+// above this many particles per candidate cell
+constexpr uint32_t kDenseRows = 2;
 // a batch of kBlock candidates lands on a list holding at most kFlush - 1
 static_assert(kFlush - 1 + kBlock <= kCap, "list capacity");
 constexpr int kCubicTable = 4096;
@@ -685,7 +688,10 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   // grids (well under one particle per cell, e.g. the depth-cut lists of a
   // slab) stream the row ranges and cull per particle.  Both give the same
   // candidates in the same order (rows, cells by x, particles in sorted
-  // order).
+  // order).  A sparse grid can still be dense where the tile is (a halo
+  // inside a slab): when the first chunk of rows holds more than
+  // kDenseRows particles per cell of the box, the tile switches to the cell
+  // enumeration, whose max-h skip and prefix search read far fewer.
   const int ncols = cx1 >= cx0 ? cx1 - cx0 + 1 : 0;
   const int nrows = cy1 >= cy0 ? cy1 - cy0 + 1 : 0;
   const int nlays = cz1 >= cz0 ? cz1 - cz0 + 1 : 0;
@@ -695,9 +701,9 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   const uint32_t nrowz = ncols > 0 ? uint32_t(nrows) * uint32_t(nlays) : 0u;
   const uint32_t nxny = uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
   const float sup_h2 = support * support * 1.00002f;
-  const bool rows_mode = g.sparse != 0 && ncand > uint32_t(kBlock);
+  bool rows_mode = g.sparse != 0 && ncand > uint32_t(kBlock);
   const bool one_layer = nlays == 1;
-  const uint64_t nitems = rows_mode ? nrowz : ncand;
+  uint64_t nitems = rows_mode ? nrowz : ncand;
   for (uint64_t c0 = 0; c0 < nitems; c0 += kBlock) {
     const bool in_box = c0 + uint64_t(t) < nitems;
     const uint32_t ci = uint32_t(c0) + uint32_t(t);
@@ -754,6 +760,14 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
     }
     uint32_t tot;
     const uint32_t coff = block_scan_excl(cn, &tot, sm.scan_ws);
+    tot = uint32_t(uniform(int(tot)));
+    if (rows_mode && c0 == 0 && uint64_t(tot) > uint64_t(kDenseRows) * ncand) {
+      // nothing of this chunk is in LDS yet: start over with the cells
+      rows_mode = false;
+      nitems = ncand;
+      c0 = uint64_t(0) - uint64_t(kBlock);
+      continue;
+    }
     sm.cell_b[t] = cb;
     sm.cell_off[t] = coff;
     if (rows_mode) {
@@ -764,7 +778,6 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       sm.cell_y[t] = cfull;
     }
     __syncthreads();
-    tot = uint32_t(uniform(int(tot)));
     for (uint32_t v0 = 0; v0 < tot; v0 += kBlock) {
       const uint32_t v = v0 + uint32_t(t);
       float4 e = make_float4(0.f, 0.f, 0.f, 0.f);

@@ -412,6 +412,39 @@ inline uint32_t blocks_for(uint64_t n, uint64_t per) {
   return static_cast<uint32_t>((n + per - 1) / per);
 }
 
+// Per-cell max h without the sort (a slab group's shared grid: its views
+// read only the per-cell max h and the grid parameters): each particle's cell
+// as k_keys computes it, and an atomic max that is only issued when the cell's
+// current max is smaller (clustered cells hold many particles).  maxh_bits
+// starts at 0; particles outside the grid count in err.
+__global__ __launch_bounds__(kBlock) void k_cell_maxh(const float4* __restrict__ p4,
+                                                      const float* __restrict__ h, uint32_t n,
+                                                      grid_params g,
+                                                      uint32_t* __restrict__ maxh_bits,
+                                                      uint32_t* __restrict__ err) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = p4[i];
+  const float c[3] = {q.x, q.y, q.z};
+  int32_t idx[3];
+  bool ok = true;
+  for (int a = 0; a < 3; ++a) {
+    const float d = c[a] - g.minc[a];
+    idx[a] = floor_to_int(d / g.cs[a]);
+    ok = ok && idx[a] >= 0 && idx[a] < g.nc[a];
+  }
+  if (!ok) {
+    atomicAdd(err, 1u);
+    return;
+  }
+  const uint32_t key = uint32_t(idx[0]) + uint32_t(idx[1]) * uint32_t(g.nc[0]) +
+                       uint32_t(idx[2]) * uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
+  const uint32_t hb = __float_as_uint(h[i]);
+  uint32_t* m = maxh_bits + key;
+  if (hb > __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(m, hb);
+}
+
 }  // namespace
 
 void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
@@ -434,6 +467,14 @@ void launch_bbox(const float4* p4, const float* h, uint64_t n, float* partial, i
 void launch_bbox_final(const float* partial, int nparts, const uint32_t* bad, bbox_result* out,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, partial, nparts, bad, out);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_cell_maxh(const float4* p4, const float* h, uint32_t n, const grid_params& g,
+                      uint32_t* maxh_bits, uint32_t* err, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_cell_maxh, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p4, h, n, g,
+                     maxh_bits, err);
   ILC_HIP(hipGetLastError());
 }
 

@@ -118,6 +118,26 @@ void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, cons
   enqueue_grid_from_keys(gs, sc, p4, h, n, g, kz0, max_h, kernel, s);
 }
 
+void enqueue_grid_maxh(grid_store& gs, const float4* p4, const float* h, uint64_t n,
+                       const grid_params& g, int32_t kz0, float max_h, uint32_t* err,
+                       hipStream_t s) {
+  const uint64_t ncells = g.ncells();
+  gs.ready = false;
+  gs.maxh_bits.ensure(ncells);
+  launch_fill_u32(gs.maxh_bits.p, 0u, ncells, s);
+  launch_cell_maxh(p4, h, uint32_t(n), g, gs.maxh_bits.p, err, s);
+  gs.volumetric = false;
+  gs.gp = g;
+  gs.n = n;
+  gs.max_h = max_h;
+  float dd = g.cs[0] * g.cs[0];
+  dd = dd + g.cs[1] * g.cs[1];
+  dd = dd + g.cs[2] * g.cs[2];
+  gs.cell_radius = 0.5f * std::sqrt(dd);
+  gs.kz0 = kz0;
+  gs.ready = true;
+}
+
 void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,
                             uint64_t n, const grid_params& g, int32_t kz0, float max_h,
                             int kernel, hipStream_t s) {

@@ -62,6 +62,13 @@ void enqueue_grid_build(grid_store& gs, grid_scratch& sc, const float4* p4, cons
                         uint64_t n, const grid_params& g, int32_t kz0, float max_h, int kernel,
                         uint32_t* err, hipStream_t s);
 
+// Only the per-cell max h (gs.maxh_bits) and the grid parameters of n
+// particles, no sorted arrays (gs.part, key, hs, perm and start stay as they
+// were): for grids whose users read nothing else.
+void enqueue_grid_maxh(grid_store& gs, const float4* p4, const float* h, uint64_t n,
+                       const grid_params& g, int32_t kz0, float max_h, uint32_t* err,
+                       hipStream_t s);
+
 // The same from keys already in gs.key (cell keys) and gs.perm (indices into
 // p4 / h), n of them, in the order the within-cell order must follow.
 void enqueue_grid_from_keys(grid_store& gs, grid_scratch& sc, const float4* p4, const float* h,

@@ -15,6 +15,10 @@ void launch_bbox(const float4* p4, const float* h, uint64_t n, float* partial, i
                  uint32_t* bad, bbox_result* out, hipStream_t s);
 void launch_keys(const float4* p4, uint32_t n, const grid_params& g, uint32_t* keys,
                  uint32_t* vals, uint32_t* err, hipStream_t s);
+// Per-cell max h (float bits, atomic max; maxh_bits cleared by the caller)
+// of n particles, without sorting them; out-of-grid particles count in err.
+void launch_cell_maxh(const float4* p4, const float* h, uint32_t n, const grid_params& g,
+                      uint32_t* maxh_bits, uint32_t* err, hipStream_t s);
 uint64_t scan_temp_words(uint64_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* temp,
                            hipStream_t s);
@@ -293,6 +297,8 @@ void launch_volumetric_index(const float4* p4, const float* h, uint32_t n, const
 void launch_smooth3d(const grid_view& g, const level_view& lv, const point_view& p, int op,
                      double* acc, hipStream_t s, unsigned long long* cnt = nullptr);
 void launch_quantity_scaled(const float* in, double scale, uint64_t n, float* w, hipStream_t s);
+void launch_quantity_stellar_mass(const float* sft, const float* mass, uint64_t n, float* w,
+                                  hipStream_t s);
 // The reference's spectral X-ray emissivity (2-D Gaunt table on the device,
 // t[i + nx j], i along ln(gamma^2) from -6, j along ln(u) from -16, step 0.2)
 void launch_xray_flux(const float* rho, const float* u, const float* xe, uint64_t n,
