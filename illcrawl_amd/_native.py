@@ -10,7 +10,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libilcpsg.so")
+# ILC_LIB_PATH selects another build of the same library (e.g. the host-side
+# sanitizer build of tests/sanitized_run.py)
+LIB_PATH = os.environ.get("ILC_LIB_PATH") or os.path.join(HERE, "lib", "libilcpsg.so")
 
 ILC_OK, ILC_EINVAL, ILC_ENOMEM, ILC_EHIP, ILC_ENOTREADY, ILC_ERANGE = range(6)
 MODE_REFERENCE, MODE_EXACT = 0, 1
@@ -115,6 +117,7 @@ SIGNATURES = {
                                               ctypes.c_int32, ctypes.c_float, ctypes.c_float,
                                               ctypes.c_float, ctypes.c_float, _P, _P]),
     "ilc_quantity_scaled": (_ST, [_P, ctypes.c_double, _U64, _P, _P]),
+    "ilc_quantity_stellar_mass": (_ST, [_P, _P, _U64, _P, _P]),
     "ilc_quantity_xray_bolometric": (_ST, [_P, _P, _P, _U64, _P, ctypes.c_int32, ctypes.c_float,
                                            ctypes.c_float, _P, _P]),
     "ilc_batch_create": (_ST, [ctypes.POINTER(PsgConfig), ctypes.POINTER(_P)]),

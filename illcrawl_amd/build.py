@@ -23,6 +23,9 @@ DEMO = os.path.join(LIBDIR, "backend_demo")
 # C++ data crawler over an HDF5 snapshot (include/illcrawl_crawler.hpp)
 CRAWL_SRC = os.path.join(ROOT, "tests", "cpp", "crawl_demo.cpp")
 CRAWL = os.path.join(LIBDIR, "crawl_demo")
+# the render tool (illcrawl_render: image, animation and tomography targets)
+RENDER_SRC = os.path.join(HERE, "app", "illcrawl_render.cpp")
+RENDER = os.path.join(LIBDIR, "illcrawl_render")
 IO_PREFIX = os.environ.get("ILC_IO_PREFIX", "/opt/conda")  # HDF5 C++ and cfitsio
 
 SOURCES = ["host.cpp", "gridbuild.cpp", "capi.cpp", "batch.cpp", "crawl.hip", "grid.hip",
@@ -40,6 +43,27 @@ def hipcc():
         if c and os.path.exists(c):
             return c
     raise RuntimeError("hipcc not found")
+
+
+def build_io_program(src, out):
+    """A host program over libilcpsg, HDF5's C++ API and cfitsio."""
+    inc, lib = os.path.join(IO_PREFIX, "include"), os.path.join(IO_PREFIX, "lib")
+    # host code only (HIP runtime API): g++, whose libstdc++ matches the one
+    # libhdf5_cpp was built against
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    # The prefix ships an older libstdc++ of its own: its libraries are linked
+    # by path, and the system library directory precedes it in the run path.
+    syslib = "/usr/lib/x86_64-linux-gnu"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-isystem", os.path.join(rocm, "include"),
+           "-isystem", inc, src, "-L", LIBDIR, "-lilcpsg",
+           "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+           os.path.join(lib, "libhdf5_cpp.so"), os.path.join(lib, "libhdf5.so"),
+           os.path.join(lib, "libcfitsio.so"), f"-Wl,-rpath-link,{lib}",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib"),
+           f"-Wl,-rpath,{syslib}", f"-Wl,-rpath,{lib}", "-o", out + ".tmp"]
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
 
 
 def build(verbose=False, jobs=None):
@@ -80,26 +104,12 @@ def build(verbose=False, jobs=None):
                    DEMO_SRC, "-L", LIBDIR, "-lilcpsg", "-Wl,-rpath,$ORIGIN", "-o", DEMO + ".tmp"]
             subprocess.run(cmd, check=True)
             os.replace(DEMO + ".tmp", DEMO)
-        # the data crawler (HDF5 in, FITS out) needs HDF5's C++ API and cfitsio
-        if os.path.exists(CRAWL_SRC) and os.path.exists(os.path.join(IO_PREFIX, "include", "H5Cpp.h")):
-            inc, lib = os.path.join(IO_PREFIX, "include"), os.path.join(IO_PREFIX, "lib")
-            # host code only (HIP runtime API): g++, whose libstdc++ matches the
-            # one libhdf5_cpp was built against
-            rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-            # The prefix ships an older libstdc++ of its own: its libraries are
-            # linked by path, and the system library directory precedes it in
-            # the run path.
-            syslib = "/usr/lib/x86_64-linux-gnu"
-            cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
-                   "-I", os.path.join(ROOT, "include"), "-isystem", os.path.join(rocm, "include"),
-                   "-isystem", inc, CRAWL_SRC, "-L", LIBDIR, "-lilcpsg",
-                   "-L", os.path.join(rocm, "lib"), "-lamdhip64",
-                   os.path.join(lib, "libhdf5_cpp.so"), os.path.join(lib, "libhdf5.so"),
-                   os.path.join(lib, "libcfitsio.so"), f"-Wl,-rpath-link,{lib}",
-                   "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib"),
-                   f"-Wl,-rpath,{syslib}", f"-Wl,-rpath,{lib}", "-o", CRAWL + ".tmp"]
-            subprocess.run(cmd, check=True)
-            os.replace(CRAWL + ".tmp", CRAWL)
+        # the data crawler and the render tool (HDF5 in, FITS out) need
+        # HDF5's C++ API and cfitsio
+        if os.path.exists(os.path.join(IO_PREFIX, "include", "H5Cpp.h")):
+            for src, out in ((CRAWL_SRC, CRAWL), (RENDER_SRC, RENDER)):
+                if os.path.exists(src):
+                    build_io_program(src, out)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return LIB

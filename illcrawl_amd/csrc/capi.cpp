@@ -721,6 +721,14 @@ ilc_status ilc_quantity_scaled(const float* in_dev, double scale, uint64_t n, fl
   });
 }
 
+ilc_status ilc_quantity_stellar_mass(const float* formation_time_dev, const float* mass_dev,
+                                     uint64_t n, float* w_dev, void* stream) {
+  return guarded([&] {
+    ILC_REQUIRE(w_dev && (n == 0 || (formation_time_dev && mass_dev)), "null argument");
+    launch_quantity_stellar_mass(formation_time_dev, mass_dev, n, w_dev, as_stream(stream));
+  });
+}
+
 ilc_status ilc_quantity_xray_bolometric(const float* rho_dev, const float* temp_dev,
                                         const float* xe_dev, uint64_t n,
                                         const float* table_host, int32_t ntab, float x0,

@@ -133,6 +133,16 @@ __global__ __launch_bounds__(kBlock) void k_quantity_scaled(const float* in, dou
   w[i] = in ? static_cast<float>(scale * static_cast<double>(in[i])) : static_cast<float>(scale);
 }
 
+// stellar_mass (quantities.cl:728-750): the mass of star particles, 0 for
+// wind cells (formation time <= 0)
+__global__ __launch_bounds__(kBlock) void k_quantity_stellar_mass(const float* sft,
+                                                                   const float* mass, uint64_t n,
+                                                                   float* w) {
+  const uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+  if (i >= n) return;
+  w[i] = sft[i] > 0.0f ? mass[i] : 0.0f;
+}
+
 constexpr int kMaxTable = 4096;
 
 // read_imagef with a linear, clamp-to-border, unnormalised sampler on a 1-D
@@ -274,6 +284,14 @@ void launch_retrieve(const double* acc, uint64_t n, float scale, float* out, hip
 void launch_quantity_scaled(const float* in, double scale, uint64_t n, float* w, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_quantity_scaled, dim3(nblocks(n)), dim3(kBlock), 0, s, in, scale, n, w);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_quantity_stellar_mass(const float* sft, const float* mass, uint64_t n, float* w,
+                                  hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_quantity_stellar_mass, dim3(nblocks(n)), dim3(kBlock), 0, s, sft, mass, n,
+                     w);
   ILC_HIP(hipGetLastError());
 }
 

@@ -259,3 +259,43 @@ def read_fits(path):
     count = int(np.prod(dims)) if dims else 0
     a = np.frombuffer(raw, dtype=dt, count=count, offset=pos)
     return a.reshape(tuple(reversed(dims))).astype(a.dtype.newbyteorder("="))
+
+
+def read_fits_header(path):
+    """The primary header's keys as {name: value}: standard keys upper-case,
+    long or dotted keys (cfitsio's HIERARCH convention) as written; string
+    values unquoted, numbers as float.  COMMENT cards are returned in order
+    under the key "COMMENT"."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    out, comments, pos = {}, [], 0
+    while pos + 80 <= len(raw):
+        card = raw[pos:pos + 80].decode("ascii")
+        pos += 80
+        if card[:8].strip() == "END":
+            break
+        if card.startswith("COMMENT"):
+            comments.append(card[8:].strip())
+            continue
+        if card.startswith("HIERARCH "):
+            key, _, rest = card[9:].partition("=")
+            key = key.strip()
+        elif card[8:10] == "= ":
+            key, rest = card[:8].strip(), card[10:]
+        else:
+            continue
+        rest = rest.strip()
+        if rest.startswith("'"):
+            end = rest.find("'", 1)
+            while end != -1 and rest[end + 1:end + 2] == "'":  # doubled quote
+                end = rest.find("'", end + 2)
+            out[key] = rest[1:end].replace("''", "'").rstrip()
+        else:
+            v = rest.split("/")[0].strip()
+            try:
+                out[key] = float(v.replace("D", "E"))
+            except ValueError:
+                out[key] = v
+    out["COMMENT"] = comments
+    return out
+

@@ -189,6 +189,14 @@ class camera {
   }
   // camera::get_pixel_area (camera.cpp:191-195)
   double get_pixel_area() const { return get_pixel_size() * get_pixel_size(); }
+  double get_roll() const { return _c.roll; }
+  double get_screen_width() const { return _c.width; }
+  // the screen basis and its lower-left corner (camera::get_screen_basis_vector0,
+  // get_screen_basis_vector1, get_screen_min_position), in double
+  void get_screen_basis(double b0[3], double b1[3], double screen_min[3]) const {
+    double ps = 0.0;
+    check(ilc_camera_basis(&_c, b0, b1, screen_min, &ps), "camera basis");
+  }
 
   // camera::generate_pixel_coordinates (camera.cpp:157-189): world frame
   device_buffer generate_pixel_coordinates(hipStream_t s = nullptr) const {

@@ -1,8 +1,9 @@
 // illcrawl_crawler.hpp -- the data crawler over the MI355X path.
 //
-//   illcrawl::reconstruction_quantity::quantity and three of its kinds
-//                                          quantity.hpp:41-66, 177-228, 323-376,
-//                                          547-605; quantity.cpp:84-131
+//   illcrawl::unit_converter               units.hpp:30-90
+//   illcrawl::reconstruction_quantity::quantity and its projective-smoothing
+//   kinds                                  quantity.hpp:41-66, 177-228, 323-376,
+//                                          547-605, 805-1017; quantity.cpp:84-131
 //   illcrawl::reconstructing_data_crawler  reconstructing_data_crawler.cpp:112-260
 //
 // The crawler streams row blocks of a snapshot group (io::dataset_streamer,
@@ -25,6 +26,29 @@
 #include "illcrawl_io.hpp"
 
 namespace illcrawl {
+
+// Illustris units (comoving ckpc/h, 1e10 Msun/h) to proper kpc and Msun at
+// scale factor a and Hubble parameter h (units.hpp:30-90).
+class unit_converter {
+ public:
+  unit_converter(double a = 1.0, double h = 1.0) : _a(a), _h(h) {}
+  double length_conversion_factor() const { return _a / _h; }
+  double mass_conversion_factor() const { return 1.e10 / _h; }
+  double volume_conversion_factor() const { return _a * _a * _a / (_h * _h * _h); }
+  double density_conversion_factor() const {
+    return mass_conversion_factor() / volume_conversion_factor();
+  }
+  double area_conversion_factor() const {
+    return length_conversion_factor() * length_conversion_factor();
+  }
+  double potential_conversion_factor() const { return 1. / _a; }
+  double get_scale_factor() const { return _a; }
+  double get_hubble_parameter() const { return _h; }
+
+ private:
+  double _a, _h;
+};
+
 namespace reconstruction_quantity {
 
 // A reconstructed quantity: the particle group and datasets it reads, their
@@ -45,7 +69,24 @@ class quantity {
                          hipStream_t s) const = 0;
   virtual double effective_line_of_sight_integration_dA(double dA, double range) const = 0;
   virtual double effective_volume_integration_dV(double dV, double volume) const = 0;
+  // the units of the pixel area and range passed to the dA above
+  // (projective_smoothing::retrieve_results); identity unless set
+  const unit_converter& get_unit_converter() const { return _units; }
+  void set_unit_converter(const unit_converter& u) { _units = u; }
+
+ private:
+  unit_converter _units;
 };
+
+// The effective dA of a projection as dm::projective_smoothing applies it
+// (projective_smoothing_reconstruction.cpp:49-84): the FP32 pixel area times
+// the area conversion, the depth times the FP32 length conversion.
+inline double projected_dA(const quantity& q, double pixel_area_f32, double depth) {
+  const unit_converter& u = q.get_unit_converter();
+  const double lc = static_cast<double>(static_cast<float>(u.length_conversion_factor()));
+  return q.effective_line_of_sight_integration_dA(pixel_area_f32 * u.area_conversion_factor(),
+                                                  lc * depth);
+}
 
 // the gas density as is (unprocessed_quantity, quantities.cl:701-711)
 class density_based_quantity : public quantity {
@@ -133,6 +174,92 @@ class xray_flux : public quantity {
   double _z, _dl, _emin, _emax;
   unsigned _ns;
   std::vector<double> _scale;
+};
+
+// Dark matter (PartType1): every particle carries the same mass, given in
+// 1e10 Msun/h and converted to Msun per comoving volume unit (the smoothing
+// sees ckpc/h coordinates), as constant_quantity (quantity.hpp:805-857,
+// quantities.cl:713-726).
+class dm_quantity : public quantity {
+ public:
+  dm_quantity(const unit_converter& converter, double dm_particle_mass)
+      : _m(converter.mass_conversion_factor() * dm_particle_mass) {
+    _m /= converter.volume_conversion_factor();
+    set_unit_converter(converter);
+  }
+  unsigned get_particle_type_id() const override { return 1; }
+  std::vector<std::string> get_required_datasets() const override { return {}; }
+  std::vector<double> get_quantity_scaling_factors() const override { return {}; }
+  void transform(const float* const*, std::size_t n, float* w_dev, hipStream_t s) const override {
+    // the mass is a kernel argument of type device_scalar
+    check(ilc_quantity_scaled(nullptr, _m, n, w_dev, s), "quantity transformation");
+  }
+  double get_dm_particle_mass() const { return _m; }
+
+ private:
+  double _m;
+};
+
+// the mean dark matter density along the line of sight (quantity.hpp:859-883)
+class dm_density : public dm_quantity {
+ public:
+  using dm_quantity::dm_quantity;
+  double effective_volume_integration_dV(double dV, double volume) const override {
+    return dV / volume;
+  }
+  double effective_line_of_sight_integration_dA(double, double range) const override {
+    return 1.0 / range;
+  }
+};
+
+// the dark matter mass per pixel (quantity.hpp:885-912)
+class dm_mass : public dm_quantity {
+ public:
+  using dm_quantity::dm_quantity;
+  double effective_volume_integration_dV(double dV, double) const override { return dV; }
+  double effective_line_of_sight_integration_dA(double dA, double) const override { return dA; }
+};
+
+// Stars (PartType4): the particle mass where GFM_StellarFormationTime > 0,
+// 0 for wind cells; the masses scaled to Msun per comoving volume unit
+// (quantity.hpp:918-1017, quantities.cl:728-750).
+class stellar_density : public quantity {
+ public:
+  explicit stellar_density(const unit_converter& converter)
+      : _mass_conversion(converter.mass_conversion_factor() /
+                         converter.volume_conversion_factor()) {
+    set_unit_converter(converter);
+  }
+  unsigned get_particle_type_id() const override { return 4; }
+  std::vector<std::string> get_required_datasets() const override {
+    return {"GFM_StellarFormationTime", io::illustris_data_loader::get_masses_identifier()};
+  }
+  std::vector<double> get_quantity_scaling_factors() const override {
+    return {1.0, _mass_conversion};
+  }
+  void transform(const float* const* in_dev, std::size_t n, float* w_dev,
+                 hipStream_t s) const override {
+    check(ilc_quantity_stellar_mass(in_dev[0], in_dev[1], n, w_dev, s),
+          "quantity transformation");
+  }
+  double get_mass_conversion_factor() const { return _mass_conversion; }
+  double effective_volume_integration_dV(double dV, double volume) const override {
+    return dV / volume;
+  }
+  double effective_line_of_sight_integration_dA(double, double range) const override {
+    return 1.0 / range;
+  }
+
+ private:
+  double _mass_conversion;
+};
+
+// the stellar mass per pixel (quantity.hpp:993-1017)
+class stellar_mass : public stellar_density {
+ public:
+  using stellar_density::stellar_density;
+  double effective_volume_integration_dV(double dV, double) const override { return dV; }
+  double effective_line_of_sight_integration_dA(double dA, double) const override { return dA; }
 };
 
 }  // namespace reconstruction_quantity

@@ -7,6 +7,9 @@
 //                                          reconstructing_data_crawler::run
 //                                          (reconstructing_data_crawler.cpp:146-178)
 //   illcrawl::io::fits<T>                 fits.hpp:210-300 (cfitsio)
+//   illcrawl::io::fits_header             fits.hpp:142-201
+//   illcrawl::io::load_data_table         util::data_table_loader
+//                                          (data_table_loader.hpp:35-97)
 //
 // Link with -lhdf5_cpp -lhdf5 -lcfitsio (the image has them under /opt/conda).
 #ifndef ILLCRAWL_IO_HPP
@@ -17,7 +20,10 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdlib>
+#include <fstream>
 #include <future>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -107,24 +113,29 @@ class dataset_streamer {
 
   // rows [first, first + count) of every dataset
   block read(std::size_t first, std::size_t count) const {
-    if (first + count > _rows) throw std::runtime_error("dataset_streamer: rows out of range");
     block out(_ds.size());
     std::vector<double> tmp;
     for (std::size_t d = 0; d < _ds.size(); ++d) {
-      const hsize_t cols = _cols[d];
-      tmp.resize(count * cols);
-      out[d].resize(count * cols);
-      if (count) {
-        H5::DataSpace fs = _ds[d].getSpace();
-        const int rank = fs.getSimpleExtentNdims();
-        hsize_t off[2] = {first, 0}, cnt[2] = {count, cols};
-        fs.selectHyperslab(H5S_SELECT_SET, cnt, off);
-        H5::DataSpace ms(rank, cnt);
-        _ds[d].read(tmp.data(), H5::PredType::NATIVE_DOUBLE, ms, fs);
-        for (std::size_t k = 0; k < tmp.size(); ++k) out[d][k] = static_cast<float>(tmp[k]);
-      }
+      read_f64(d, first, count, tmp);
+      out[d].resize(tmp.size());
+      for (std::size_t k = 0; k < tmp.size(); ++k) out[d][k] = static_cast<float>(tmp[k]);
     }
     return out;
+  }
+
+  // rows [first, first + count) of one dataset as double, row-major
+  void read_f64(std::size_t d, std::size_t first, std::size_t count,
+                std::vector<double>& out) const {
+    if (first + count > _rows) throw std::runtime_error("dataset_streamer: rows out of range");
+    const hsize_t cols = _cols.at(d);
+    out.resize(count * cols);
+    if (!count) return;
+    H5::DataSpace fs = _ds[d].getSpace();
+    const int rank = fs.getSimpleExtentNdims();
+    hsize_t off[2] = {first, 0}, cnt[2] = {count, cols};
+    fs.selectHyperslab(H5S_SELECT_SET, cnt, off);
+    H5::DataSpace ms(rank, cnt);
+    _ds[d].read(out.data(), H5::PredType::NATIVE_DOUBLE, ms, fs);
   }
 
   // f(first_row, block) for the blocks of blocksize rows, in order; block
@@ -219,6 +230,97 @@ class fits {
  private:
   std::string _filename;
 };
+
+// Keys of a FITS file's primary header, written as the reference writes
+// them: a COMMENT card with the comment, then the key (cfitsio uses the
+// HIERARCH convention for the long dotted names).
+class fits_header {
+ public:
+  explicit fits_header(const std::string& filename) : _filename(filename) {
+    int status = 0;
+    if (fits_open_file(&_file, _filename.c_str(), READWRITE, &status))
+      throw std::runtime_error("fits_header: Could not open fits file " + filename);
+  }
+  fits_header(const fits_header&) = delete;
+  fits_header& operator=(const fits_header&) = delete;
+  ~fits_header() {
+    int status = 0;
+    if (_file) fits_close_file(_file, &status);
+  }
+
+  void add_entry(const std::string& name, double value, const std::string& comment = "") {
+    write(name, TDOUBLE, &value, comment, false);
+  }
+  void add_entry(const std::string& name, int value, const std::string& comment = "") {
+    write(name, TINT, &value, comment, false);
+  }
+  void add_entry(const std::string& name, unsigned value, const std::string& comment = "") {
+    write(name, TUINT, &value, comment, false);
+  }
+  void add_entry(const std::string& name, unsigned long long value,
+                 const std::string& comment = "") {
+    write(name, TULONGLONG, &value, comment, false);
+  }
+  // strings carry the comment on the key card as well
+  void add_entry(const std::string& name, const std::string& value,
+                 const std::string& comment = "") {
+    write(name, TSTRING, const_cast<char*>(value.c_str()), comment, true);
+  }
+  void add_entry(const std::string& name, const char* value, const std::string& comment = "") {
+    add_entry(name, std::string(value), comment);
+  }
+
+ private:
+  void write(const std::string& name, int type, void* value, const std::string& comment,
+             bool key_comment) {
+    int status = 0;
+    fits_write_comment(_file, comment.c_str(), &status);
+    status = 0;
+    if (fits_write_key(_file, type, name.c_str(), value,
+                       key_comment ? comment.c_str() : nullptr, &status)) {
+      char text[FLEN_STATUS] = {};
+      fits_get_errstatus(status, text);
+      throw std::runtime_error("Error while updating fits key: " + std::string(text));
+    }
+  }
+  std::string _filename;
+  fitsfile* _file = nullptr;
+};
+
+// A whitespace-separated numeric table in a text file (lines starting with
+// '#' and empty lines skipped, every row the same length), as
+// util::data_table_loader reads gaunt_factor.dat: row-major values, cols
+// entries per row.  Values parse to float with correct rounding.
+inline std::vector<float> load_data_table(const std::string& filename, std::size_t& cols,
+                                          std::size_t& rows) {
+  std::ifstream file(filename);
+  if (!file.is_open()) throw std::runtime_error("Could not load data table from file: " + filename);
+  std::vector<float> data;
+  std::string line;
+  cols = 0;
+  rows = 0;
+  while (std::getline(file, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream in(line);
+    std::string tok;
+    std::size_t n = 0;
+    while (in >> tok) {
+      char* end = nullptr;
+      const float v = std::strtof(tok.c_str(), &end);
+      if (end == tok.c_str() || *end != '\0')
+        throw std::runtime_error("Could not convert entry in data table " + filename + ": " + tok);
+      data.push_back(v);
+      ++n;
+    }
+    if (n == 0) continue;
+    if (cols != 0 && n != cols)
+      throw std::runtime_error("Invalid row in file " + filename + ": expected " +
+                               std::to_string(cols) + " entries, got " + std::to_string(n));
+    cols = n;
+    ++rows;
+  }
+  return data;
+}
 
 }  // namespace io
 }  // namespace illcrawl

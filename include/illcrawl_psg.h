@@ -360,6 +360,13 @@ ilc_status ilc_quantity_xray_spectral_flux(const float* rho_dev, const float* u_
 ilc_status ilc_quantity_scaled(const float* in_dev, double scale, uint64_t n, float* w_dev,
                                void* stream);
 
+/* quantity_transformation with stellar_mass (quantity.hpp:950-1017,
+ * quantities.cl:728-750): w[i] = mass[i] where the formation time is > 0 (a
+ * star), 0 where it is not (a wind cell); the inputs as the crawler scaled
+ * them (formation time x 1, mass x the mass-per-volume unit factor). */
+ilc_status ilc_quantity_stellar_mass(const float* formation_time_dev, const float* mass_dev,
+                                     uint64_t n, float* w_dev, void* stream);
+
 /* Bolometric free-free emissivity w = n_e n_i sqrt(T) g(T), n_e = n_i =
  * 4.046263 rho x_e (the ion density convention of quantities.cl:112), g read
  * from a 1-D table (ntab floats, node i at x0 + i*dx on ln T) with the

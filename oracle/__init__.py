@@ -20,7 +20,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ILC_ORACLE_LIB selects another build of oracle.c (tests/sanitized_run.py)
+_LIB_PATH = os.environ.get("ILC_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 K_QUARTIC_REF = 0
 K_QUARTIC = 1

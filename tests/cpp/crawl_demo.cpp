@@ -4,7 +4,11 @@
 //       <dir>/{pos.f64 (n*3), rho.f32, u.f32, xe.f32, mass.f32, hsml.f32}
 //       -> <dir>/snap.hdf5, group PartType0 (Coordinates float64 n x 3,
 //       Density, InternalEnergy, ElectronAbundance, Masses, SmoothingLength
-//       float32), the Illustris layout
+//       float32), the Illustris layout; with <dir>/subfind_hsml.f32 also
+//       PartType0/SubfindHsml; with <dir>/dm_pos.f64 (n1*3) and dm_hsml.f32
+//       group PartType1 (Coordinates, SubfindHsml); with <dir>/star_pos.f64
+//       (n4*3), star_hsml.f32, star_sft.f32 and star_mass.f32 group PartType4
+//       (Coordinates, SubfindHsml, GFM_StellarFormationTime, Masses)
 //   crawl_demo write-fits <raw.f32> <nx> <ny> <out.fits>   (cfitsio; checks its own read back)
 //   crawl_demo fits-to-raw <in.fits> <out.f32>              (cfitsio read; prints the extents)
 //   crawl_demo render <dir> <blocksize> <quantity: density|mass|xray> <nx> <width> <box>
@@ -39,6 +43,14 @@ std::vector<T> read_raw(const std::string& path, std::size_t n) {
   return v;
 }
 
+bool exists(const std::string& path) { return std::ifstream(path).good(); }
+
+std::size_t file_size(const std::string& path) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  return static_cast<std::size_t>(f.tellg());
+}
+
 template <class T>
 void write_dataset(H5::Group& g, const std::string& name, const std::vector<T>& v, hsize_t rows,
                    hsize_t cols, const H5::PredType& type) {
@@ -59,6 +71,25 @@ int write_snapshot(const std::string& dir, std::size_t n) {
   for (int k = 0; k < 5; ++k)
     write_dataset(g, names[k], read_raw<float>(dir + "/" + files[k] + ".f32", n), n, 1,
                   H5::PredType::NATIVE_FLOAT);
+  if (exists(dir + "/subfind_hsml.f32"))
+    write_dataset(g, "SubfindHsml", read_raw<float>(dir + "/subfind_hsml.f32", n), n, 1,
+                  H5::PredType::NATIVE_FLOAT);
+  // other particle groups: rows from the size of the coordinate file
+  auto group = [&](const std::string& name, const std::string& prefix,
+                   const std::vector<std::pair<std::string, std::string>>& cols) {
+    const std::string pos = dir + "/" + prefix + "_pos.f64";
+    if (!exists(pos)) return;
+    const std::size_t m = file_size(pos) / (3 * sizeof(double));
+    H5::Group gg = f.createGroup(name);
+    write_dataset(gg, "Coordinates", read_raw<double>(pos, 3 * m), m, 3,
+                  H5::PredType::NATIVE_DOUBLE);
+    for (const auto& c : cols)
+      write_dataset(gg, c.first, read_raw<float>(dir + "/" + prefix + "_" + c.second + ".f32", m),
+                    m, 1, H5::PredType::NATIVE_FLOAT);
+  };
+  group("PartType1", "dm", {{"SubfindHsml", "hsml"}});
+  group("PartType4", "star",
+        {{"SubfindHsml", "hsml"}, {"GFM_StellarFormationTime", "sft"}, {"Masses", "mass"}});
   return 0;
 }
 
