@@ -165,6 +165,9 @@ struct ilc_batch {
   dbuf<uint2> items_dev;
   pinned<uint2> items_host;
   uint32_t nitems = 0;
+  // the items as the gather takes them: every queue heaviest tiles first
+  dbuf<uint2> items_ord;
+  dbuf<uint32_t> item_keys, item_idx, item_temp;
   queue_segs segs{};
   dbuf<uint32_t> qcounters;  // per-queue work counters (kQueueStride apart)
   dbuf<uint32_t> counter;  // [0] unused, [1] out-of-grid particles
@@ -489,6 +492,10 @@ void plan(ilc_batch* b, hipStream_t s) {
   b->segs.seg[kQueues] = static_cast<uint32_t>(k);
   b->nitems = static_cast<uint32_t>(nitems);
   b->items_dev.ensure(nitems);
+  b->items_ord.ensure(nitems);
+  b->item_keys.ensure(nitems);
+  b->item_idx.ensure(nitems);
+  b->item_temp.ensure(radix_temp_words(nitems));
   b->desc_dev.ensure(nv);
   b->desc_host.ensure(nv);
   b->counter.ensure(2);
@@ -738,8 +745,10 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
                            hipMemcpyHostToDevice, s));
     ILC_HIP(hipMemcpyAsync(b->items_dev.p, b->items_host.p, sizeof(uint2) * b->nitems,
                            hipMemcpyHostToDevice, s));
+    launch_order_items(b->desc_dev.p, b->items_dev.p, b->segs, b->nitems, b->ts, b->item_keys.p,
+                       b->item_idx.p, b->item_temp.p, b->items_ord.p, s);
     if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
-    launch_gather_batch(b->desc_dev.p, b->items_dev.p, b->segs, b->qcounters.p, op, b->kernel,
+    launch_gather_batch(b->desc_dev.p, b->items_ord.p, b->segs, b->qcounters.p, op, b->kernel,
                         b->ts, b->cubic.p, b->grid_blocks, s);
     if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
     for (size_t v = 0; v < nv; ++v) {

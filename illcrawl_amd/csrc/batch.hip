@@ -672,6 +672,70 @@ __global__ __launch_bounds__(kBlock) void k_gather_u32(const uint32_t* __restric
   if (i < n) out[i] = src[idx[i]];
 }
 
+// Work-item order of the persistent gather.  A tile's cost is dominated by
+// the particles of the grid cells under its points: a tile over a dense core
+// streams and evaluates the most, and started last it runs on alone while
+// the rest of the GPU idles.  Each item gets the key queue << 5 | class,
+// class = 31 - bit length of that particle count (capped), so that a stable
+// sort by key keeps every queue's items together, heaviest class first,
+// tiles of one class in their original (neighbouring) order.  The count
+// visits at most 32 rows (a stride over taller boxes) and 8 layers: it is
+// an estimate.
+__global__ __launch_bounds__(kBlock) void k_item_cost(const view_desc* __restrict__ views,
+                                                      const uint2* __restrict__ items,
+                                                      queue_segs segs, uint32_t n, int tw, int th,
+                                                      uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t q = 0;
+  while (q + 1 < uint32_t(kQueues) && i >= segs.seg[q + 1]) ++q;
+  const uint2 it = items[i];
+  const view_desc& vd = views[it.x];
+  const grid_view& g = vd.g;
+  uint64_t cost = 0;
+  const uint32_t tx = it.y % vd.tiles_x, ty = it.y / vd.tiles_x;
+  const uint32_t px0 = tx * uint32_t(tw), py0 = ty * uint32_t(th);
+  if (px0 < vd.p.lnx && py0 < vd.p.lny && g.nc[0] > 0 && g.nc[1] > 0 && g.nc[2] > 0) {
+    const uint32_t px1 = min(px0 + uint32_t(tw), vd.p.lnx) - 1u;
+    const uint32_t py1 = min(py0 + uint32_t(th), vd.p.lny) - 1u;
+    const uint64_t ia = uint64_t(py0) * vd.p.lnx + px0, ib = uint64_t(py1) * vd.p.lnx + px1;
+    if (ib < vd.p.n) {
+      const float4 a = vd.p.pts[ia], b = vd.p.pts[ib];
+      const float x0 = fminf(a.x, b.x), x1 = fmaxf(a.x, b.x);
+      const float y0 = fminf(a.y, b.y), y1 = fmaxf(a.y, b.y);
+      const float fx0 = floorf((x0 - g.minc[0]) / g.cs[0]), fx1 = floorf((x1 - g.minc[0]) / g.cs[0]);
+      const float fy0 = floorf((y0 - g.minc[1]) / g.cs[1]), fy1 = floorf((y1 - g.minc[1]) / g.cs[1]);
+      // the whole box off the grid (or not a number): no cells
+      if (fx1 >= 0.0f && fy1 >= 0.0f && fx0 < float(g.nc[0]) && fy0 < float(g.nc[1])) {
+        const int cx0 = int(fmaxf(fx0, 0.0f)), cx1 = int(fminf(fx1, float(g.nc[0] - 1)));
+        const int cy0 = int(fmaxf(fy0, 0.0f)), cy1 = int(fminf(fy1, float(g.nc[1] - 1)));
+        const int nrows = cy1 - cy0 + 1;
+        const int step = (nrows + 31) / 32;
+        const int nl = min(g.nc[2], 8);
+        const uint32_t nx = uint32_t(g.nc[0]), nxny = nx * uint32_t(g.nc[1]);
+        for (int cz = 0; cz < nl; ++cz)
+          for (int cy = cy0; cy <= cy1; cy += step) {
+            const uint32_t kb = uint32_t(cy) * nx + uint32_t(cz) * nxny;
+            cost += uint64_t(g.start[kb + uint32_t(cx1) + 1u] - g.start[kb + uint32_t(cx0)]) *
+                    uint64_t(step);
+          }
+      }
+    }
+  }
+  const uint32_t c32 = cost > 0xffffffffull ? 0xffffffffu : uint32_t(cost);
+  const int bits = c32 ? 32 - __clz(c32) : 0;
+  keys[i] = (q << 5) | uint32_t(31 - min(bits, 31));
+  idx[i] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_permute_items(const uint2* __restrict__ items,
+                                                          const uint32_t* __restrict__ idx,
+                                                          uint32_t n, uint2* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = items[idx[i]];
+}
+
 // a row's view is the last v with cbase[v] <= its global key (every view has
 // at least one cell, so the bases rise strictly)
 __global__ __launch_bounds__(kBlock) void k_local_keys(const uint32_t* __restrict__ gkey,
@@ -842,6 +906,20 @@ void launch_local_keys(const uint32_t* gkey, const uint32_t* cbase, int nv, uint
   ILC_REQUIRE(nv >= 1, "no views");
   hipLaunchKernelGGL(k_local_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, gkey,
                      cbase, nv, n, lkey);
+  ILC_HIP(hipGetLastError());
+}
+
+void launch_order_items(const view_desc* views, const uint2* items, const queue_segs& segs,
+                        uint32_t n, const tile_shape& ts, uint32_t* keys, uint32_t* idx,
+                        uint32_t* radix_temp, uint2* out, hipStream_t s) {
+  if (!n) return;
+  const int th = (kBlock / ts.tw) * ts.ppt;
+  hipLaunchKernelGGL(k_item_cost, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, views,
+                     items, segs, n, ts.tw, th, keys, idx);
+  ILC_HIP(hipGetLastError());
+  launch_radix_sort(keys, idx, n, 8, radix_temp, s);
+  hipLaunchKernelGGL(k_permute_items, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, items,
+                     idx, n, out);
   ILC_HIP(hipGetLastError());
 }
 

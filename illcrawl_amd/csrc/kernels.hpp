@@ -240,6 +240,12 @@ constexpr int kQueueStride = 64;
 struct queue_segs {
   uint32_t seg[kQueues + 1];
 };
+// The persistent gather's items reordered inside every queue: tiles with the
+// most particles under them first (stable inside a cost class).  keys, idx:
+// n words each; radix_temp: radix_temp_words(n); out: n items.
+void launch_order_items(const view_desc* views, const uint2* items, const queue_segs& segs,
+                        uint32_t n, const tile_shape& ts, uint32_t* keys, uint32_t* idx,
+                        uint32_t* radix_temp, uint2* out, hipStream_t s);
 void launch_gather_batch(const view_desc* views, const uint2* items, const queue_segs& segs,
                          uint32_t* counters, int op, int kernel, const tile_shape& ts,
                          const float* cubic_table, int grid_blocks, hipStream_t s);
