@@ -50,13 +50,21 @@ FLOP_PER_EVAL_3D = 13
 # HBM bytes per k_gather_batch launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs of this workload scaled by a 1 GiB copy (profiles/k_gather_batch_xray_16M_64views.md);
 # keyed by (workload, mode, tile)
-MEASURED_TRAFFIC = {("c3", "reference", (16, 16)): dict(read=19.30e9, write=1.781e9),
+MEASURED_TRAFFIC = {("c3", "reference", (16, 16, 256)): dict(read=19.30e9, write=1.781e9),
                     # k_smooth3d, profiles/k_smooth3d_slices_16M.md
                     ("v1", "reference", (16, 16)): dict(read=1.244e9, write=0.461e9)}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def parse_tile(text):
+    """'WxH' (one tile per 256-thread workgroup) or 'WxH/G' (G threads per
+    workgroup: 256, or 64 for one-wave 16x16 tiles) -> (W, H, G)."""
+    shape, _, wg = text.partition("/")
+    w, h = (int(x) for x in shape.split("x"))
+    return w, h, int(wg) if wg else 256
 
 
 KINDS = {"quartic_ref": 0, "quartic": 1, "cubic": 2}
@@ -347,7 +355,7 @@ def batch_adjoint_main(args):
     r = B.BatchRenderer(mode="exact")
     r.set_gaunt_table(tab, x0, dx)
     if args.tile:
-        r.set_tile(*(int(x) for x in args.tile.split("x")))
+        r.set_tile(*parse_tile(args.tile))
     r.set_particles(snap.pos, snap.mass, snap.hsml, rho_t, snap.temp, snap.xe)
     r.set_views(views)
     r.render()
@@ -574,7 +582,9 @@ def main():
     ap.add_argument("--kernel", default=None, choices=["quartic", "quartic_ref", "cubic"],
                     help="projected kernel: default quartic_ref (the reference's), quartic "
                          "for c4; a kernel other than quartic_ref renders in exact mode")
-    ap.add_argument("--tile", default=None, help="WxH gather tile; default: autotuned")
+    ap.add_argument("--tile", default=None,
+                    help="WxH[/G] gather tile (G threads per workgroup, 256 or 64); "
+                         "default: autotuned")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
@@ -628,11 +638,12 @@ def main():
         desc = dict(desc, depth_cut="line-of-sight kernel clip")
     r.set_views(views)
     if args.tile:
-        w, h = (int(x) for x in args.tile.split("x"))
-        r.set_tile(w, h)
-        tile, tuned = (w, h), None
+        tile = parse_tile(args.tile)
+        r.set_tile(*tile)
+        tuned = None
     else:
-        tile, best_ms = r.autotune()
+        _, best_ms = r.autotune()
+        tile = r.tile()
         tuned = dict(tile=list(tile), best_ms=best_ms)
         log(f"[bench] autotuned tile {tile} ({best_ms:.1f} ms)")
     if args.breakdown:

@@ -132,6 +132,9 @@ SIGNATURES = {
     "ilc_batch_view_info": (_ST, [_P, ctypes.c_int32, ctypes.POINTER(GridInfo)]),
     "ilc_batch_backward": (_ST, [_P, _P, _P, _P, _P, _P]),
     "ilc_batch_set_tile": (_ST, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "ilc_batch_set_tile_wg": (_ST, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "ilc_batch_get_tile": (_ST, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.POINTER(ctypes.c_int32)]),
     "ilc_batch_set_depth_clip": (_ST, [_P, ctypes.c_int32]),
     "ilc_batch_last_gather_ms": (_ST, [_P, ctypes.POINTER(ctypes.c_float)]),
     "ilc_batch_autotune": (_ST, [_P, ctypes.POINTER(ctypes.c_int32),

@@ -598,10 +598,13 @@ class BatchRenderer:
         cfg.target_particles_per_cell = int(target_particles_per_cell)
         cfg.mode = _MODES[mode]
         cfg.kernel = _KERNELS[kernel]
-        if tile is not None:
+        # (w, h) at creation; (w, h, workgroup) through set_tile
+        if tile is not None and len(tile) == 2:
             cfg.tile_w, cfg.tile_h = int(tile[0]), int(tile[1])
         self._h = ctypes.c_void_p()
         N.call("ilc_batch_create", ctypes.byref(cfg), ctypes.byref(self._h))
+        if tile is not None and len(tile) > 2:
+            self.set_tile(*tile)
         self.device = torch.device("cuda", int(device))
         self._keep = []
         self.views = []
@@ -689,8 +692,16 @@ class BatchRenderer:
                     cell_size=list(gi.cell_size), max_h=gi.max_h,
                     num_particles=gi.num_particles)
 
-    def set_tile(self, w, h):
-        N.call("ilc_batch_set_tile", self._h, int(w), int(h))
+    def set_tile(self, w, h, workgroup=256):
+        """Gather tile of w x h points, one tile per workgroup of `workgroup`
+        threads (256, or 64 for one-wave 16x16 tiles)."""
+        N.call("ilc_batch_set_tile_wg", self._h, int(w), int(h), int(workgroup))
+
+    def tile(self):
+        """(tile_w, tile_h, workgroup) of the gather."""
+        w, h, g = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        N.call("ilc_batch_get_tile", self._h, ctypes.byref(w), ctypes.byref(h), ctypes.byref(g))
+        return w.value, h.value, g.value
 
     def set_depth_clip(self, clip=True):
         """Line-of-sight kernel clip of every view's depth range instead of the

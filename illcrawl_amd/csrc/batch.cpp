@@ -456,7 +456,7 @@ void plan(ilc_batch* b, hipStream_t s) {
   b->sh_ncells = ncells;
   sig.push_back(b->shared ? ntot : 0);
   sig.push_back(uint64_t(b->clip));
-  sig.push_back(uint64_t(b->ts.tw) << 32 | uint64_t(b->ts.ppt));
+  sig.push_back(uint64_t(b->ts.tw) << 40 | uint64_t(b->ts.nb) << 20 | uint64_t(b->ts.ppt));
   // work items: views with the coarsest pixels first (their tiles cover the
   // most area, so over dense regions they stream the most particles and run
   // longest: started first, they finish while the finer views' many light
@@ -1075,15 +1075,32 @@ ilc_status ilc_batch_set_depth_clip(ilc_batch* b, int32_t clip) {
 }
 
 ilc_status ilc_batch_set_tile(ilc_batch* b, int32_t tile_w, int32_t tile_h) {
+  return ilc_batch_set_tile_wg(b, tile_w, tile_h, kBlock);
+}
+
+ilc_status ilc_batch_set_tile_wg(ilc_batch* b, int32_t tile_w, int32_t tile_h,
+                                 int32_t workgroup) {
   return guarded([&] {
     ILC_REQUIRE(b, "null batch");
-    ILC_REQUIRE(tile_w > 0 && tile_h > 0 && (tile_w * tile_h) % kBlock == 0,
-                "tile %dx%d is not a multiple of %d points", tile_w, tile_h, kBlock);
-    tile_shape ts{tile_w, tile_w * tile_h / kBlock};
+    ILC_REQUIRE(workgroup == 64 || workgroup == 256, "workgroup of %d threads (64 or 256)",
+                workgroup);
+    ILC_REQUIRE(tile_w > 0 && tile_h > 0 && (tile_w * tile_h) % workgroup == 0,
+                "tile %dx%d is not a multiple of %d points", tile_w, tile_h, workgroup);
+    tile_shape ts{tile_w, tile_w * tile_h / workgroup, workgroup};
     (void)tile_grid;  // validated on use
     b->ts = ts;
     b->graph_valid = false;
     b->ran_eagerly = false;
+  });
+}
+
+ilc_status ilc_batch_get_tile(const ilc_batch* b, int32_t* tile_w, int32_t* tile_h,
+                              int32_t* workgroup) {
+  return guarded([&] {
+    ILC_REQUIRE(b, "null batch");
+    if (tile_w) *tile_w = b->ts.tw;
+    if (tile_h) *tile_h = (b->ts.nb / b->ts.tw) * b->ts.ppt;
+    if (workgroup) *workgroup = b->ts.nb;
   });
 }
 
@@ -1093,7 +1110,9 @@ ilc_status ilc_batch_autotune(ilc_batch* b, int32_t* tile_w, int32_t* tile_h, do
     require_ready(b);
     device_guard dg(b->device);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    static const tile_shape cand[] = {{16, 1}, {32, 2}, {32, 4}, {64, 4}};
+    // one tile per workgroup of 256 threads, and 16x16 tiles of one wave
+    static const tile_shape cand[] = {{16, 1, 256}, {32, 2, 256}, {32, 4, 256}, {64, 4, 256},
+                                      {16, 4, 64},  {8, 1, 64}};
     ensure_cubic(b);  // the cubic gather reads its table from the first launch
     struct events {
       hipEvent_t e[2] = {nullptr, nullptr};
@@ -1139,7 +1158,7 @@ ilc_status ilc_batch_autotune(ilc_batch* b, int32_t* tile_w, int32_t* tile_h, do
     b->graph_valid = false;
     b->ran_eagerly = false;
     if (tile_w) *tile_w = best.tw;
-    if (tile_h) *tile_h = (kBlock / best.tw) * best.ppt;
+    if (tile_h) *tile_h = (best.nb / best.tw) * best.ppt;
     if (best_ms) *best_ms = best_t;
   });
 }

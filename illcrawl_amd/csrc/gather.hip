@@ -55,13 +55,18 @@ namespace ilc {
 
 namespace {
 
-constexpr int kCap = 320;    // list capacity (entries)
-constexpr int kFlush = 64;   // evaluate when a list reaches this size
+// a workgroup of nb threads evaluates a list when it reaches flush_of(nb)
+// entries (one-wave workgroups: fewer, to keep their LDS small)
+constexpr int flush_of(int nb) { return nb >= 256 ? 64 : 32; }
+// list capacity (entries): a batch of nb candidates lands on a list holding
+// at most flush_of(nb) - 1
+constexpr int cap_of(int nb) { return nb >= 256 ? 320 : 96; }
+constexpr int kCap = cap_of(256);
 // a tile of a sparse grid leaves the row streaming for the cell enumeration
 // above this many particles per candidate cell
 constexpr uint32_t kDenseRows = 2;
-// a batch of kBlock candidates lands on a list holding at most kFlush - 1
-static_assert(kFlush - 1 + kBlock <= kCap, "list capacity");
+static_assert(flush_of(256) - 1 + 256 <= cap_of(256) && flush_of(64) - 1 + 64 <= cap_of(64),
+              "list capacity");
 constexpr int kCubicTable = 4096;
 
 __device__ inline int32_t floor_to_int(float f) {
@@ -105,8 +110,9 @@ __device__ inline uint32_t prefix_count(uint64_t mask) {
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 
-// Exclusive scan over the 256 threads of the workgroup; *total = sum.
+// Exclusive scan over the NB threads of the workgroup; *total = sum.
 // Contains two workgroup barriers.
+template <int NB>
 __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* ws) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t incl = v;
@@ -117,7 +123,7 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t
   if (lane == 63) ws[wave] = incl;
   __syncthreads();
   uint32_t off = 0, tot = 0;
-  for (int w = 0; w < 4; ++w) {
+  for (int w = 0; w < NB / 64; ++w) {
     if (w < wave) off += ws[w];
     tot += ws[w];
   }
@@ -177,46 +183,51 @@ struct adjoint_io {
 };
 
 // the small and masked lists share one array from its two ends: before a
-// batch of kBlock candidates each holds at most kFlush - 1 entries
-constexpr int kPool = kCap + kFlush;
-static_assert(2 * (kFlush - 1) + kBlock <= kPool, "pool capacity");
+// batch of nb candidates each holds at most flush_of(nb) - 1 entries
+constexpr int pool_of(int nb) { return cap_of(nb) + flush_of(nb); }
+static_assert(2 * (flush_of(256) - 1) + 256 <= pool_of(256) &&
+                  2 * (flush_of(64) - 1) + 64 <= pool_of(64),
+              "pool capacity");
 
-template <bool REF, int KERNEL, bool ADJ = false, bool CLIP = false>
+template <bool REF, int KERNEL, bool ADJ = false, bool CLIP = false, int NB = 256>
 struct gather_smem {
-  float4 Lb[KERNEL == 0 && !ADJ ? kCap : 1];  // big: expansion coefficients
+  static constexpr int CAP = cap_of(NB), POOL = pool_of(NB);
+  float4 Lb[KERNEL == 0 && !ADJ ? CAP : 1];  // big: expansion coefficients
   // small (position form) from the front; reference mode: masked (position
-  // form) from the back, the m-th at Ls[kPool - 1 - m]
-  float4 Ls[REF ? kPool : kCap];
-  alignas(16) int Lmc[REF ? kCap : 1];  // masked: packed cell (cx | cy << 16)
+  // form) from the back, the m-th at Ls[POOL - 1 - m]
+  float4 Ls[REF ? POOL : CAP];
+  alignas(16) int Lmc[REF ? CAP : 1];  // masked: packed cell (cx | cy << 16)
   // line-of-sight clip: per big / small entry, the depth range relative to
   // the particle in units of h, times c^(1/2) (see clip_pow)
-  float2 Lbz[CLIP ? kCap : 1];
-  float2 Lsz[CLIP ? kCap : 1];
+  float2 Lbz[CLIP ? CAP : 1];
+  float2 Lsz[CLIP ? CAP : 1];
   uint32_t wcnt[2][16];
   float redf[4][4];
   int redi[4][10];
   float tab[KERNEL == 2 ? kCubicTable : 1];
   float dtab[KERNEL == 2 && ADJ ? kCubicTable : 1];
-  uint32_t cell_b[kBlock];
-  uint32_t cell_off[kBlock];
-  int cell_info[kBlock];  // cells: packed cell cx | cy << 16 (unsigned); rows: base key
-  int cell_y[kBlock];     // cells: 1 when in every point's search box; rows: the row's cy
+  uint32_t cell_b[NB];
+  uint32_t cell_off[NB];
+  int cell_info[NB];  // cells: packed cell cx | cy << 16 (unsigned); rows: base key
+  int cell_y[NB];     // cells: 1 when in every point's search box; rows: the row's cy
   uint32_t scan_ws[4];
-  uint32_t Lidx[ADJ ? kCap : 1];    // adjoint: sorted particle index
-  float4 Pt[ADJ ? kBlock * 4 : 1];  // adjoint: tile points (x, y, dL/dvalue, -), row-major
+  uint32_t Lidx[ADJ ? CAP : 1];    // adjoint: sorted particle index
+  float4 Pt[ADJ ? NB * 4 : 1];  // adjoint: tile points (x, y, dL/dvalue, -), row-major
   float4 Sb[ADJ ? 16 : 1];          // adjoint: 8x8 point blocks' boxes (x0, y0, x1, y1)
 };
 
 // One tile (tx, ty) of the point image: everything between the point loads
-// and the accumulator update.  All 256 threads of the workgroup call it with
-// the same arguments.
-template <int TW, int PPT, int OP, int KERNEL>
+// and the accumulator update.  All NB threads of the workgroup (NB / 64
+// waves) call it with the same arguments.
+template <int TW, int PPT, int OP, int KERNEL, int NB = 256>
 __device__ __attribute__((always_inline)) inline void gather_tile(
     const grid_view& g, const point_view& p, uint32_t tx, uint32_t ty, double* __restrict__ acc_out,
     unsigned long long* __restrict__ counts, float support,
-    gather_smem<OP == OP_REFERENCE, KERNEL, OP == OP_ADJOINT, OP == OP_CLIP>& sm,
+    gather_smem<OP == OP_REFERENCE, KERNEL, OP == OP_ADJOINT, OP == OP_CLIP, NB>& sm,
     const adjoint_io* adj = nullptr) {
-  constexpr int ROWS = kBlock / TW;
+  constexpr int NW = NB / 64;
+  constexpr int kCap = cap_of(NB), kPool = pool_of(NB), kFlush = flush_of(NB);
+  constexpr int ROWS = NB / TW;
   constexpr int TH = ROWS * PPT;
   constexpr bool REF = OP == OP_REFERENCE;
   constexpr bool COUNT = OP == OP_COUNT;
@@ -224,7 +235,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   constexpr bool CLIP = OP == OP_CLIP;
   constexpr bool QUARTIC = KERNEL == 0 && !COUNT && !ADJ;
   static_assert(!CLIP || KERNEL == 0, "the line-of-sight clip is the quartic's");
-  static_assert(!ADJ || PPT * kBlock <= kBlock * 4, "adjoint tile table");
+  static_assert(!ADJ || (NB == 256 && PPT <= 4), "adjoint tile table");
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -339,7 +350,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   int r_ilx = sm.redi[0][4], r_ily = sm.redi[0][5], r_ihx = sm.redi[0][6], r_ihy = sm.redi[0][7];
   int r_anyz = sm.redi[0][8], r_allz = sm.redi[0][9];
   int any_valid = sm.wcnt[0][0];
-  for (int w = 1; w < 4; ++w) {
+  for (int w = 1; w < NW; ++w) {
     tbx0 = fminf(tbx0, sm.redf[w][0]);
     tby0 = fminf(tby0, sm.redf[w][1]);
     tbx1 = fmaxf(tbx1, sm.redf[w][2]);
@@ -615,7 +626,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   auto eval_adjoint = [&](int cnt) {
     if constexpr (ADJ) {
       cnt = uniform(cnt);
-      for (int j0 = 0; j0 < cnt; j0 += kBlock) {
+      for (int j0 = 0; j0 < cnt; j0 += NB) {
         const int j = j0 + t;
         const bool act = j < cnt;
         const float4 q = act ? sm.Ls[j] : make_float4(0.f, 0.f, 0.f, 0.f);  // (x, y, 1/h^2, coef)
@@ -701,10 +712,10 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   const uint32_t nrowz = ncols > 0 ? uint32_t(nrows) * uint32_t(nlays) : 0u;
   const uint32_t nxny = uint32_t(g.nc[0]) * uint32_t(g.nc[1]);
   const float sup_h2 = support * support * 1.00002f;
-  bool rows_mode = g.sparse != 0 && ncand > uint32_t(kBlock);
+  bool rows_mode = g.sparse != 0 && ncand > uint32_t(NB);
   const bool one_layer = nlays == 1;
   uint64_t nitems = rows_mode ? nrowz : ncand;
-  for (uint64_t c0 = 0; c0 < nitems; c0 += kBlock) {
+  for (uint64_t c0 = 0; c0 < nitems; c0 += NB) {
     const bool in_box = c0 + uint64_t(t) < nitems;
     const uint32_t ci = uint32_t(c0) + uint32_t(t);
     uint32_t cb = 0, cn = 0;
@@ -759,13 +770,13 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       }
     }
     uint32_t tot;
-    const uint32_t coff = block_scan_excl(cn, &tot, sm.scan_ws);
+    const uint32_t coff = block_scan_excl<NB>(cn, &tot, sm.scan_ws);
     tot = uint32_t(uniform(int(tot)));
     if (rows_mode && c0 == 0 && uint64_t(tot) > uint64_t(kDenseRows) * ncand) {
       // nothing of this chunk is in LDS yet: start over with the cells
       rows_mode = false;
       nitems = ncand;
-      c0 = uint64_t(0) - uint64_t(kBlock);
+      c0 = uint64_t(0) - uint64_t(NB);
       continue;
     }
     sm.cell_b[t] = cb;
@@ -778,7 +789,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       sm.cell_y[t] = cfull;
     }
     __syncthreads();
-    for (uint32_t v0 = 0; v0 < tot; v0 += kBlock) {
+    for (uint32_t v0 = 0; v0 < tot; v0 += NB) {
       const uint32_t v = v0 + uint32_t(t);
       float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
       bool keep = false, full = false, big = false, cover = false;
@@ -789,7 +800,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
         // last cell whose offset is <= v (it has a nonzero count)
         int lo = 0;
 #pragma unroll
-        for (int step = kBlock / 2; step > 0; step >>= 1)
+        for (int step = NB / 2; step > 0; step >>= 1)
           if (sm.cell_off[lo + step] <= v) lo += step;
         const uint32_t sidx = sm.cell_b[lo] + (v - sm.cell_off[lo]);
         cinf = sm.cell_info[lo];
@@ -862,7 +873,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
       __syncthreads();
       int baseb = nb, basebp = nbp, bases = ns, basem = nm, totb = 0, totbp = 0, tots = 0,
           totm = 0;
-      for (int w = 0; w < 4; ++w) {
+      for (int w = 0; w < NW; ++w) {
         const int cb_ = sm.wcnt[slot][w], cs_ = sm.wcnt[slot][4 + w],
                   cm_ = sm.wcnt[slot][8 + w], cp_ = sm.wcnt[slot][12 + w];
         if (w < wave) {
@@ -877,7 +888,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
         totm += cm_;
       }
       if constexpr (QUARTIC) {
-        // nb + nbp <= kFlush - 1 + kBlock <= kCap: the two ends never meet
+        // nb + nbp <= kFlush - 1 + NB <= kCap: the two ends never meet
         if (big && cover) {
           const int pb = baseb + int(prefix_count(bb));
           sm.Lb[pb] = e;
@@ -945,15 +956,24 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
   }
 }
 
-template <int TW, int PPT, int OP, int KERNEL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1 ? 8 : (PPT <= 2 ? 5 : 4), 8))) void k_gather(grid_view g, point_view p, uint32_t tiles_x,
+#ifndef ILC_WG64_WAVES
+#define ILC_WG64_WAVES 5
+#endif
+// waves per SIMD asked of the compiler: one point per lane keeps the
+// candidate pipeline's latency hidden behind many waves
+constexpr int gather_waves(int ppt, int nb) {
+  return ppt <= 1 ? 8 : (nb == 64 ? ILC_WG64_WAVES : (ppt <= 2 ? 5 : 4));
+}
+
+template <int TW, int PPT, int OP, int KERNEL, int NB>
+__global__ __launch_bounds__(NB) __attribute__((amdgpu_waves_per_eu(gather_waves(PPT, NB), 8))) void k_gather(grid_view g, point_view p, uint32_t tiles_x,
                                                    uint32_t ntiles, double* __restrict__ acc_out,
                                                    unsigned long long* __restrict__ counts,
                                                    const float* __restrict__ cubic_table,
                                                    float support) {
-  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false, OP == OP_CLIP> sm;
+  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false, OP == OP_CLIP, NB> sm;
   if constexpr (KERNEL == 2) {
-    for (int i = threadIdx.x; i < kCubicTable; i += kBlock) sm.tab[i] = cubic_table[i];
+    for (int i = threadIdx.x; i < kCubicTable; i += NB) sm.tab[i] = cubic_table[i];
   }
   // XCD-contiguous tile order: workgroups are dealt round-robin to the 8
   // XCDs, so give XCD x the x-th eighth of the tiles (shared L2 reuse of the
@@ -962,7 +982,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1
   const uint32_t per = ntiles / 8u;
   const uint32_t tile = bid < per * 8u ? (bid % 8u) * per + bid / 8u : bid;
   const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-  gather_tile<TW, PPT, OP, KERNEL>(g, p, tx, ty, acc_out, counts, support, sm);
+  gather_tile<TW, PPT, OP, KERNEL, NB>(g, p, tx, ty, acc_out, counts, support, sm);
 }
 
 // Persistent gather over (view, tile) items.  Workgroups are dealt to the 8
@@ -971,17 +991,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1
 // on at a time are neighbours that read the same particle spans.  A
 // workgroup drains its XCD's queue, then helps with the others; every queue
 // counter only grows, so every workgroup leaves once all queues are empty.
-template <int TW, int PPT, int OP, int KERNEL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1 ? 8 : (PPT <= 2 ? 5 : 4), 8))) void k_gather_batch(const view_desc* __restrict__ views,
+template <int TW, int PPT, int OP, int KERNEL, int NB>
+__global__ __launch_bounds__(NB) __attribute__((amdgpu_waves_per_eu(gather_waves(PPT, NB), 8))) void k_gather_batch(const view_desc* __restrict__ views,
                                                          const uint2* __restrict__ items,
                                                          queue_segs segs,
                                                          uint32_t* __restrict__ counters,
                                                          const float* __restrict__ cubic_table,
                                                          float support) {
-  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false, OP == OP_CLIP> sm;
+  __shared__ gather_smem<OP == OP_REFERENCE, KERNEL, false, OP == OP_CLIP, NB> sm;
   __shared__ uint32_t next;
   if constexpr (KERNEL == 2) {
-    for (int i = threadIdx.x; i < kCubicTable; i += kBlock) sm.tab[i] = cubic_table[i];
+    for (int i = threadIdx.x; i < kCubicTable; i += NB) sm.tab[i] = cubic_table[i];
   }
   const uint32_t home = blockIdx.x % uint32_t(kQueues);
   for (uint32_t k = 0; k < uint32_t(kQueues); ++k) {
@@ -997,8 +1017,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT <= 1
       const uint32_t vi = uint32_t(uniform(int(it.x))), ti = uint32_t(uniform(int(it.y)));
       const view_desc& vd = views[vi];
       const uint32_t tiles_x = vd.tiles_x;
-      gather_tile<TW, PPT, OP, KERNEL>(vd.g, vd.p, ti % tiles_x, ti / tiles_x, vd.acc, nullptr,
-                                       support, sm);
+      gather_tile<TW, PPT, OP, KERNEL, NB>(vd.g, vd.p, ti % tiles_x, ti / tiles_x, vd.acc,
+                                           nullptr, support, sm);
     }
   }
 }
@@ -1038,58 +1058,59 @@ __global__ __launch_bounds__(kBlock) void k_unpermute4(const uint32_t* __restric
   od[i] = d[s];
 }
 
-template <int TW, int PPT, int OP, int KERNEL>
+template <int TW, int PPT, int OP, int KERNEL, int NB>
 struct plain_launcher {
   static int run(const grid_view& g, const point_view& p, double* acc,
                  unsigned long long* counts, const float* tab, float support, hipStream_t s) {
-    constexpr int TH = (kBlock / TW) * PPT;
+    constexpr int TH = (NB / TW) * PPT;
     const uint32_t tiles_x = (p.lnx + TW - 1) / TW;
     const uint32_t tiles_y = (p.lny + TH - 1) / TH;
     const uint32_t ntiles = tiles_x * tiles_y;
     if (!ntiles) return 0;
-    hipLaunchKernelGGL((k_gather<TW, PPT, OP, KERNEL>), dim3(ntiles), dim3(kBlock), 0, s, g, p,
+    hipLaunchKernelGGL((k_gather<TW, PPT, OP, KERNEL, NB>), dim3(ntiles), dim3(NB), 0, s, g, p,
                        tiles_x, ntiles, acc, counts, tab, support);
     ILC_HIP(hipGetLastError());
     return 0;
   }
 };
 
-template <int TW, int PPT, int OP, int KERNEL>
+template <int TW, int PPT, int OP, int KERNEL, int NB>
 struct batch_launcher {
   static int run(const view_desc* v, const uint2* it, const queue_segs& segs, uint32_t* c,
                  const float* tab, float sup, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_gather_batch<TW, PPT, OP, KERNEL>), dim3(grid), dim3(kBlock), 0, s, v,
+    hipLaunchKernelGGL((k_gather_batch<TW, PPT, OP, KERNEL, NB>), dim3(grid), dim3(NB), 0, s, v,
                        it, segs, c, tab, sup);
     ILC_HIP(hipGetLastError());
     return 0;
   }
 };
 
-template <int TW, int PPT, int OP, int KERNEL>
+template <int TW, int PPT, int OP, int KERNEL, int NB>
 struct batch_occ {
   static int run() {
     int per_cu = 0;
     ILC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, k_gather_batch<TW, PPT, OP, KERNEL>, kBlock, 0));
+        &per_cu, k_gather_batch<TW, PPT, OP, KERNEL, NB>, NB, 0));
     return per_cu;
   }
 };
 
-// dispatch over (tile shape, op, kernel) to F<TW, PPT, OP, KERNEL>::run
-template <template <int, int, int, int> class F, class... A>
+// dispatch over (tile shape, op, kernel) to F<TW, PPT, OP, KERNEL, NB>::run
+template <template <int, int, int, int, int> class F, class... A>
 int dispatch(const tile_shape& ts, int op, int kernel, A&&... a) {
-  auto by_op = [&](auto tw, auto ppt) -> int {
+  auto by_op = [&](auto tw, auto ppt, auto nb) -> int {
     constexpr int TW = decltype(tw)::value, PPT = decltype(ppt)::value;
+    constexpr int NB = decltype(nb)::value;
     if (op == OP_CLIP) {
       if (kernel == ILC_KERNEL_CUBIC) fail(ILC_EINVAL, "the line-of-sight clip needs a quartic kernel");
-      return F<TW, PPT, OP_CLIP, 0>::run(a...);
+      return F<TW, PPT, OP_CLIP, 0, NB>::run(a...);
     }
     if (op == OP_REFERENCE) {
       if (kernel == ILC_KERNEL_CUBIC) fail(ILC_EINVAL, "reference mode uses the quartic kernel");
-      return F<TW, PPT, OP_REFERENCE, 0>::run(a...);
+      return F<TW, PPT, OP_REFERENCE, 0, NB>::run(a...);
     }
-    if (kernel == ILC_KERNEL_CUBIC) return F<TW, PPT, OP_EXACT, 2>::run(a...);
-    return F<TW, PPT, OP_EXACT, 0>::run(a...);
+    if (kernel == ILC_KERNEL_CUBIC) return F<TW, PPT, OP_EXACT, 2, NB>::run(a...);
+    return F<TW, PPT, OP_EXACT, 0, NB>::run(a...);
   };
   using I16 = std::integral_constant<int, 16>;
   using I32 = std::integral_constant<int, 32>;
@@ -1097,11 +1118,18 @@ int dispatch(const tile_shape& ts, int op, int kernel, A&&... a) {
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
   using I4 = std::integral_constant<int, 4>;
-  if (ts.tw == 16 && ts.ppt == 1) return by_op(I16{}, I1{});
-  if (ts.tw == 32 && ts.ppt == 2) return by_op(I32{}, I2{});
-  if (ts.tw == 32 && ts.ppt == 4) return by_op(I32{}, I4{});
-  if (ts.tw == 64 && ts.ppt == 4) return by_op(I64{}, I4{});
-  fail(ILC_EINVAL, "gather: unsupported tile shape tw=%d ppt=%d", ts.tw, ts.ppt);
+  using W64 = std::integral_constant<int, 64>;
+  using W256 = std::integral_constant<int, 256>;
+  if (ts.nb == 256) {
+    if (ts.tw == 16 && ts.ppt == 1) return by_op(I16{}, I1{}, W256{});
+    if (ts.tw == 32 && ts.ppt == 2) return by_op(I32{}, I2{}, W256{});
+    if (ts.tw == 32 && ts.ppt == 4) return by_op(I32{}, I4{}, W256{});
+    if (ts.tw == 64 && ts.ppt == 4) return by_op(I64{}, I4{}, W256{});
+  } else if (ts.nb == 64) {
+    if (ts.tw == 16 && ts.ppt == 4) return by_op(I16{}, I4{}, W64{});
+    if (ts.tw == 8 && ts.ppt == 1) return by_op(std::integral_constant<int, 8>{}, I1{}, W64{});
+  }
+  fail(ILC_EINVAL, "gather: unsupported tile shape tw=%d ppt=%d nb=%d", ts.tw, ts.ppt, ts.nb);
 }
 
 }  // namespace
@@ -1136,7 +1164,7 @@ void launch_unpermute4(const uint32_t* perm, uint32_t n, const float* a, const f
 
 void tile_grid(const tile_shape& ts, uint32_t lnx, uint32_t lny, uint32_t* tiles_x,
                uint32_t* ntiles) {
-  const uint32_t th = uint32_t(kBlock / ts.tw) * uint32_t(ts.ppt);
+  const uint32_t th = uint32_t(ts.nb / ts.tw) * uint32_t(ts.ppt);
   *tiles_x = (lnx + ts.tw - 1) / ts.tw;
   *ntiles = *tiles_x * ((lny + th - 1) / th);
 }
@@ -1167,7 +1195,7 @@ void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,
   const float support = kernel == ILC_KERNEL_CUBIC ? 2.0f : 1.0f;
   if (op == OP_COUNT) {
     // the count has its own tile shape: its cost is independent of the render's
-    plain_launcher<32, 4, OP_COUNT, 0>::run(g, p, nullptr, counts, nullptr, support, s);
+    plain_launcher<32, 4, OP_COUNT, 0, 256>::run(g, p, nullptr, counts, nullptr, support, s);
     return;
   }
   if (kernel == ILC_KERNEL_CUBIC && !cubic_table)

@@ -199,8 +199,9 @@ struct point_view {
 enum gather_op { OP_REFERENCE = 0, OP_EXACT = 1, OP_COUNT = 2, OP_ADJOINT = 3, OP_CLIP = 4 };
 
 struct tile_shape {
-  int tw;   // tile width (points)
-  int ppt;  // points per thread (tile height = 256/tw * ppt)
+  int tw;         // tile width (points)
+  int ppt;        // points per thread (tile height = nb/tw * ppt)
+  int nb = 256;   // threads per workgroup (one tile per workgroup)
 };
 
 void launch_gather(const grid_view& g, const point_view& p, int op, int kernel,

@@ -454,8 +454,18 @@ ilc_status ilc_batch_backward(ilc_batch* b, const float* const* dl_dimage_dev, f
  * of the last non-graph render: the one persistent launch, or the sum of the
  * per-view launches of a plain render.  Measurement only. */
 ilc_status ilc_batch_last_gather_ms(const ilc_batch* b, float* ms);
-/* Gather tile (points); tile_w * tile_h must be a multiple of 256. */
+/* Gather tile (points); tile_w * tile_h must be a multiple of 256.  One
+ * tile per workgroup of 256 threads. */
 ilc_status ilc_batch_set_tile(ilc_batch* b, int32_t tile_w, int32_t tile_h);
+/* The same with the workgroup size: 256 threads (the shapes of
+ * ilc_batch_set_tile), or 64 (one wave per 16x16 tile, four points per
+ * lane: four times the tiles in flight per CU, for latency-bound sparse
+ * views).  No reference counterpart: tuning of this build's gather. */
+ilc_status ilc_batch_set_tile_wg(ilc_batch* b, int32_t tile_w, int32_t tile_h,
+                                 int32_t workgroup);
+/* The current tile shape and workgroup size (after ilc_batch_autotune). */
+ilc_status ilc_batch_get_tile(const ilc_batch* b, int32_t* tile_w, int32_t* tile_h,
+                              int32_t* workgroup);
 /* Depth cut of every view.  clip = 0 (default): the reference's centre
  * mask, a particle counts whole when its camera depth lies in [0, depth] and
  * not at all otherwise (projection.cl:80-91).  clip = 1: the line-of-sight

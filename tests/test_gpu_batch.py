@@ -99,12 +99,13 @@ def test_batch_equals_single_view(B, scene, mode):
         assert np.array_equal(r.image_f64(k).cpu().numpy(), ref), f"plain view {k}"
 
 
-def test_batch_view_vs_oracle(B, scene, orc):
+@pytest.mark.parametrize("tile", [None, (16, 16, 64), (8, 8, 64)])
+def test_batch_view_vs_oracle(B, scene, orc, tile):
     from illcrawl_amd import synth
     snap, views = scene
     tab = synth.gaunt_table_1d()
-    say("case: batch view 1 vs oracle")
-    r = make_batch(B, snap, views, "reference", tab)
+    say(f"case: batch view 1 vs oracle, tile {tile}")
+    r = make_batch(B, snap, views, "reference", tab, tile=tile)
     r.render()
     for k in (0, 1):
         v = views[k]
@@ -130,13 +131,20 @@ def test_plain_and_tiles_agree(B, scene):
     r.render(plain=True)
     for k in range(len(views)):
         assert_same_sum(r.image_f64(k).cpu().numpy(), base[k], f"plain view {k}")
-    for tile in ((16, 16), (32, 16), (64, 16)):
+    # (16, 16, 64): one-wave workgroups, four points per lane
+    for tile in ((16, 16), (32, 16), (64, 16), (8, 8, 64), (16, 16, 64)):
         r.set_tile(*tile)
         r.render()
         for k in range(len(views)):
             a = r.image_f64(k).cpu().numpy()
             rel = np.abs(a - base[k]).max() / base[k].max()
             assert rel < 1e-6, (tile, k, rel)
+    assert r.tile() == (16, 16, 64)
+    r.render(plain=True)
+    for k in range(len(views)):
+        a = r.image_f64(k).cpu().numpy()
+        rel = np.abs(a - base[k]).max() / base[k].max()
+        assert rel < 1e-6, ("plain, one-wave tiles", k, rel)
 
 
 def test_graph_replays_match_eager(B, scene):
@@ -287,15 +295,16 @@ def test_windowed_grids_equal_single_view(B, mode):
         assert np.array_equal(r.image_f64(k).cpu().numpy(), eager[k]), f"graph view {k}"
 
 
-@pytest.mark.parametrize("mode", ["reference", "exact"])
-def test_slab_stack_shares_one_grid(B, orc, mode):
+@pytest.mark.parametrize("mode,tile", [("reference", None), ("exact", None),
+                                       ("reference", (16, 16, 64)), ("reference", (8, 8, 64))])
+def test_slab_stack_shares_one_grid(B, orc, mode, tile):
     """Axis-aligned slabs project every particle to the same (x', y'): the
     batch builds one full grid for the stack (cells, per-cell max h) and each
     slab's lists from its depth cut only.  Particles outside the cut have
     w = 0 and add nothing, so the images equal the single-view ones up to the
     FP32 grouping of the partial sums, and match the oracle."""
     from illcrawl_amd import synth
-    say(f"case: slab stack, mode={mode}")
+    say(f"case: slab stack, mode={mode}, tile {tile}")
     snap = synth.make_snapshot(1 << 16, 11, 10000.0, n_halos=3, rs_range=(200.0, 500.0),
                                gas=True)
     tab = synth.gaunt_table_1d()
@@ -303,7 +312,7 @@ def test_slab_stack_shares_one_grid(B, orc, mode):
                              quantity=1)
     for v in views:
         v["scale"] = 1.0
-    r = make_batch(B, snap, views, mode, tab)
+    r = make_batch(B, snap, views, mode, tab, tile=tile)
     r.render()
     for k, v in enumerate(views):
         ref, proj = single_view(B, snap, v, mode, tab)
