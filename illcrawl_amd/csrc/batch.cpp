@@ -546,7 +546,9 @@ void build_group(ilc_batch* b, size_t gi, const build_ctx& c) {
 }
 
 // view v's grid and cleared accumulator
-void build_view(ilc_batch* b, view_state& vs, const build_ctx& c) {
+// clear: clear the view's accumulator (renders whose gather adds to it; the
+// persistent gather writes it whole)
+void build_view(ilc_batch* b, view_state& vs, const build_ctx& c, bool clear) {
   const hipStream_t st = c.st;
   c.p4tmp->ensure(b->ps.n);
   if (vs.windowed || vs.compact) {
@@ -575,7 +577,7 @@ void build_view(ilc_batch* b, view_state& vs, const build_ctx& c) {
     vs.grid.density_n = 0;
   }
   own_grid(b, vs);
-  launch_fill_f64(vs.acc.p, 0.0, uint64_t(vs.v.cam.nx) * vs.v.cam.ny, st);
+  if (clear) launch_fill_f64(vs.acc.p, 0.0, uint64_t(vs.v.cam.nx) * vs.v.cam.ny, st);
 }
 
 // every view's grid in one pipeline on stream s (persistent renders)
@@ -663,7 +665,6 @@ void build_shared(ilc_batch* b, hipStream_t s) {
     vs.gbase = vs.sbase;
     vs.gn = kept;
     vs.gready = true;
-    launch_fill_f64(vs.acc.p, 0.0, uint64_t(vs.v.cam.nx) * vs.v.cam.ny, s);
   }
 }
 
@@ -672,16 +673,26 @@ view_desc make_desc(const ilc_batch* b, size_t v) {
   const uint64_t P = uint64_t(vs.v.cam.nx) * vs.v.cam.ny;
   view_desc d{};
   d.g = grid_of(b, vs);
-  d.p = point_view{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny};
+  // the pixel corners, computed by the gather as the point kernel does
+  d.p = point_view{vs.pts.p, static_cast<uint32_t>(P), vs.v.cam.nx, vs.v.cam.ny, 1,
+                   pixel_gen_of(vs.f)};
   d.acc = vs.acc.p;
   d.tiles_x = vs.tiles_x;
   d.ntiles = vs.ntiles;
   return d;
 }
 
-// The persistent gather's descriptor of view v.  desc_host is the source of
-// the captured graph's descriptor copy, so only persistent renders write it.
-void fill_desc(ilc_batch* b, size_t v) { b->desc_host.p[v] = make_desc(b, v); }
+// The persistent gather's descriptor of view v: it writes the accumulator
+// and the image itself.  desc_host is the source of the captured graph's
+// descriptor copy, so only persistent renders write it.
+void fill_desc(ilc_batch* b, size_t v) {
+  view_desc d = make_desc(b, v);
+  const view_state& vs = *b->views[v];
+  d.img = vs.img.p;
+  d.scale = vs.v.scale;
+  d.fused = 1;
+  b->desc_host.p[v] = d;
+}
 
 void ensure_lanes(ilc_batch* b) {
   if (!b->ev_fork) ILC_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
@@ -711,7 +722,7 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
     for (size_t gi = 0; gi < b->group_leader.size(); ++gi) build_group(b, gi, main);
     for (size_t v = 0; v < nv; ++v) {
       view_state& vs = *b->views[v];
-      build_view(b, vs, main);
+      build_view(b, vs, main, true);
       const view_desc d = make_desc(b, v);
       if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
       launch_gather(d.g, d.p, op, b->kernel, b->ts, vs.acc.p, nullptr, b->cubic.p, s);
@@ -733,7 +744,7 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
       for (size_t v = 0; v < nv; ++v) fill_desc(b, v);
     } else {
       for (size_t v = 0; v < nv; ++v) {
-        build_view(b, *b->views[v], lane_ctx(job++));
+        build_view(b, *b->views[v], lane_ctx(job++), false);
         fill_desc(b, v);
       }
     }
@@ -751,10 +762,7 @@ void device_work(ilc_batch* b, bool persistent, hipStream_t s, bool timed = fals
     launch_gather_batch(b->desc_dev.p, b->items_ord.p, b->segs, b->qcounters.p, op, b->kernel,
                         b->ts, b->cubic.p, b->grid_blocks, s);
     if (timed) ILC_HIP(hipEventRecord(timing_event(b), s));
-    for (size_t v = 0; v < nv; ++v) {
-      view_state& vs = *b->views[v];
-      launch_retrieve(vs.acc.p, uint64_t(vs.v.cam.nx) * vs.v.cam.ny, vs.v.scale, vs.img.p, s);
-    }
+    // the gather wrote every view's accumulator and image
   }
   ILC_HIP(hipMemcpyAsync(b->flags_host.p, b->counter.p, sizeof(uint32_t) * 2,
                          hipMemcpyDeviceToHost, s));

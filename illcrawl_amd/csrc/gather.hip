@@ -50,6 +50,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "pixel.hpp"
 
 namespace ilc {
 
@@ -224,7 +225,8 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
     const grid_view& g, const point_view& p, uint32_t tx, uint32_t ty, double* __restrict__ acc_out,
     unsigned long long* __restrict__ counts, float support,
     gather_smem<OP == OP_REFERENCE, KERNEL, OP == OP_ADJOINT, OP == OP_CLIP, NB>& sm,
-    const adjoint_io* adj = nullptr) {
+    const adjoint_io* adj = nullptr, float* __restrict__ img_out = nullptr, float img_scale = 1.0f,
+    int fused = 0) {
   constexpr int NW = NB / 64;
   constexpr int kCap = cap_of(NB), kPool = pool_of(NB), kFlush = flush_of(NB);
   constexpr int ROWS = NB / TW;
@@ -272,7 +274,7 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
     pid[k] = py * p.lnx + px;
     pv[k] = px < p.lnx && py < p.lny && pid[k] < p.n;
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (pv[k]) q = p.pts[pid[k]];
+    if (pv[k]) q = p.gen ? pixel_point(p.pg, px, py) : p.pts[pid[k]];
     ex[k] = q.x;
     ey[k] = q.y;
     a64[k] = 0.0;
@@ -950,6 +952,11 @@ __device__ __attribute__((always_inline)) inline void gather_tile(
     if (!pv[k]) continue;
     if constexpr (COUNT) {
       if (counts) counts[pid[k]] += cnt_acc[k];
+    } else if (fused) {
+      // the only visit of this point in the launch: no read of the
+      // accumulator, and the image as the retrieval computes it
+      acc_out[pid[k]] = a64[k];
+      img_out[pid[k]] = static_cast<float>(a64[k] * static_cast<double>(img_scale));
     } else {
       acc_out[pid[k]] += a64[k];
     }
@@ -1018,7 +1025,8 @@ __global__ __launch_bounds__(NB) __attribute__((amdgpu_waves_per_eu(gather_waves
       const view_desc& vd = views[vi];
       const uint32_t tiles_x = vd.tiles_x;
       gather_tile<TW, PPT, OP, KERNEL, NB>(vd.g, vd.p, ti % tiles_x, ti / tiles_x, vd.acc,
-                                           nullptr, support, sm);
+                                           nullptr, support, sm, nullptr, vd.img, vd.scale,
+                                           vd.fused);
     }
   }
 }

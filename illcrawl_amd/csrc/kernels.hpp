@@ -188,11 +188,24 @@ struct grid_view {
   uint32_t ncells;
 };
 
-// Evaluation points laid out as an image (lnx x lny, row-major).
+// Pixel-corner points of a camera (pixel.hpp): projection rows and centre,
+// the FP32 corner and screen axes, the pixel size.
+struct pixel_gen {
+  float m[9], c[3];
+  float mn[3], b0[3], b1[3];
+  float ps;
+};
+pixel_gen pixel_gen_of(const camera_frame& f);
+
+// Evaluation points laid out as an image (lnx x lny, row-major): loaded from
+// pts, or (gen != 0) the pixel corners of the camera pg, computed where they
+// are used.
 struct point_view {
   const float4* pts;
   uint32_t n;
   uint32_t lnx, lny;
+  int gen = 0;
+  pixel_gen pg{};
 };
 
 // OP_CLIP: exact mode with the line-of-sight kernel clip (grid_view::clip)
@@ -220,12 +233,19 @@ void launch_unpermute4(const uint32_t* perm, uint32_t n, const float* a, const f
                        hipStream_t s);
 
 // One view of a batch: its grid, its points, its accumulator, its tiling.
+// fused = 1: the gather writes the view's accumulator (acc = sum, without
+// reading it) and its image (img = float(sum * double(scale)), as
+// launch_retrieve) instead of adding to a cleared accumulator; every
+// (view, tile) item is visited once per launch.
 struct view_desc {
   grid_view g;
   point_view p;
   double* acc;
   uint32_t tiles_x;
   uint32_t ntiles;
+  float* img = nullptr;
+  float scale = 1.0f;
+  int fused = 0;
 };
 
 // tiles of a point image for a tile shape

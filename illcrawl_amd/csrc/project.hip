@@ -15,6 +15,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "pixel.hpp"
 
 namespace ilc {
 
@@ -55,24 +56,12 @@ struct pix_args {
   float ps;
 };
 
-__global__ __launch_bounds__(kBlock) void k_pixel_points(pix_args a, uint32_t nx, uint32_t ny,
+__global__ __launch_bounds__(kBlock) void k_pixel_points(pixel_gen a, uint32_t nx, uint32_t ny,
                                                           float4* out) {
-#pragma clang fp contract(off)
   const uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
   if (i >= uint64_t(nx) * ny) return;
   const uint32_t iy = uint32_t(i / nx), ix = uint32_t(i - uint64_t(iy) * nx);
-  const float sx = float(ix) * a.ps;
-  const float sy = float(iy) * a.ps;
-  float q[3];
-  for (int c = 0; c < 3; ++c) {
-    const float t0 = sx * a.b0[c];
-    const float t1 = a.mn[c] + t0;
-    const float t2 = sy * a.b1[c];
-    q[c] = t1 + t2;
-  }
-  const float vx = q[0] - a.pa.c[0], vy = q[1] - a.pa.c[1], vz = q[2] - a.pa.c[2];
-  out[i] = make_float4(dot3(a.pa.m + 0, vx, vy, vz), dot3(a.pa.m + 3, vx, vy, vz),
-                       dot3(a.pa.m + 6, vx, vy, vz), 0.0f);
+  out[i] = pixel_point(a, ix, iy);
 }
 
 // camera_generate_pixel_coordinates (camera.cl:23-41): world-frame corners,
@@ -224,16 +213,22 @@ void launch_project_particles(const camera_frame& f, float depth, const float4* 
   ILC_HIP(hipGetLastError());
 }
 
-void launch_pixel_points(const camera_frame& f, uint32_t nx, uint32_t ny, float4* out,
-                         hipStream_t s) {
-  pix_args a;
-  a.pa = make_proj(f);
+pixel_gen pixel_gen_of(const camera_frame& f) {
+  pixel_gen a{};
+  for (int k = 0; k < 9; ++k) a.m[k] = f.minv[k];
   for (int c = 0; c < 3; ++c) {
+    a.c[c] = f.center[c];
     a.mn[c] = f.min_corner[c];
     a.b0[c] = f.b0f[c];
     a.b1[c] = f.b1f[c];
   }
   a.ps = f.psf;
+  return a;
+}
+
+void launch_pixel_points(const camera_frame& f, uint32_t nx, uint32_t ny, float4* out,
+                         hipStream_t s) {
+  const pixel_gen a = pixel_gen_of(f);
   hipLaunchKernelGGL(k_pixel_points, dim3(nblocks(uint64_t(nx) * ny)), dim3(kBlock), 0, s, a, nx,
                      ny, out);
   ILC_HIP(hipGetLastError());
