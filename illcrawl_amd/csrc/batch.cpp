@@ -691,6 +691,9 @@ void fill_desc(ilc_batch* b, size_t v) {
   d.img = vs.img.p;
   d.scale = vs.v.scale;
   d.fused = 1;
+  const uint64_t rows = (vs.windowed || vs.compact) ? uint64_t(vs.nwin) : b->ps.n;
+  d.cost_unit = static_cast<uint32_t>(
+      std::min<uint64_t>(0xffffffu, std::max<uint64_t>(1, rows / std::max<uint32_t>(vs.ntiles, 1u))));
   b->desc_host.p[v] = d;
 }
 

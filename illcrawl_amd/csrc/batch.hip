@@ -675,12 +675,13 @@ __global__ __launch_bounds__(kBlock) void k_gather_u32(const uint32_t* __restric
 // Work-item order of the persistent gather.  A tile's cost is dominated by
 // the particles of the grid cells under its points: a tile over a dense core
 // streams and evaluates the most, and started last it runs on alone while
-// the rest of the GPU idles.  Each item gets the key queue << 5 | class,
-// class = 31 - bit length of that particle count (capped), so that a stable
-// sort by key keeps every queue's items together, heaviest class first,
-// tiles of one class in their original (neighbouring) order.  The count
-// visits at most 32 rows (a stride over taller boxes) and 8 layers: it is
-// an estimate.
+// the rest of the GPU idles.  Each item gets the key queue << 5 | class.  A
+// heavy tile (at least 16 times the view's mean particles per tile) has
+// class 31 - bit length of its count, every other tile class 31.  A stable
+// sort by key keeps every queue's items together, the heavy tiles first and
+// heaviest first, and the light majority in its original order (neighbours
+// in one queue share their XCD's L2).  The count visits at most 32 rows (a
+// stride over taller boxes) and 8 layers: it is an estimate.
 __global__ __launch_bounds__(kBlock) void k_item_cost(const view_desc* __restrict__ views,
                                                       const uint2* __restrict__ items,
                                                       queue_segs segs, uint32_t n, int tw, int th,
@@ -725,7 +726,8 @@ __global__ __launch_bounds__(kBlock) void k_item_cost(const view_desc* __restric
   }
   const uint32_t c32 = cost > 0xffffffffull ? 0xffffffffu : uint32_t(cost);
   const int bits = c32 ? 32 - __clz(c32) : 0;
-  keys[i] = (q << 5) | uint32_t(31 - min(bits, 31));
+  const bool heavy = cost >= 16u * uint64_t(vd.cost_unit);
+  keys[i] = (q << 5) | (heavy ? uint32_t(31 - min(bits, 31)) : 31u);
   idx[i] = i;
 }
 

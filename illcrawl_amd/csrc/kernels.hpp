@@ -246,6 +246,9 @@ struct view_desc {
   float* img = nullptr;
   float scale = 1.0f;
   int fused = 0;
+  // the view's mean particles per tile (kept rows / tiles, at least 1): the
+  // unit of the work-item order's cost classes
+  uint32_t cost_unit = 1;
 };
 
 // tiles of a point image for a tile shape

@@ -243,7 +243,8 @@ def test_batch_counts_and_autotune(B, scene, orc):
     assert counts[2] == orc.count_evals_lattice(pp, snap.hsml, cam.pixel_points(), v["nx"],
                                                 v["ny"])
     (tw, th), ms = r.autotune()
-    assert tw * th % 256 == 0 and ms > 0
+    w, h, wg = r.tile()
+    assert (w, h) == (tw, th) and wg in (64, 256) and tw * th % wg == 0 and ms > 0
     r.render()
     assert r.count_evals() == counts
 
