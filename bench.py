@@ -50,7 +50,9 @@ FLOP_PER_EVAL_3D = 13
 # HBM bytes per k_gather_batch launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs of this workload scaled by a 1 GiB copy (profiles/k_gather_batch_xray_16M_64views.md);
 # keyed by (workload, mode, tile)
-MEASURED_TRAFFIC = {("c3", "reference", (16, 16, 256)): dict(read=19.30e9, write=1.781e9),
+MEASURED_TRAFFIC = {("c3", "reference", (16, 16, 256)): dict(read=22.61e9, write=3.250e9),
+                    # profiles/k_gather_batch_xray_64M_32slabs.md
+                    ("c5", "reference", (16, 16, 256)): dict(read=21.16e9, write=19.59e9),
                     # k_smooth3d, profiles/k_smooth3d_slices_16M.md
                     ("v1", "reference", (16, 16)): dict(read=1.244e9, write=0.461e9)}
 
