@@ -162,6 +162,9 @@ struct ilc_batch {
   pinned<uint32_t> sh_cbase_host;
   dbuf<float> sh_wx, sh_hs;
   dbuf<float4> sh_p4c, sh_part;
+  // every view a slab of one stack (the multi-view passes test two slabs per
+  // particle instead of every view)
+  slab_stack stack;
   dbuf<uint2> items_dev;
   pinned<uint2> items_host;
   uint32_t nitems = 0;
@@ -232,6 +235,41 @@ struct device_guard {
   }
 };
 
+// The views as one slab stack (slab_stack), when they are one: a single slab
+// group of centre-masked views (no window, no screen test, no clip), the
+// depth row (0, 0, +-1) the same for all, depth ranges [s, s + depth],
+// s = m8 * cz, that follow one another without overlap, each thicker than
+// FP32 rounding of its coordinates.  Else n = 0.
+slab_stack find_stack(const ilc_batch* b) {
+  slab_stack st;
+  const size_t nv = b->views.size();
+  if (nv < 2 || nv > size_t(kStackMax) || b->clip) return st;
+  const float m8 = b->views[0]->f.minv[8];
+  if (!(m8 == 1.0f || m8 == -1.0f)) return st;
+  std::vector<std::pair<double, size_t>> order;
+  for (size_t v = 0; v < nv; ++v) {
+    const view_state& vs = *b->views[v];
+    if (!vs.compact || vs.group != 0 || vs.win.use_window || vs.win.use_screen ||
+        !vs.win.use_depth || vs.win.clip || vs.f.minv[8] != m8)
+      return st;
+    order.emplace_back(double(m8 * vs.f.center[2]), v);
+  }
+  std::sort(order.begin(), order.end());
+  for (size_t k = 0; k < nv; ++k) {
+    const double s0 = order[k].first;
+    const double d = double(static_cast<float>(b->views[order[k].second]->v.cam.depth));
+    if (!(std::isfinite(s0) && d > 1e-5 * (std::fabs(s0) + d + 1.0))) return st;
+    if (k + 1 < nv && order[k + 1].first < s0 + d) return st;
+  }
+  for (size_t k = 0; k < nv; ++k) {
+    st.s[k] = static_cast<float>(order[k].first);
+    st.view[k] = static_cast<int32_t>(order[k].second);
+  }
+  st.m8 = m8;
+  st.n = static_cast<int>(nv);
+  return st;
+}
+
 view_proj proj_of(const ilc_batch* b, const view_state& vs) {
   view_proj vp;
   for (int k = 0; k < 9; ++k) vp.m[k] = vs.f.minv[k];
@@ -300,26 +338,6 @@ void plan(ilc_batch* b, hipStream_t s) {
     b->bdesc_host.p[v].vp = proj_of(b, *b->views[v]);
     b->bdesc_host.p[v].win = window_f{0.f, 0.f, 0.f, 0.f, 0, 0};
   }
-  ILC_HIP(hipMemcpyAsync(b->bdesc_dev.p, b->bdesc_host.p, sizeof(view_build_desc) * nv,
-                         hipMemcpyHostToDevice, s));
-  launch_multi_bbox(b->ps, b->bdesc_dev.p, static_cast<int>(nv), gaunt_of(b), b->bbox_parts.p,
-                    b->bad.p, b->bbox_dev.p, s);
-  ILC_HIP(hipMemcpyAsync(b->bbox_host.p, b->bbox_dev.p, sizeof(bbox_result) * nv,
-                         hipMemcpyDeviceToHost, s));
-  ILC_HIP(hipStreamSynchronize(s));
-  std::vector<uint64_t> sig;
-  for (size_t v = 0; v < nv; ++v) {
-    view_state& vs = *b->views[v];
-    vs.gp = plan_grid(b->bbox_host.p[v], b->ps.n, b->target, b->mode, &vs.kz0);
-    vs.max_h = b->bbox_host.p[v].max_h;
-    uint32_t bits[10];
-    std::memcpy(bits, vs.gp.nc, 12);
-    std::memcpy(bits + 3, vs.gp.minc, 12);
-    std::memcpy(bits + 6, vs.gp.cs, 12);
-    std::memcpy(bits + 9, &vs.max_h, 4);
-    for (uint32_t x : bits) sig.push_back(x);
-    sig.push_back(uint64_t(vs.kz0));
-  }
   // slab groups: views whose projection rows for x' and y' and centre x, y are
   // bitwise equal, with no z term in those rows and no x, y term in the depth
   // row, project every particle to the same (x', y') and differ only in the
@@ -351,6 +369,28 @@ void plan(ilc_batch* b, hipStream_t s) {
   for (size_t v = 0; v < nv; ++v) b->views[v]->compact = b->views[v]->group >= 0;
   while (b->group_grids.size() < b->group_leader.size())
     b->group_grids.push_back(std::make_unique<grid_store>());
+  bool one_group = !b->group_leader.empty();
+  for (size_t v = 0; v < nv; ++v) one_group = one_group && b->views[v]->group == 0;
+  ILC_HIP(hipMemcpyAsync(b->bdesc_dev.p, b->bdesc_host.p, sizeof(view_build_desc) * nv,
+                         hipMemcpyHostToDevice, s));
+  launch_multi_bbox(b->ps, b->bdesc_dev.p, static_cast<int>(nv), gaunt_of(b), b->bbox_parts.p,
+                    b->bad.p, b->bbox_dev.p, one_group, s);
+  ILC_HIP(hipMemcpyAsync(b->bbox_host.p, b->bbox_dev.p, sizeof(bbox_result) * nv,
+                         hipMemcpyDeviceToHost, s));
+  ILC_HIP(hipStreamSynchronize(s));
+  std::vector<uint64_t> sig;
+  for (size_t v = 0; v < nv; ++v) {
+    view_state& vs = *b->views[v];
+    vs.gp = plan_grid(b->bbox_host.p[v], b->ps.n, b->target, b->mode, &vs.kz0);
+    vs.max_h = b->bbox_host.p[v].max_h;
+    uint32_t bits[10];
+    std::memcpy(bits, vs.gp.nc, 12);
+    std::memcpy(bits + 3, vs.gp.minc, 12);
+    std::memcpy(bits + 6, vs.gp.cs, 12);
+    std::memcpy(bits + 9, &vs.max_h, 4);
+    for (uint32_t x : bits) sig.push_back(x);
+    sig.push_back(uint64_t(vs.kz0));
+  }
 
   // windows: the screen grown by the largest reach of any point (F3 radius
   // <= max h; support up to 2 h) plus whole cells.  Reference mode needs every
@@ -422,9 +462,10 @@ void plan(ilc_batch* b, hipStream_t s) {
     ILC_HIP(hipMemcpyAsync(b->fviews_dev.p, b->fviews_host.p, sizeof(int32_t) * size_t(nf),
                            hipMemcpyHostToDevice, s));
     launch_fill_u32(b->wtotal.p, 0u, 2 * nv, s);
+    b->stack = find_stack(b);
     launch_multi_window_count(b->ps.pos3, b->ps.hsml, b->ps.n, b->bdesc_dev.p, b->fviews_dev.p,
                               nf, b->wcounts_all.p, b->wtotal.p,
-                              b->wcounts_all.p + nv * size_t(nt), b->wtotal.p + nv, s);
+                              b->wcounts_all.p + nv * size_t(nt), b->wtotal.p + nv, b->stack, s);
     ILC_HIP(hipMemcpyAsync(b->wtotal_host.p, b->wtotal.p, sizeof(uint32_t) * 2 * nv,
                            hipMemcpyDeviceToHost, s));
     ILC_HIP(hipStreamSynchronize(s));
@@ -619,7 +660,7 @@ void build_shared(ilc_batch* b, hipStream_t s) {
                               b->fviews_dev.p, static_cast<int>(nv), b->sh_offs.p, b->sh_p4c.p,
                               b->sh_sk.p, b->sh_perm.p, b->sh_gkey.p, b->sh_orig.p,
                               b->counter.p + 1, b->sh_cbase.p, b->sh_maxh.p,
-                              b->clip ? b->sh_zc.p : nullptr, s);
+                              b->clip ? b->sh_zc.p : nullptr, b->stack, s);
   // stable sorts by descending h, then by global cell key (view, then cell):
   // a view's rows end up in one range, by cell, biggest h first, ties in
   // input order

@@ -163,11 +163,18 @@ __global__ __launch_bounds__(kBlock) void k_view_pass(particle_soa ps, view_proj
 constexpr int kMultiItems = 8;
 constexpr int kMultiChunk = kBlock * kMultiItems;
 
+// one_group: every view is a slab of one slab group (the same x', y'
+// projection, bit for bit, and the same quantity).  A particle's bounding-box
+// terms are then the same for every view, unless its weight is not finite
+// inside some slab's depth cut (that slab counts it as bad): a workgroup
+// without such a particle projects once and writes the same partial for
+// every view; the others take the per-view loop.
 __global__ __launch_bounds__(kBlock) void k_multi_bbox(particle_soa ps,
                                                         const view_build_desc* __restrict__ vd,
                                                         int nv, gaunt_table gt,
                                                         float* __restrict__ part,
-                                                        uint32_t* __restrict__ bad) {
+                                                        uint32_t* __restrict__ bad,
+                                                        int one_group) {
 #pragma clang fp contract(off)
   __shared__ float tab[kMaxGaunt];
   __shared__ float red[2][4][5];
@@ -207,6 +214,64 @@ __global__ __launch_bounds__(kBlock) void k_multi_bbox(particle_soa ps,
         e = e * sqrtf(T);
         wx[r] = e * g;
       }
+    }
+  }
+  if (one_group) {
+    const view_proj vp = vd[0].vp;
+    float mn0 = INFINITY, mn1 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY, mh = 0.0f;
+    uint32_t nbad = 0;
+    int wbad = 0;
+#pragma unroll
+    for (int r = 0; r < kMultiItems; ++r) {
+      if (!in[r]) continue;
+      const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
+      const float ax = dot3(vp.m + 0, vx, vy, vz);
+      const float ay = dot3(vp.m + 3, vx, vy, vz);
+      const float w = vp.quantity == 1 ? wx[r] : wm[r];
+      const bool ok = isfinite(ax) && isfinite(ay) && hh[r] >= 1.1e-19f && hh[r] <= 1.8e19f;
+      if (ok && !isfinite(w)) wbad = 1;
+      if (!ok) {
+        ++nbad;
+        continue;
+      }
+      mn0 = fminf(mn0, ax);
+      mx0 = fmaxf(mx0, ax);
+      mn1 = fminf(mn1, ay);
+      mx1 = fmaxf(mx1, ay);
+      mh = fmaxf(mh, hh[r]);
+    }
+    if (!__syncthreads_or(wbad)) {
+      const float rv[5] = {wmin(mn0), wmin(mn1), wmax(mx0), wmax(mx1), wmax(mh)};
+      uint32_t bsum = nbad;
+      for (int o = 32; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 64);
+      if (lane == 0) {
+        for (int k = 0; k < 5; ++k) red[0][wave][k] = rv[k];
+        redb[0][wave] = bsum;
+      }
+      __syncthreads();
+      float q[5];
+      for (int k = 0; k < 5; ++k) q[k] = red[0][0][k];
+      uint32_t bb = redb[0][0];
+      for (int w = 1; w < 4; ++w) {
+        q[0] = fminf(q[0], red[0][w][0]);
+        q[1] = fminf(q[1], red[0][w][1]);
+        q[2] = fmaxf(q[2], red[0][w][2]);
+        q[3] = fmaxf(q[3], red[0][w][3]);
+        q[4] = fmaxf(q[4], red[0][w][4]);
+        bb += redb[0][w];
+      }
+      for (int v = threadIdx.x; v < nv; v += kBlock) {
+        float* o = part + (uint64_t(v) * nparts + blockIdx.x) * 8;
+        o[0] = q[0];
+        o[1] = q[1];
+        o[2] = 0.0f;
+        o[3] = q[2];
+        o[4] = q[3];
+        o[5] = 0.0f;
+        o[6] = q[4];
+        if (bb) atomicAdd(bad + v, bb);
+      }
+      return;
     }
   }
   for (int v = 0; v < nv; ++v) {
@@ -369,18 +434,53 @@ __device__ inline bool reaches_screen(const window_f& win, float ax, float ay, f
 // counts of kViewChunk views are exchanged at once
 constexpr int kViewChunk = 16;
 
+// The slabs of a stack that can keep a particle at depth coordinate
+// t = m8 * z: k, the last slab with s_k <= t, and k - 1 (-1: none).  Each is
+// then tested with its view's own arithmetic.  ss: the s_k, n of them.
+__device__ inline int last_slab_below(const float* ss, int n, float t) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ss[mid] <= t)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo - 1;
+}
+
+// What a slab's centre mask reads of its view: the depth row, the centre and
+// the depth (kept in LDS by the stack passes).
+struct slab_par {
+  float m[3], c[3], depth;
+};
+
+__device__ inline slab_par slab_par_of(const view_proj& vp) {
+  return slab_par{{vp.m[6], vp.m[7], vp.m[8]}, {vp.c[0], vp.c[1], vp.c[2]}, vp.depth};
+}
+
+// the centre mask of a slab at (x, y, z), in the arithmetic of the per-view
+// passes
+__device__ inline bool slab_keeps(const slab_par& sp, float x, float y, float z, float h) {
+#pragma clang fp contract(off)
+  const float vx = x - sp.c[0], vy = y - sp.c[1], vz = z - sp.c[2];
+  return depth_keep(dot3(sp.m, vx, vy, vz), h, sp.depth, 0);
+}
+
 // every listed view's kept rows per tile, with and without the screen test,
 // from one read of the tile (k_count_totals sums them per view)
 __global__ __launch_bounds__(kBlock) void k_multi_window_count(
     const float* __restrict__ pos3, const float* __restrict__ hsml, uint64_t n,
     const view_build_desc* __restrict__ vd, const int32_t* __restrict__ views, int nviews,
-    uint32_t ntiles, uint32_t* __restrict__ counts, uint32_t* __restrict__ counts_plain) {
+    uint32_t ntiles, uint32_t* __restrict__ counts, uint32_t* __restrict__ counts_plain,
+    slab_stack stack) {
 #pragma clang fp contract(off)
   __shared__ uint32_t wc[2][kViewChunk][8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
   float px[kWinItems], py[kWinItems], pz[kWinItems], ih2[kWinItems], hr[kWinItems];
   bool in[kWinItems];
+  int bad = 0;
 #pragma unroll
   for (int r = 0; r < kWinItems; ++r) {
     const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
@@ -391,6 +491,38 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_count(
     const float h = in[r] ? hsml[i] : 1.0f;
     hr[r] = h;
     ih2[r] = 1.0f / (h * h);  // as the grid's SoA holds it
+    bad |= (isfinite(px[r]) && isfinite(py[r]) && isfinite(pz[r])) ? 0 : 1;
+  }
+  if (stack.n > 0) {
+    // a slab stack: two slabs to test per particle.  A tile with a
+    // non-finite coordinate takes the per-view loop (a NaN depth passes
+    // every centre mask there).
+    __shared__ float ss[kStackMax];
+    __shared__ int32_t sv[kStackMax];
+    __shared__ uint32_t sc[kStackMax];
+    __shared__ slab_par sp[kStackMax];
+    if (int(threadIdx.x) < stack.n) {
+      ss[threadIdx.x] = stack.s[threadIdx.x];
+      sv[threadIdx.x] = stack.view[threadIdx.x];
+      sc[threadIdx.x] = 0;
+      sp[threadIdx.x] = slab_par_of(vd[stack.view[threadIdx.x]].vp);
+    }
+    if (!__syncthreads_or(bad)) {
+#pragma unroll
+      for (int r = 0; r < kWinItems; ++r) {
+        if (!in[r]) continue;
+        const int k = last_slab_below(ss, stack.n, stack.m8 * pz[r]);
+        for (int u = max(k - 1, 0); u <= k; ++u)
+          if (slab_keeps(sp[u], px[r], py[r], pz[r], hr[r])) atomicAdd(&sc[u], 1u);
+      }
+      __syncthreads();
+      if (int(threadIdx.x) < stack.n) {
+        const uint64_t o = uint64_t(sv[threadIdx.x]) * ntiles + blockIdx.x;
+        counts[o] = sc[threadIdx.x];
+        counts_plain[o] = sc[threadIdx.x];
+      }
+      return;
+    }
   }
   int slot = 0;
   for (int k0 = 0; k0 < nviews; k0 += kViewChunk, slot ^= 1) {
@@ -513,22 +645,31 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     uint32_t* __restrict__ hqval, uint32_t* __restrict__ gkey, uint32_t* __restrict__ orig,
     uint32_t* __restrict__ err,
     const uint32_t* __restrict__ cbase, uint32_t* __restrict__ maxh_bits,
-    float2* __restrict__ zc) {
+    float2* __restrict__ zc, slab_stack stack) {
 #pragma clang fp contract(off)
   static_assert(kViewChunk <= 16 && kWinItems == 8, "keep bits: 8 views per 64-bit word");
-  __shared__ __align__(16) uint32_t wc[2][kViewChunk][kWinItems][4];
+  // per-wave kept counts: wc[2][kViewChunk][kWinItems][4] for the per-view
+  // loop, or [kWinItems][4][kStackMax] for a slab stack
+  constexpr int kCountWords = kWinItems * 4 * kStackMax;
+  static_assert(2 * kViewChunk * kWinItems * 4 <= kCountWords, "count table");
+  __shared__ __align__(16) uint32_t cwords[kCountWords];
+  auto& wc = *reinterpret_cast<uint32_t(*)[2][kViewChunk][kWinItems][4]>(cwords);
   // the tile's positions, read once (SoA); h and 1/h^2 in registers
   __shared__ float sx[kWinTile], sy[kWinTile], sz[kWinTile];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
   float hr[kWinItems], ih2[kWinItems];
+  int bad = 0;
 #pragma unroll
   for (int r = 0; r < kWinItems; ++r) {
     const uint64_t i = base + uint64_t(r) * kBlock + threadIdx.x;
     const bool ok = i < ps.n;
-    sx[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i] : 0.0f;
-    sy[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i + 1] : 0.0f;
-    sz[r * kBlock + threadIdx.x] = ok ? ps.pos3[3 * i + 2] : 0.0f;
+    const float x = ok ? ps.pos3[3 * i] : 0.0f, y = ok ? ps.pos3[3 * i + 1] : 0.0f;
+    const float z = ok ? ps.pos3[3 * i + 2] : 0.0f;
+    sx[r * kBlock + threadIdx.x] = x;
+    sy[r * kBlock + threadIdx.x] = y;
+    sz[r * kBlock + threadIdx.x] = z;
+    bad |= (isfinite(x) && isfinite(y) && isfinite(z)) ? 0 : 1;
     const float h = ok ? ps.hsml[i] : 1.0f;
     hr[r] = h;
     ih2[r] = 1.0f / (h * h);
@@ -585,6 +726,108 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     }
     return keep;
   };
+  // kept row j of view v: particle i, at tile position t
+  auto emit = [&](int v, uint32_t j, int t, uint64_t i) {
+    const view_proj vp = vd[v].vp;
+    const window_f win = vd[v].win;
+    const float vx = sx[t] - vp.c[0], vy = sy[t] - vp.c[1], vz = sz[t] - vp.c[2];
+    const float ax = dot3(vp.m + 0, vx, vy, vz);
+    const float ay = dot3(vp.m + 3, vx, vy, vz);
+    const float az = dot3(vp.m + 6, vx, vy, vz);
+    const grid_params& g = vd[v].g;
+    float w = vp.quantity == 1 ? wx[i] : static_cast<float>(vp.scale * double(ps.mass[i]));
+    // h rides in the z slot, so the SoA pass reads one line per row; the
+    // descending-h sort starts from (hq key, row)
+    const float hh = ps.hsml[i];
+    if (!depth_keep(az, hh, vp.depth, win.clip)) w = 0.0f;
+    if (zc) zc[j] = make_float2((0.0f - az) / hh, (vp.depth - az) / hh);
+    p4c[j] = make_float4(ax, ay, hh, w);
+    hqkey[j] = hq_key(hh);
+    hqval[j] = j;
+    bool in_grid;
+    const uint32_t key = cell_key(g, ax, ay, &in_grid);
+    if (!in_grid) atomicAdd(err, 1u);
+    gkey[j] = cbase[v] + key;
+    orig[j] = uint32_t(i);
+  };
+  if (stack.n > 0 && !__syncthreads_or(bad)) {
+    // A slab stack: each particle's two candidate slabs, tested with their
+    // views' arithmetic.  Kept rows of one slab go in the order of the
+    // per-view loop (item r, then wave, then lane): per (r, wave) each slab
+    // present gets its count and every lane its rank among the wave's lanes
+    // that keep the particle for that slab.  (A tile with a non-finite
+    // coordinate takes the per-view loop, as in the count pass.)
+    __shared__ float ss[kStackMax];
+    __shared__ int32_t sv[kStackMax];
+    __shared__ slab_par sp[kStackMax];
+    auto& cnt = *reinterpret_cast<uint32_t(*)[kWinItems][4][kStackMax]>(cwords);
+    if (int(threadIdx.x) < stack.n) {
+      ss[threadIdx.x] = stack.s[threadIdx.x];
+      sv[threadIdx.x] = stack.view[threadIdx.x];
+      sp[threadIdx.x] = slab_par_of(vd[stack.view[threadIdx.x]].vp);
+    }
+    for (int e = threadIdx.x; e < kCountWords; e += kBlock) cwords[e] = 0u;
+    __syncthreads();
+    const uint64_t below = (uint64_t(1) << lane) - 1u;
+    // per item: slab + 1 (7 bits) and rank (6 bits) of the two slots
+    uint32_t packed[kWinItems];
+#pragma unroll
+    for (int r = 0; r < kWinItems; ++r) {
+      const int t = r * kBlock + threadIdx.x;
+      const bool ok = base + uint64_t(t) < ps.n;
+      int a = -1, b = -1;
+      if (ok) {
+        const int k = last_slab_below(ss, stack.n, stack.m8 * sz[t]);
+        if (k >= 0 && slab_keeps(sp[k], sx[t], sy[t], sz[t], hr[r])) a = k;
+        if (k >= 1 && slab_keeps(sp[k - 1], sx[t], sy[t], sz[t], hr[r])) b = k - 1;
+      }
+      bool pa = a >= 0, pb = b >= 0;
+      uint32_t ra = 0, rb = 0;
+      for (;;) {
+        const uint64_t ma = __ballot(pa), mb = __ballot(pb);
+        if (!(ma | mb)) break;
+        // the slab of the lowest pending lane
+        const int l0 = __builtin_amdgcn_readfirstlane(int(__builtin_ctzll(ma ? ma : mb)));
+        const int sl = __builtin_amdgcn_readfirstlane(
+            ma ? __builtin_amdgcn_readlane(a, l0) : __builtin_amdgcn_readlane(b, l0));
+        const bool ia = pa && a == sl, ib = pb && b == sl;
+        const uint64_t peers = __ballot(ia || ib);
+        const uint32_t rk = uint32_t(__popcll(peers & below));
+        if (ia) {
+          ra = rk;
+          pa = false;
+        }
+        if (ib) {
+          rb = rk;
+          pb = false;
+        }
+        if (lane == 0) cnt[r][wave][sl] = uint32_t(__popcll(peers));
+      }
+      packed[r] = uint32_t(a + 1) | (ra << 7) | (uint32_t(b + 1) << 13) | (rb << 20);
+    }
+    __syncthreads();
+    // each slab's first position per (r, wave): its tile offset plus the rows
+    // before, in (r, wave) order
+    if (int(threadIdx.x) < stack.n) {
+      uint32_t run = offs[uint64_t(sv[threadIdx.x]) * ntiles + blockIdx.x];
+      for (int r = 0; r < kWinItems; ++r)
+        for (int w = 0; w < 4; ++w) {
+          const uint32_t c = cnt[r][w][threadIdx.x];
+          cnt[r][w][threadIdx.x] = run;
+          run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int r = 0; r < kWinItems; ++r) {
+      const int t = r * kBlock + threadIdx.x;
+      const uint64_t i = base + uint64_t(t);
+      const int a = int(packed[r] & 127u) - 1, b = int((packed[r] >> 13) & 127u) - 1;
+      if (a >= 0) emit(sv[a], cnt[r][wave][a] + ((packed[r] >> 7) & 63u), t, i);
+      if (b >= 0) emit(sv[b], cnt[r][wave][b] + ((packed[r] >> 20) & 63u), t, i);
+    }
+    return;
+  }
   int slot = 0;
   for (int k0 = 0; k0 < nviews; k0 += kViewChunk, slot ^= 1) {
     const int nk = min(kViewChunk, nviews - k0);
@@ -617,9 +860,6 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     __syncthreads();
     for (int kk = 0; kk < nk; ++kk) {
       const int v = views[k0 + kk];
-      const view_proj vp = vd[v].vp;
-      const window_f win = vd[v].win;
-      const uint32_t cb = cbase[v];
       const uint32_t bits = uint32_t((kk < 8 ? bits_lo : bits_hi) >> (8 * (kk & 7))) & 0xffu;
       uint32_t out = offs[uint64_t(v) * ntiles + blockIdx.x];
 #pragma unroll 1
@@ -634,29 +874,11 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
         const bool keep = (bits >> r) & 1u;
         const uint64_t bal = __ballot(keep);
         if (keep) {
-          const float vx = sx[t] - vp.c[0], vy = sy[t] - vp.c[1], vz = sz[t] - vp.c[2];
-          const float ax = dot3(vp.m + 0, vx, vy, vz);
-          const float ay = dot3(vp.m + 3, vx, vy, vz);
-          const float az = dot3(vp.m + 6, vx, vy, vz);
-          const grid_params& g = vd[v].g;
           const uint32_t j =
               out + before +
               __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
                                         __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
-          float w = vp.quantity == 1 ? wx[i] : static_cast<float>(vp.scale * double(ps.mass[i]));
-          // h rides in the z slot, so the SoA pass reads one line per row;
-          // the descending-h sort starts from (hq key, row)
-          const float hh = ps.hsml[i];
-          if (!depth_keep(az, hh, vp.depth, win.clip)) w = 0.0f;
-          if (zc) zc[j] = make_float2((0.0f - az) / hh, (vp.depth - az) / hh);
-          p4c[j] = make_float4(ax, ay, hh, w);
-          hqkey[j] = hq_key(hh);
-          hqval[j] = j;
-          bool in_grid;
-          const uint32_t key = cell_key(g, ax, ay, &in_grid);
-          if (!in_grid) atomicAdd(err, 1u);
-          gkey[j] = cb + key;
-          orig[j] = uint32_t(i);
+          emit(v, j, t, i);
         }
         out += tot;
       }
@@ -884,13 +1106,15 @@ void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  uint32_t* hqval,
                                  uint32_t* gkey, uint32_t* orig, uint32_t* err,
                                  const uint32_t* cbase, uint32_t* maxh_bits, float2* zc,
-                                 hipStream_t s) {
+                                 const slab_stack& stack, hipStream_t s) {
   const uint32_t nb = window_tiles(ps.n);
   if (!nb || nviews <= 0) return;
   ILC_REQUIRE(cbase != nullptr, "the scatter needs the views' first cells");
+  ILC_REQUIRE(stack.n == 0 || (stack.n == nviews && stack.n <= kStackMax && zc == nullptr),
+              "a slab stack lists every view, at most %d, centre masks only", kStackMax);
   hipLaunchKernelGGL(k_multi_window_scatter, dim3(nb), dim3(kBlock), 0, s, ps, wx, vd_dev,
                      views_dev, nviews, nb, offs, p4c, hqkey, hqval, gkey, orig, err, cbase,
-                     maxh_bits, zc);
+                     maxh_bits, zc, stack);
   ILC_HIP(hipGetLastError());
 }
 
@@ -927,14 +1151,14 @@ void launch_order_items(const view_desc* views, const uint2* items, const queue_
 
 void launch_multi_bbox(const particle_soa& ps, const view_build_desc* vd_dev, int nv,
                        const gaunt_table& gt, float* partial, uint32_t* bad, bbox_result* out,
-                       hipStream_t s) {
+                       bool one_group, hipStream_t s) {
   if (nv <= 0) return;
   if (gt.n > kMaxGaunt || (gt.n > 0 && !gt.y))
     fail(ILC_EINVAL, "Gaunt table of %d entries (at most %d)", gt.n, kMaxGaunt);
   const int nparts = multi_bbox_parts(ps.n);
   launch_fill_u32(bad, 0u, uint64_t(nv), s);
   hipLaunchKernelGGL(k_multi_bbox, dim3(nparts), dim3(kBlock), 0, s, ps, vd_dev, nv, gt, partial,
-                     bad);
+                     bad, one_group ? 1 : 0);
   ILC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_multi_bbox_final, dim3(nv), dim3(kBlock), 0, s, partial, nparts, bad, out);
   ILC_HIP(hipGetLastError());
@@ -943,11 +1167,14 @@ void launch_multi_bbox(const particle_soa& ps, const view_build_desc* vd_dev, in
 void launch_multi_window_count(const float* pos3, const float* hsml, uint64_t n,
                                const view_build_desc* vd_dev, const int32_t* views_dev,
                                int nviews, uint32_t* counts, uint32_t* totals,
-                               uint32_t* counts_plain, uint32_t* totals_plain, hipStream_t s) {
+                               uint32_t* counts_plain, uint32_t* totals_plain,
+                               const slab_stack& stack, hipStream_t s) {
   const uint32_t nb = window_tiles(n);
   if (!nb || nviews <= 0) return;
+  ILC_REQUIRE(stack.n == 0 || (stack.n == nviews && stack.n <= kStackMax),
+              "a slab stack lists every view, at most %d", kStackMax);
   hipLaunchKernelGGL(k_multi_window_count, dim3(nb), dim3(kBlock), 0, s, pos3, hsml, n, vd_dev,
-                     views_dev, nviews, nb, counts, counts_plain);
+                     views_dev, nviews, nb, counts, counts_plain, stack);
   ILC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_count_totals, dim3(uint32_t(nviews)), dim3(kBlock), 0, s, views_dev, nb,
                      counts, totals, counts_plain, totals_plain);

@@ -111,20 +111,39 @@ struct view_build_desc {
 };
 // Bounding boxes of every view in one read of the snapshot: out[v] as
 // launch_view_bbox would give it (every view's vp.scale must be 1).  partial
-// holds multi_bbox_parts(n) * 8 * nv floats, bad nv words.
+// holds multi_bbox_parts(n) * 8 * nv floats, bad nv words.  one_group: every
+// view is a slab of one slab group (the same x', y' projection bit for bit,
+// the same quantity).
 int multi_bbox_parts(uint64_t n);
 void launch_multi_bbox(const particle_soa& ps, const view_build_desc* vd_dev, int nv,
                        const gaunt_table& gt, float* partial, uint32_t* bad, bbox_result* out,
-                       hipStream_t s);
+                       bool one_group, hipStream_t s);
+// A slab stack: the listed views are slabs of one stack (one slab group:
+// the same x', y' projection; depth row (0, 0, m8) with m8 = +-1; centre
+// masks, no window or screen test) whose depth ranges [s_k, s_k + depth_k],
+// s_k = m8 * cz_k, follow one another without overlap, with room for FP32
+// rounding.  A particle at t = m8 * z can then only be kept by the last slab
+// with s_k <= t or the one before it: the multi-view passes test those two
+// with the views' own arithmetic instead of every slab.  n = 0: no stack.
+constexpr int kStackMax = 64;
+struct slab_stack {
+  int n = 0;
+  float m8 = 1.0f;
+  float s[kStackMax];    // s_k in ascending order
+  int32_t view[kStackMax];  // the view of the k-th slab
+};
+
 // Kept particles per window tile (window_tiles(n) of them) of every listed view
 // in one read: counts[v * ntiles + t], totals[v] += the same (not cleared).
 // counts / totals: kept rows (window, depth cut and screen test);
 // counts_plain / totals_plain: the same without the screen test (the rows a
 // single view's window scatter keeps)
+// stack (n > 0): the listed views are that stack's slabs, every one of them.
 void launch_multi_window_count(const float* pos3, const float* hsml, uint64_t n,
                                const view_build_desc* vd_dev, const int32_t* views_dev,
                                int nviews, uint32_t* counts, uint32_t* totals,
-                               uint32_t* counts_plain, uint32_t* totals_plain, hipStream_t s);
+                               uint32_t* counts_plain, uint32_t* totals_plain,
+                               const slab_stack& stack, hipStream_t s);
 uint32_t window_tiles(uint64_t n);
 // X-ray weight of every particle (the quantity of particle_of, view-free).
 void launch_particle_xray(const particle_soa& ps, const gaunt_table& gt, float* wx,
@@ -143,7 +162,7 @@ void launch_multi_window_scatter(const particle_soa& ps, const float* wx,
                                  int nviews, const uint32_t* offs, float4* p4c, uint32_t* hqkey,
                                  uint32_t* hqval, uint32_t* gkey, uint32_t* orig, uint32_t* err,
                                  const uint32_t* cbase, uint32_t* maxh_bits, float2* zc,
-                                 hipStream_t s);
+                                 const slab_stack& stack, hipStream_t s);
 // out[i] = src[idx[i]]
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* out,
                        hipStream_t s);

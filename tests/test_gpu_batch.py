@@ -347,6 +347,35 @@ def test_slab_stack_shares_one_grid(B, orc, mode, tile):
     assert np.array_equal(r.image_f64(2).cpu().numpy().ravel(), got)
 
 
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+def test_slab_stack_pass_equals_per_view_pass(B, mode):
+    """The multi-view passes of a slab stack test two slabs per particle (the
+    last slab starting at or before its depth, and the one before); beside a
+    view that is no slab they test every view.  The slabs' kept rows, their
+    order, grids and images are the same either way, bit for bit."""
+    from illcrawl_amd import synth
+    say(f"case: slab stack pass vs per-view pass, mode={mode}")
+    snap = synth.make_snapshot(1 << 16, 12, 10000.0, n_halos=3, rs_range=(200.0, 500.0),
+                               gas=True)
+    tab = synth.gaunt_table_1d()
+    slabs = synth.slab_views(snap.box, n_slabs=6, thickness=800.0, nx=64, width=8000.0,
+                             quantity=1)
+    for v in slabs:
+        v["scale"] = 1.0
+    other = synth.centred_view(snap.box, look_at=(0.3, 0.2, 0.9), width=6000.0, nx=64)
+    other.update(quantity=1, scale=1.0)
+    r = make_batch(B, snap, slabs, mode, tab)
+    r.render()
+    a = [r.image_f64(k).cpu().numpy() for k in range(len(slabs))]
+    kept = [r.view_info(k)["num_particles"] for k in range(len(slabs))]
+    r2 = make_batch(B, snap, slabs + [other], mode, tab)
+    r2.render()
+    for k in range(len(slabs)):
+        assert r2.view_info(k)["num_particles"] == kept[k], k
+        assert np.array_equal(r2.image_f64(k).cpu().numpy(), a[k]), k
+    assert sum(kept) < snap.n and min(kept) > 0
+
+
 def test_batch_backward_equals_sum_of_single_views(B):
     """ilc_batch_backward (exact mode) equals, per particle, the sum over
     views of the single-view adjoint (itself pinned to the FP64 oracle)
