@@ -755,7 +755,8 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     // views' arithmetic.  Kept rows of one slab go in the order of the
     // per-view loop (item r, then wave, then lane): per (r, wave) each slab
     // present gets its count and every lane its rank among the wave's lanes
-    // that keep the particle for that slab.  (A tile with a non-finite
+    // that keep the particle for that slab (a lane's two slabs differ).  The
+    // first lane of each slab writes the count.  (A tile with a non-finite
     // coordinate takes the per-view loop, as in the count pass.)
     __shared__ float ss[kStackMax];
     __shared__ int32_t sv[kStackMax];
@@ -781,27 +782,34 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
         if (k >= 0 && slab_keeps(sp[k], sx[t], sy[t], sz[t], hr[r])) a = k;
         if (k >= 1 && slab_keeps(sp[k - 1], sx[t], sy[t], sz[t], hr[r])) b = k - 1;
       }
-      bool pa = a >= 0, pb = b >= 0;
+      // multi-split on the slab index: the ballots of its six bits in each
+      // slot give, for any slab x, the lanes holding x in that slot
+      uint64_t va = __ballot(a >= 0), vb = __ballot(b >= 0);
+      uint64_t ba[6], bb[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        ba[q] = __ballot(a >= 0 && ((a >> q) & 1));
+        bb[q] = __ballot(b >= 0 && ((b >> q) & 1));
+      }
+      auto peers_of = [&](int x) {
+        uint64_t pa = va, pb = vb;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          pa &= ((x >> q) & 1) ? ba[q] : ~ba[q];
+          pb &= ((x >> q) & 1) ? bb[q] : ~bb[q];
+        }
+        return pa | pb;
+      };
       uint32_t ra = 0, rb = 0;
-      for (;;) {
-        const uint64_t ma = __ballot(pa), mb = __ballot(pb);
-        if (!(ma | mb)) break;
-        // the slab of the lowest pending lane
-        const int l0 = __builtin_amdgcn_readfirstlane(int(__builtin_ctzll(ma ? ma : mb)));
-        const int sl = __builtin_amdgcn_readfirstlane(
-            ma ? __builtin_amdgcn_readlane(a, l0) : __builtin_amdgcn_readlane(b, l0));
-        const bool ia = pa && a == sl, ib = pb && b == sl;
-        const uint64_t peers = __ballot(ia || ib);
-        const uint32_t rk = uint32_t(__popcll(peers & below));
-        if (ia) {
-          ra = rk;
-          pa = false;
-        }
-        if (ib) {
-          rb = rk;
-          pb = false;
-        }
-        if (lane == 0) cnt[r][wave][sl] = uint32_t(__popcll(peers));
+      if (a >= 0) {
+        const uint64_t pe = peers_of(a);
+        ra = uint32_t(__popcll(pe & below));
+        if (ra == 0) cnt[r][wave][a] = uint32_t(__popcll(pe));
+      }
+      if (b >= 0) {
+        const uint64_t pe = peers_of(b);
+        rb = uint32_t(__popcll(pe & below));
+        if (rb == 0) cnt[r][wave][b] = uint32_t(__popcll(pe));
       }
       packed[r] = uint32_t(a + 1) | (ra << 7) | (uint32_t(b + 1) << 13) | (rb << 20);
     }
