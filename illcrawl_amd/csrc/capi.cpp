@@ -7,6 +7,8 @@
 #include <cstring>
 #include <new>
 #include <memory>
+#include <exception>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -626,6 +628,64 @@ void crawl_block_body(const ilc_volume* vol, const float* xyz_dev, uint64_t n,
 }
 }  // namespace
 
+// Scratch of ilc_crawl_block (tile counts, offsets, scan sums), leased from a
+// per-process pool: a call takes a free buffer of its device (allocating, or
+// growing one, only when none is large enough) and returns it when done, so
+// repeated blocks neither allocate nor free (hipFree synchronises the whole
+// device), and concurrent calls get different buffers.  The pool lives until
+// the process ends.
+struct crawl_pool {
+  struct buf {
+    int device;
+    uint32_t* p;
+    uint64_t words;
+  };
+  std::mutex m;
+  std::vector<buf> free;
+};
+
+crawl_pool& the_crawl_pool() {
+  static crawl_pool* pool = new crawl_pool;  // never destroyed
+  return *pool;
+}
+
+struct crawl_lease {
+  crawl_pool::buf b{-1, nullptr, 0};
+  uint32_t* get(uint64_t words) {
+    int dev = 0;
+    ILC_HIP(hipGetDevice(&dev));
+    crawl_pool& pool = the_crawl_pool();
+    {
+      std::lock_guard<std::mutex> lk(pool.m);
+      for (size_t k = 0; k < pool.free.size(); ++k)
+        if (pool.free[k].device == dev && pool.free[k].words >= words) {
+          b = pool.free[k];
+          pool.free.erase(pool.free.begin() + long(k));
+          return b.p;
+        }
+      // the device's largest free buffer is too small: replace it
+      for (size_t k = 0; k < pool.free.size(); ++k)
+        if (pool.free[k].device == dev) {
+          (void)hipFree(pool.free[k].p);
+          pool.free.erase(pool.free.begin() + long(k));
+          break;
+        }
+    }
+    void* p = nullptr;
+    ILC_HIP(hipMalloc(&p, sizeof(uint32_t) * words));
+    b = crawl_pool::buf{dev, static_cast<uint32_t*>(p), words};
+    return b.p;
+  }
+  ~crawl_lease() {
+    if (!b.p) return;
+    // on an error the block's kernels may still be running on the buffer
+    if (std::uncaught_exceptions() > 0) (void)hipDeviceSynchronize();
+    crawl_pool& pool = the_crawl_pool();
+    std::lock_guard<std::mutex> lk(pool.m);
+    pool.free.push_back(b);
+  }
+};
+
 extern "C" {
 
 ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t n,
@@ -633,18 +693,10 @@ ilc_status ilc_crawl_block(const ilc_volume* vol, const float* xyz_dev, uint64_t
                            float* particles4_dev, float* const* columns_out_dev,
                            uint64_t* index_out_dev, uint64_t* n_out, void* stream) {
   return guarded([&] {
-    uint32_t* work = nullptr;
-    struct freer {
-      uint32_t** p;
-      ~freer() {
-        if (*p) (void)hipFree(*p);
-      }
-    } fr{&work};
+    crawl_lease lease;
     crawl_block_body(vol, xyz_dev, n, columns_dev, ncolumns, particles4_dev, columns_out_dev,
-                     index_out_dev, n_out, stream, [&](uint64_t words) {
-                       ILC_HIP(hipMalloc(&work, sizeof(uint32_t) * words));
-                       return work;
-                     });
+                     index_out_dev, n_out, stream,
+                     [&](uint64_t words) { return lease.get(words); });
   });
 }
 
