@@ -61,6 +61,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# every gather tile shape (w, h, threads per workgroup)
+PLAIN_TILES = [(16, 16, 256), (32, 16, 256), (32, 32, 256), (64, 16, 256), (8, 8, 64),
+               (16, 16, 64)]
+
+
 def parse_tile(text):
     """'WxH' (one tile per 256-thread workgroup) or 'WxH/G' (G threads per
     workgroup: 256, or 64 for one-wave 16x16 tiles) -> (W, H, G)."""
@@ -682,9 +687,25 @@ def main():
     gather_ms = float(np.mean(gather))
     log(f"[bench] batched, eager: {eager_ms:.2f} ms/step, gather {gather_ms:.2f} ms")
 
-    # plain: one view per launch sequence
-    plain_ms, plain_gather = None, None
+    # plain: one view per launch sequence, at the tile shape that is fastest
+    # for the plain path itself (timed over every shape, one render each),
+    # not at the batch's
+    plain_ms, plain_gather, plain_tile = None, None, None
     if args.plain_steps > 0:
+        batch_tile = r.tile()
+        best = None
+        for cand in PLAIN_TILES:
+            r.set_tile(*cand)
+            r.render(plain=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r.render(plain=True)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            if best is None or ms < best[0]:
+                best = (ms, cand)
+        plain_tile = best[1]
+        r.set_tile(*plain_tile)
         r.render(plain=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -695,7 +716,9 @@ def main():
         torch.cuda.synchronize()
         plain_ms = (time.perf_counter() - t0) * 1e3 / args.plain_steps
         plain_gather = float(np.mean(pg))
-        log(f"[bench] plain: {plain_ms:.2f} ms/step, gathers {plain_gather:.2f} ms")
+        log(f"[bench] plain: {plain_ms:.2f} ms/step, gathers {plain_gather:.2f} ms "
+            f"(its fastest tile {plain_tile})")
+        r.set_tile(*batch_tile)
         r.render()
 
     value = total_evals / (ms_per_step * 1e-3)
@@ -723,7 +746,8 @@ def main():
         "pixels_per_s": npix / (ms_per_step * 1e-3),
         "batched_vs_plain": (plain_ms / ms_per_step) if plain_ms else None,
         "plain": dict(ms_per_step=plain_ms, evals_per_s=(total_evals / (plain_ms * 1e-3))
-                      if plain_ms else None, gather_ms_sum=plain_gather),
+                      if plain_ms else None, gather_ms_sum=plain_gather,
+                      tile=list(plain_tile) if plain_tile else None),
         "timing": "hipGraph replay of each step; eager launches below",
         "eager": dict(ms_per_step=eager_ms, evals_per_s=total_evals / (eager_ms * 1e-3)),
         "gather_ms": gather_ms,
