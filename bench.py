@@ -53,6 +53,9 @@ FLOP_PER_EVAL_3D = 13
 MEASURED_TRAFFIC = {("c3", "reference", (16, 16, 256)): dict(read=22.61e9, write=3.250e9),
                     # profiles/k_gather_batch_xray_64M_32slabs.md
                     ("c5", "reference", (16, 16, 256)): dict(read=21.16e9, write=19.59e9),
+                    # profiles/k_gather_batch_colden_1M_512_and_xray_16M_2048.md
+                    ("c1", "reference", (8, 8, 64)): dict(read=28.80e6, write=10.97e6),
+                    ("c2", "reference", (16, 16, 256)): dict(read=252.97e6, write=161.38e6),
                     # k_smooth3d, profiles/k_smooth3d_slices_16M.md
                     ("v1", "reference", (16, 16)): dict(read=1.244e9, write=0.461e9)}
 
