@@ -156,75 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float4* __restrict__ p4, 
   vals[i] = i;
 }
 
-constexpr int kItems = 16;
-constexpr int kTile = kBlock * kItems;  // 4096 keys per workgroup
-
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys,
-                                                        uint32_t n, int shift, uint32_t ntiles,
-                                                        uint32_t* __restrict__ hist) {
-  __shared__ uint32_t hh[256];
-  hh[threadIdx.x] = 0;
-  __syncthreads();
-  const uint32_t base = blockIdx.x * kTile;
-  for (int r = 0; r < kItems; ++r) {
-    const uint32_t i = base + r * kBlock + threadIdx.x;
-    if (i < n) atomicAdd(&hh[(keys[i] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  hist[threadIdx.x * ntiles + blockIdx.x] = hh[threadIdx.x];
-}
-
-// Stable scatter of one 8-bit digit.  Items of a tile are visited in input
-// order (round r, thread t -> item r*256 + t); within a wave, lanes sharing a
-// digit are found with 8 ballots and ranked by popcount of lower lanes;
-// across waves and rounds, per-digit running counts in LDS keep the order.
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ kin,
-                                                           const uint32_t* __restrict__ vin,
-                                                           uint32_t* __restrict__ kout,
-                                                           uint32_t* __restrict__ vout, uint32_t n,
-                                                           int shift, uint32_t ntiles,
-                                                           const uint32_t* __restrict__ offs) {
-  __shared__ uint32_t base_d[256];
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wc[4][256];
-  const uint32_t t = threadIdx.x;
-  const int wave = t >> 6;
-  base_d[t] = offs[t * ntiles + blockIdx.x];
-  run[t] = 0;
-  for (int w = 0; w < 4; ++w) wc[w][t] = 0;
-  __syncthreads();
-  const uint64_t below = lanes_below();
-  const uint32_t tile0 = blockIdx.x * kTile;
-  for (int r = 0; r < kItems; ++r) {
-    const uint32_t i = tile0 + r * kBlock + t;
-    const bool valid = i < n;
-    const uint32_t k = valid ? kin[i] : 0u;
-    const uint32_t v = valid ? vin[i] : 0u;
-    const uint32_t d = (k >> shift) & 255u;
-    uint64_t peers = __ballot(valid);
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
-    }
-    const uint32_t rank = __popcll(peers & below);
-    if (valid && rank == 0) wc[wave][d] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = base_d[d] + run[d] + rank;
-      for (int w = 0; w < wave; ++w) pos += wc[w][d];
-      kout[pos] = k;
-      vout[pos] = v;
-    }
-    __syncthreads();
-    run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
-    for (int w = 0; w < 4; ++w) wc[w][t] = 0;
-    __syncthreads();
-  }
-}
-
-// ---- exclusive scan of uint32 (reduce, scan of block sums, apply)
-
+// exclusive scan over the workgroup (kBlock threads)
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
   __shared__ uint32_t ws[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -245,12 +177,149 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
   return off + incl - v;
 }
 
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;  // 4096 keys per workgroup
+
+// The lanes of the wave whose digit equals this lane's (8 ballots), among
+// the valid lanes.
+__device__ inline uint64_t digit_peers(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+// Per-tile digit counts, one LDS add per distinct digit of a wave (a digit
+// with few distinct values would serialise a wave's 64 adds on one word).
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys,
+                                                        uint32_t n, int shift, uint32_t ntiles,
+                                                        uint32_t* __restrict__ hist) {
+  __shared__ uint32_t hh[256];
+  hh[threadIdx.x] = 0;
+  const uint32_t base = blockIdx.x * kTile;
+  uint32_t k[kItems];
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * kBlock + threadIdx.x;
+    k[r] = i < n ? keys[i] : 0u;
+  }
+  __syncthreads();
+  const uint64_t below = lanes_below();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const bool valid = base + r * kBlock + threadIdx.x < n;
+    const uint32_t d = (k[r] >> shift) & 255u;
+    const uint64_t peers = digit_peers(d, valid);
+    if (valid && (peers & below) == 0) atomicAdd(&hh[d], uint32_t(__popcll(peers)));
+  }
+  __syncthreads();
+  hist[threadIdx.x * ntiles + blockIdx.x] = hh[threadIdx.x];
+}
+
+// Stable scatter of one 8-bit digit.  Items of a tile are visited in input
+// order (round r, thread t -> item r*256 + t); within a wave, lanes sharing a
+// digit are found with 8 ballots and ranked by popcount of lower lanes;
+// across waves and rounds, per-digit running counts in LDS keep the order.
+// The tile is first ordered by digit in LDS and then written out in that
+// order, so each digit's items leave as one contiguous run: item i of the
+// ordered tile, of digit d, goes to offs[d, tile] + i - (the tile's items
+// below digit d).
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout,
+                                                           uint32_t* __restrict__ vout, uint32_t n,
+                                                           int shift, uint32_t ntiles,
+                                                           const uint32_t* __restrict__ offs) {
+  __shared__ uint32_t sk[kTile];
+  __shared__ uint32_t sv[kTile];
+  __shared__ uint32_t gofs[256];
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wc[4][256];
+  const uint32_t t = threadIdx.x;
+  const int wave = t >> 6;
+  const uint32_t tile0 = blockIdx.x * kTile;
+  uint32_t k[kItems], v[kItems];
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = tile0 + r * kBlock + t;
+    k[r] = i < n ? kin[i] : 0u;
+    v[r] = i < n ? vin[i] : 0u;
+  }
+  // the tile's count of digit t from the scanned counts (the last entry's
+  // successor is the total, n)
+  const uint64_t e = uint64_t(t) * ntiles + blockIdx.x;
+  const uint32_t g0 = offs[e];
+  const uint32_t cnt = (e + 1 < uint64_t(256) * ntiles ? offs[e + 1] : n) - g0;
+  uint32_t tot;
+  const uint32_t ls = block_exclusive_scan(cnt, &tot);
+  run[t] = ls;
+  gofs[t] = g0 - ls;
+  for (int w = 0; w < 4; ++w) wc[w][t] = 0;
+  __syncthreads();
+  const uint64_t below = lanes_below();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const bool valid = tile0 + r * kBlock + t < n;
+    const uint32_t d = (k[r] >> shift) & 255u;
+    const uint64_t peers = digit_peers(d, valid);
+    const uint32_t rank = __popcll(peers & below);
+    if (valid && rank == 0) wc[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wc[w][d];
+      sk[pos] = k[r];
+      sv[pos] = v[r];
+    }
+    __syncthreads();
+    run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+    for (int w = 0; w < 4; ++w) wc[w][t] = 0;
+    __syncthreads();
+  }
+  const uint32_t nv = min(uint32_t(kTile), n - tile0);
+  for (uint32_t i = t; i < nv; i += kBlock) {
+    const uint32_t key = sk[i];
+    const uint32_t pos = gofs[(key >> shift) & 255u] + i;
+    kout[pos] = key;
+    vout[pos] = sv[i];
+  }
+}
+
+// ---- exclusive scan of uint32 (reduce, scan of block sums, apply)
+
+
+// A workgroup scans one tile of kTile words.  A whole tile of a 16-byte
+// aligned array is read as kItems / 4 passes of one 16-byte vector per
+// thread over consecutive 4 KB (coalesced); vector j * kBlock + t holds words
+// 4 (j * kBlock + t) .. + 3 of the tile.  Other tiles read one word per thread
+// and pass, the same way.
+constexpr int kVecPasses = kItems / 4;
+static_assert(kItems % 4 == 0, "whole vectors per thread");
+
+__device__ inline bool whole_vec_tile(const void* p, uint64_t base, uint64_t n) {
+  return base + kTile <= n && (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
                                                          uint32_t* __restrict__ sums) {
-  const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
+  const uint64_t b0 = uint64_t(blockIdx.x) * kTile;
   uint32_t s = 0;
-  for (int k = 0; k < kItems; ++k)
-    if (base + k < n) s += in[base + k];
+  if (whole_vec_tile(in, b0, n)) {
+    const uint4* v = reinterpret_cast<const uint4*>(in + b0);
+#pragma unroll
+    for (int j = 0; j < kVecPasses; ++j) {
+      const uint4 q = v[j * kBlock + threadIdx.x];
+      s += (q.x + q.y) + (q.z + q.w);
+    }
+  } else {
+    for (int k = 0; k < kItems; ++k) {
+      const uint64_t i = b0 + uint64_t(k) * kBlock + threadIdx.x;
+      if (i < n) s += in[i];
+    }
+  }
   s = wave_sum(s);
   __shared__ uint32_t ws[4];
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
@@ -281,7 +350,31 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums(uint32_t* __restrict__ sum
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ in,
                                                         uint32_t* __restrict__ out, uint64_t n,
                                                         const uint32_t* __restrict__ sums) {
-  const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
+  const uint64_t b0 = uint64_t(blockIdx.x) * kTile;
+  if (whole_vec_tile(in, b0, n) && (reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
+    // the tile's vectors in order, kBlock at a time: a block scan of the
+    // vector sums of each pass, carried over the passes
+    const uint4* vin = reinterpret_cast<const uint4*>(in + b0);
+    uint4* vout = reinterpret_cast<uint4*>(out + b0);
+    uint4 q[kVecPasses];
+#pragma unroll
+    for (int j = 0; j < kVecPasses; ++j) q[j] = vin[j * kBlock + threadIdx.x];
+    uint32_t carry = sums[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kVecPasses; ++j) {
+      uint32_t tot;
+      const uint32_t off = block_exclusive_scan((q[j].x + q[j].y) + (q[j].z + q[j].w), &tot) + carry;
+      uint4 o;
+      o.x = off;
+      o.y = o.x + q[j].x;
+      o.z = o.y + q[j].y;
+      o.w = o.z + q[j].z;
+      vout[j * kBlock + threadIdx.x] = o;
+      carry += tot;
+    }
+    return;
+  }
+  const uint64_t base = b0 + uint64_t(threadIdx.x) * kItems;
   uint32_t v[kItems], s = 0;
   for (int k = 0; k < kItems; ++k) {
     v[k] = base + k < n ? in[base + k] : 0u;
