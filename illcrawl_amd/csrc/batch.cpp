@@ -435,6 +435,14 @@ void plan(ilc_batch* b, hipStream_t s) {
       vs.win.sy0 = std::nextafter(float(-hy - mg), -INFINITY);
       vs.win.sy1 = std::nextafter(float(hy + mg), INFINITY);
       vs.win.cull2 = sup * sup * 1.0001f;
+      // two cells of margin absorb the FP32 cell arithmetic of the device
+      auto cell_at = [&](float x, int a) {
+        return std::floor((double(x) - double(vs.gp.minc[a])) / double(vs.gp.cs[a]));
+      };
+      vs.win.scx0 = int(std::max(cell_at(vs.win.sx0, 0) - 2.0, -1.0));
+      vs.win.scx1 = int(std::min(cell_at(vs.win.sx1, 0) + 2.0, double(vs.gp.nc[0])));
+      vs.win.scy0 = int(std::max(cell_at(vs.win.sy0, 1) - 2.0, -1.0));
+      vs.win.scy1 = int(std::min(cell_at(vs.win.sy1, 1) + 2.0, double(vs.gp.nc[1])));
     } else if (vs.compact) {
       vs.win = window_f{0.f, 0.f, 0.f, 0.f, 0, 1};
     } else {

@@ -656,6 +656,8 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
   auto& wc = *reinterpret_cast<uint32_t(*)[2][kViewChunk][kWinItems][4]>(cwords);
   // the tile's positions, read once (SoA); h and 1/h^2 in registers
   __shared__ float sx[kWinTile], sy[kWinTile], sz[kWinTile];
+  // one view's kept tile positions (the per-view loop)
+  __shared__ uint16_t klist[kWinTile];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t base = uint64_t(blockIdx.x) * kWinTile;
   float hr[kWinItems], ih2[kWinItems];
@@ -677,7 +679,8 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
   __syncthreads();
   // cell key of (x', y', 0) in a view's grid: grid3d_get_cell_indices,
   // particle_grid.cl:45-53
-  auto cell_key = [](const grid_params& g, float ax, float ay, bool* in_grid) {
+  auto cell_key = [](const grid_params& g, float ax, float ay, bool* in_grid,
+                     int32_t* cxy = nullptr) {
     const float c3[3] = {ax, ay, 0.0f};
     int32_t idx[3];
     bool ok = true;
@@ -687,6 +690,10 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
       ok = ok && idx[a] >= 0 && idx[a] < g.nc[a];
     }
     *in_grid = ok;
+    if (cxy) {
+      cxy[0] = idx[0];
+      cxy[1] = idx[1];
+    }
     return ok ? uint32_t(idx[0]) + uint32_t(idx[1]) * uint32_t(g.nc[0]) +
                     uint32_t(idx[2]) * uint32_t(g.nc[0]) * uint32_t(g.nc[1])
               : 0u;
@@ -713,14 +720,16 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
       if (!reaches_screen(win, ax, ay, ih2r)) {
         keep = false;
         bool in_grid;
-        const uint32_t key = cell_key(g, ax, ay, &in_grid);
-        if (in_grid) {
+        int32_t cxy[2];
+        const uint32_t key = cell_key(g, ax, ay, &in_grid, cxy);
+        if (!in_grid) {
+          atomicAdd(err, 1u);
+        } else if (cxy[0] >= win.scx0 && cxy[0] <= win.scx1 && cxy[1] >= win.scy0 &&
+                   cxy[1] <= win.scy1) {
           uint32_t* m = maxh_bits + cb + key;
           const uint32_t hb = __float_as_uint(h);
           if (hb > __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(m, hb);
-        } else {
-          atomicAdd(err, 1u);
         }
       }
     }
@@ -869,27 +878,32 @@ __global__ __launch_bounds__(kBlock) void k_multi_window_scatter(
     for (int kk = 0; kk < nk; ++kk) {
       const int v = views[k0 + kk];
       const uint32_t bits = uint32_t((kk < 8 ? bits_lo : bits_hi) >> (8 * (kk & 7))) & 0xffu;
-      uint32_t out = offs[uint64_t(v) * ntiles + blockIdx.x];
-#pragma unroll 1
+      // the view's kept tile positions in row order (item r, then wave,
+      // then lane), listed in LDS; then every thread emits consecutive rows
+      uint32_t total = 0;
+#pragma unroll
       for (int r = 0; r < kWinItems; ++r) {
         const uint4 cw = *reinterpret_cast<const uint4*>(wc[slot][kk][r]);
-        const uint32_t tot = cw.x + cw.y + cw.z + cw.w;
-        if (tot == 0) continue;
         const uint32_t before = (wave > 0 ? cw.x : 0u) + (wave > 1 ? cw.y : 0u) +
                                 (wave > 2 ? cw.z : 0u);
-        const int t = r * kBlock + threadIdx.x;
-        const uint64_t i = base + uint64_t(t);
         const bool keep = (bits >> r) & 1u;
         const uint64_t bal = __ballot(keep);
-        if (keep) {
-          const uint32_t j =
-              out + before +
-              __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
-                                        __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
-          emit(v, j, t, i);
-        }
-        out += tot;
+        if (keep)
+          klist[total + before +
+                __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
+                                          __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u))] =
+              uint16_t(r * kBlock + threadIdx.x);
+        total += cw.x + cw.y + cw.z + cw.w;
       }
+      if (total == 0) continue;
+      __syncthreads();
+      const uint32_t out = offs[uint64_t(v) * ntiles + blockIdx.x];
+      for (uint32_t e = threadIdx.x; e < total; e += kBlock) {
+        const int t = klist[e];
+        emit(v, out + e, t, base + uint64_t(t));
+      }
+      // the list is rewritten for the next view
+      __syncthreads();
     }
   }
 }

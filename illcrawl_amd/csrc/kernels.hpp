@@ -98,6 +98,10 @@ struct window_f {
   // cull arithmetic) only raise their cell's max h
   int use_screen;
   float sx0, sx1, sy0, sy1, cull2;
+  // the grid cells that can hold a pixel point (the screen's cell range with
+  // a margin): only their max h is read by the reference's search radius, so
+  // only rows in them raise it
+  int scx0, scx1, scy0, scy1;
   // line-of-sight kernel clip: a row counts for the depth cut when its
   // support [z' - h, z' + h] overlaps [0, depth] (instead of its centre lying
   // inside), and the scatter writes its clip parameters
