@@ -50,9 +50,9 @@ FLOP_PER_EVAL_3D = 13
 # HBM bytes per k_gather_batch launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs of this workload scaled by a 1 GiB copy (profiles/k_gather_batch_xray_16M_64views.md);
 # keyed by (workload, mode, tile)
-MEASURED_TRAFFIC = {("c3", "reference", (16, 16, 256)): dict(read=22.61e9, write=3.250e9),
+MEASURED_TRAFFIC = {("c3", "reference", (16, 16, 256)): dict(read=22.63e9, write=3.250e9),
                     # profiles/k_gather_batch_xray_64M_32slabs.md
-                    ("c5", "reference", (16, 16, 256)): dict(read=21.16e9, write=19.59e9),
+                    ("c5", "reference", (16, 16, 256)): dict(read=21.15e9, write=19.59e9),
                     # profiles/k_gather_batch_colden_1M_512_and_xray_16M_2048.md
                     ("c1", "reference", (8, 8, 64)): dict(read=28.80e6, write=10.97e6),
                     ("c2", "reference", (16, 16, 256)): dict(read=252.97e6, write=161.38e6),
