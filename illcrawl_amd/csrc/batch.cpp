@@ -636,7 +636,7 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   const uint64_t ntot = b->sh_ntot, nc = b->sh_ncells;
   const uint32_t n32 = static_cast<uint32_t>(ntot);
   b->sh_offs.ensure(nv * size_t(nt) + 1);
-  b->sh_temp.ensure(std::max({radix_temp_words(ntot), scan_temp_words(nv * uint64_t(nt) + 1),
+  b->sh_temp.ensure(std::max({radix2_temp_words(ntot), scan_temp_words(nv * uint64_t(nt) + 1),
                               scan_temp_words(nc + 1)}));
   for (auto* d : {&b->sh_gkey, &b->sh_orig, &b->sh_key, &b->sh_perm, &b->sh_sk})
     d->ensure(ntot);
@@ -669,18 +669,17 @@ void build_shared(ilc_batch* b, hipStream_t s) {
                               b->sh_sk.p, b->sh_perm.p, b->sh_gkey.p, b->sh_orig.p,
                               b->counter.p + 1, b->sh_cbase.p, b->sh_maxh.p,
                               b->clip ? b->sh_zc.p : nullptr, b->stack, s);
-  // stable sorts by descending h, then by global cell key (view, then cell):
-  // a view's rows end up in one range, by cell, biggest h first, ties in
-  // input order
-  launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, 16, b->sh_temp.p, s);
-  launch_gather_u32(b->sh_gkey.p, b->sh_perm.p, n32, b->sh_sk.p, s);
-  launch_radix_sort(b->sh_sk.p, b->sh_perm.p, n32, key_bits_for(b->sh_ncells - 1), b->sh_temp.p,
+  // stable sorts by descending h (each row's global cell key moving with
+  // it), then by global cell key (view, then cell): a view's rows end up in
+  // one range, by cell, biggest h first, ties in input order
+  launch_radix_sort2(b->sh_sk.p, b->sh_perm.p, b->sh_gkey.p, n32, 16, b->sh_temp.p, s);
+  launch_radix_sort(b->sh_gkey.p, b->sh_perm.p, n32, key_bits_for(b->sh_ncells - 1), b->sh_temp.p,
                     s);
-  // view-local cell keys for the gather; the global ones (sh_sk) give the
+  // view-local cell keys for the gather; the global ones (sh_gkey) give the
   // cell ranges
-  launch_local_keys(b->sh_sk.p, b->sh_cbase.p, static_cast<int>(nv), n32, b->sh_key.p, s);
+  launch_local_keys(b->sh_gkey.p, b->sh_cbase.p, static_cast<int>(nv), n32, b->sh_key.p, s);
   launch_fill_u32(b->sh_counts.p, 0u, nc + 1, s);
-  launch_build_soa(b->sh_p4c.p, nullptr, b->sh_perm.p, b->sh_sk.p, n32, kernel_norm(b->kernel),
+  launch_build_soa(b->sh_p4c.p, nullptr, b->sh_perm.p, b->sh_gkey.p, n32, kernel_norm(b->kernel),
                    b->kernel != ILC_KERNEL_CUBIC,
                    b->sh_part.p, b->sh_hs.p, b->sh_counts.p, b->sh_maxh.p, s,
                    b->clip ? b->sh_zc.p : nullptr, b->clip ? b->sh_zcs.p : nullptr);

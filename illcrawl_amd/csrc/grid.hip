@@ -227,26 +227,32 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restric
 // order, so each digit's items leave as one contiguous run: item i of the
 // ordered tile, of digit d, goes to offs[d, tile] + i - (the tile's items
 // below digit d).
+// V2: a second value array moves with the first.
+template <bool V2>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout,
                                                            uint32_t* __restrict__ vout, uint32_t n,
                                                            int shift, uint32_t ntiles,
-                                                           const uint32_t* __restrict__ offs) {
+                                                           const uint32_t* __restrict__ offs,
+                                                           const uint32_t* __restrict__ vin2,
+                                                           uint32_t* __restrict__ vout2) {
   __shared__ uint32_t sk[kTile];
   __shared__ uint32_t sv[kTile];
+  __shared__ uint32_t sv2[V2 ? kTile : 1];
   __shared__ uint32_t gofs[256];
   __shared__ uint32_t run[256];
   __shared__ uint32_t wc[4][256];
   const uint32_t t = threadIdx.x;
   const int wave = t >> 6;
   const uint32_t tile0 = blockIdx.x * kTile;
-  uint32_t k[kItems], v[kItems];
+  uint32_t k[kItems], v[kItems], v2[V2 ? kItems : 1];
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const uint32_t i = tile0 + r * kBlock + t;
     k[r] = i < n ? kin[i] : 0u;
     v[r] = i < n ? vin[i] : 0u;
+    if constexpr (V2) v2[r] = i < n ? vin2[i] : 0u;
   }
   // the tile's count of digit t from the scanned counts (the last entry's
   // successor is the total, n)
@@ -273,6 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __rest
       for (int w = 0; w < wave; ++w) pos += wc[w][d];
       sk[pos] = k[r];
       sv[pos] = v[r];
+      if constexpr (V2) sv2[pos] = v2[r];
     }
     __syncthreads();
     run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
@@ -285,6 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __rest
     const uint32_t pos = gofs[(key >> shift) & 255u] + i;
     kout[pos] = key;
     vout[pos] = sv[i];
+    if constexpr (V2) vout2[pos] = sv2[i];
   }
 }
 
@@ -598,31 +606,54 @@ uint64_t radix_temp_words(uint64_t n) {
   return 2 * n + 256 * ntiles + scan_temp_words(256 * ntiles);
 }
 
-void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
-                       hipStream_t s) {
+namespace {
+void radix_sort_impl(uint32_t* keys, uint32_t* vals, uint32_t* vals2, uint32_t n, int key_bits,
+                     uint32_t* temp, hipStream_t s) {
   if (n <= 1 || key_bits <= 0) return;
   const uint32_t ntiles = blocks_for(n, kTile);
   uint32_t* k2 = temp;
   uint32_t* v2 = temp + n;
   uint32_t* hist = temp + 2 * uint64_t(n);
   uint32_t* stemp = hist + 256 * uint64_t(ntiles);
-  uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
+  uint32_t* w2 = temp + radix_temp_words(n);  // the second values' other half (vals2 only)
+  uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2, *wa = vals2, *wb = w2;
   int passes = (key_bits + 7) / 8;
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
     hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, ka, n, shift, ntiles, hist);
     ILC_HIP(hipGetLastError());
     launch_exclusive_scan(hist, hist, uint64_t(256) * ntiles, stemp, s);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, ka, va, kb, vb, n, shift,
-                       ntiles, hist);
+    if (vals2)
+      hipLaunchKernelGGL(k_radix_scatter<true>, dim3(ntiles), dim3(kBlock), 0, s, ka, va, kb, vb, n,
+                         shift, ntiles, hist, wa, wb);
+    else
+      hipLaunchKernelGGL(k_radix_scatter<false>, dim3(ntiles), dim3(kBlock), 0, s, ka, va, kb, vb,
+                         n, shift, ntiles, hist, nullptr, nullptr);
     ILC_HIP(hipGetLastError());
     std::swap(ka, kb);
     std::swap(va, vb);
+    std::swap(wa, wb);
   }
   if (ka != keys) {
     ILC_HIP(hipMemcpyAsync(keys, ka, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
     ILC_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    if (vals2)
+      ILC_HIP(hipMemcpyAsync(vals2, wa, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
   }
+}
+}  // namespace
+
+void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
+                       hipStream_t s) {
+  radix_sort_impl(keys, vals, nullptr, n, key_bits, temp, s);
+}
+
+uint64_t radix2_temp_words(uint64_t n) { return radix_temp_words(n) + n; }
+
+void launch_radix_sort2(uint32_t* keys, uint32_t* vals, uint32_t* vals2, uint32_t n, int key_bits,
+                        uint32_t* temp, hipStream_t s) {
+  ILC_REQUIRE(vals2 != nullptr, "second value array");
+  radix_sort_impl(keys, vals, vals2, n, key_bits, temp, s);
 }
 
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,

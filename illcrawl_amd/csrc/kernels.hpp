@@ -25,6 +25,11 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32
 uint64_t radix_temp_words(uint64_t n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, uint32_t* temp,
                        hipStream_t s);
+// The same sort with a second value array moved alongside; temp holds
+// radix2_temp_words(n).
+uint64_t radix2_temp_words(uint64_t n);
+void launch_radix_sort2(uint32_t* keys, uint32_t* vals, uint32_t* vals2, uint32_t n, int key_bits,
+                        uint32_t* temp, hipStream_t s);
 // coef = w norm / h^2 (pow04: coef^0.4, the projected quartic's form); h
 // NULL: each particle's h is the z of its p4
 void launch_build_soa(const float4* p4, const float* h, const uint32_t* perm,
