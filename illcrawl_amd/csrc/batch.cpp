@@ -658,8 +658,13 @@ void build_shared(ilc_batch* b, hipStream_t s) {
     launch_particle_xray(b->ps, gaunt_of(b), b->sh_wx.p, s);
   }
   // per-cell max h starts at zero: the scatter raises it for the rows its
-  // screen test leaves out, the SoA pass for the kept rows
-  launch_fill_u32(b->sh_maxh.p, 0u, nc, s);
+  // screen test leaves out, the SoA pass for the kept rows.  A batch of
+  // depth-compacted views only (slabs of groups) reads every max h from its
+  // groups' full grids and needs none of it.
+  bool all_compact = true;
+  for (auto& v : b->views) all_compact = all_compact && v->compact;
+  uint32_t* const maxh = all_compact ? nullptr : b->sh_maxh.p;
+  if (maxh) launch_fill_u32(maxh, 0u, nc, s);
   if (b->clip) {
     b->sh_zc.ensure(ntot);
     b->sh_zcs.ensure(ntot);
@@ -667,7 +672,7 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   launch_multi_window_scatter(b->ps, any_xray ? b->sh_wx.p : nullptr, b->bdesc_dev.p,
                               b->fviews_dev.p, static_cast<int>(nv), b->sh_offs.p, b->sh_p4c.p,
                               b->sh_sk.p, b->sh_perm.p, b->sh_gkey.p, b->sh_orig.p,
-                              b->counter.p + 1, b->sh_cbase.p, b->sh_maxh.p,
+                              b->counter.p + 1, b->sh_cbase.p, maxh,
                               b->clip ? b->sh_zc.p : nullptr, b->stack, s);
   // stable sorts by descending h (each row's global cell key moving with
   // it), then by global cell key (view, then cell): a view's rows end up in
@@ -681,7 +686,7 @@ void build_shared(ilc_batch* b, hipStream_t s) {
   launch_fill_u32(b->sh_counts.p, 0u, nc + 1, s);
   launch_build_soa(b->sh_p4c.p, nullptr, b->sh_perm.p, b->sh_gkey.p, n32, kernel_norm(b->kernel),
                    b->kernel != ILC_KERNEL_CUBIC,
-                   b->sh_part.p, b->sh_hs.p, b->sh_counts.p, b->sh_maxh.p, s,
+                   b->sh_part.p, b->sh_hs.p, b->sh_counts.p, maxh, s,
                    b->clip ? b->sh_zc.p : nullptr, b->clip ? b->sh_zcs.p : nullptr);
   launch_exclusive_scan(b->sh_counts.p, b->sh_start.p, nc + 1, b->sh_temp.p, s);
   for (size_t v = 0; v < nv; ++v) {

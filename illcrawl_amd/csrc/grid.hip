@@ -407,7 +407,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
 // particle_grid.cl:114-216): keys are sorted, so the particles of a cell form
 // one run; each wave reduces its part of every run (count, max h) and one
 // lane per run segment adds it with an atomic (a cell spans at most a few
-// waves, so there is no contention).  count and maxh_bits start at 0.
+// waves, so there is no contention).  count and maxh_bits start at 0;
+// maxh_bits may be null (no max h wanted).
 __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__ p4,
                                                        const float* __restrict__ h,
                                                        const uint32_t* __restrict__ perm,
@@ -450,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_build_soa(const float4* __restrict__
   }
   if (valid && head) {
     atomicAdd(&count[key], run_end - lane);
-    atomicMax(&maxh_bits[key], __float_as_uint(m));
+    if (maxh_bits) atomicMax(&maxh_bits[key], __float_as_uint(m));
   }
 }
 
