@@ -274,6 +274,14 @@ __global__ __launch_bounds__(kBlock) void k_multi_bbox(particle_soa ps,
       return;
     }
   }
+  // a weight's finiteness decides a particle's validity only inside a view's
+  // depth range: with every weight of the tile finite, the depth coordinate
+  // is not needed
+  int wnan = 0;
+#pragma unroll
+  for (int r = 0; r < kMultiItems; ++r)
+    wnan |= (in[r] && ((any_mass && !isfinite(wm[r])) || (any_xray && !isfinite(wx[r])))) ? 1 : 0;
+  const bool wfinite = !__syncthreads_or(wnan);
   for (int v = 0; v < nv; ++v) {
     const view_proj vp = vd[v].vp;
     float mn0 = INFINITY, mn1 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY, mh = 0.0f;
@@ -284,10 +292,14 @@ __global__ __launch_bounds__(kBlock) void k_multi_bbox(particle_soa ps,
       const float vx = px[r] - vp.c[0], vy = py[r] - vp.c[1], vz = pz[r] - vp.c[2];
       const float ax = dot3(vp.m + 0, vx, vy, vz);
       const float ay = dot3(vp.m + 3, vx, vy, vz);
-      const float az = dot3(vp.m + 6, vx, vy, vz);
-      float w = vp.quantity == 1 ? wx[r] : wm[r];
-      if (az < 0.0f || az > vp.depth) w = 0.0f;
-      const bool ok = isfinite(ax) && isfinite(ay) && isfinite(w) && hh[r] >= 1.1e-19f &&
+      bool wok = true;
+      if (!wfinite) {
+        const float az = dot3(vp.m + 6, vx, vy, vz);
+        float w = vp.quantity == 1 ? wx[r] : wm[r];
+        if (az < 0.0f || az > vp.depth) w = 0.0f;
+        wok = isfinite(w);
+      }
+      const bool ok = isfinite(ax) && isfinite(ay) && wok && hh[r] >= 1.1e-19f &&
                       hh[r] <= 1.8e19f;
       if (!ok) {
         ++nbad;
